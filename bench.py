@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark of the aggregation-AMG preconditioned CG solve on one MI355X.
+
+Workload: 3D 7-point steady heat conduction on a 128^3 cell grid
+(2,097,152 unknowns, 14,581,760 nonzeros), log-normal cell conductivity
+(mu 0, sigma 1, seed 1), harmonic face averages, Dirichlet T=0 / T=1 on the
+x faces.  One step = AMG setup + PCG solve (V(2,2) Jacobi, omega 2/3,
+coarsest <= 256 rows by dense LU) to a 1e-8 FP64 relative residual, with the
+matrix and right-hand side already resident in HBM.
+
+  chosen: FP32 V-cycle (levels stored in FP32) under FP64 CG, FP64 coarse LU
+  plain:  the same method with the cycle in FP64
+
+value = seconds per chosen setup+solve (lower is better); the plain time and
+the ratio are reported beside it.  roofline = the level-0 FP64 CSR SpMV
+(216,924,164 algorithmic bytes per launch at 128^3) timed with HIP events on
+the library stream.  cpu_baseline = the CPU oracle (FP64, OpenMP) on a 64^3
+sample of the same problem.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "setup+solve s to 1e-8 FP64 rel. residual at 128³, chosen vs plain; SpMV GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nx", type=int, default=128)
+    ap.add_argument("--cpu-nx", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver runs on a single "
+              "MI355X and multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from thermofem_amd import problems as P
+    from thermofem_amd._lib import check, lib
+    from thermofem_amd.device import Context, DeviceArray, DeviceCsr
+    from thermofem_amd.solver import AmgOptions, Cg, KrylovOptions, SolveStatus
+
+    nx = args.nx
+    log(f"generating heat3d log-normal {nx}^3")
+    (rp, ci, v), b = P.heat3d_lognormal(nx, seed=1)
+    n, nnz = len(rp) - 1, len(ci)
+    A = DeviceCsr(rp, ci, v)
+    db = DeviceArray.from_host(b)
+    dx = DeviceArray(n, np.float64)
+    zeros = np.zeros(n)
+
+    def make(precision):
+        amg = AmgOptions(coarse_rows=256, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0,
+                         strength_threshold=0.25, matching_rounds=8, level_precision=precision)
+        return Cg(KrylovOptions(relative_tolerance=1e-8, use_graph=True), amg)
+
+    def step(solver):
+        dx.upload(zeros)
+        solver.setup_device(n, nnz, *A.args())
+        return solver.apply_device(db, dx)
+
+    results = {}
+    for name, prec in (("chosen", 32), ("plain", 64)):
+        solver = make(prec)
+        for w in range(args.warmup):
+            log(f"{name}: warmup {w}")
+            step(solver)
+        times, setups, solves = [], [], []
+        res = None
+        for k in range(args.steps):
+            t0 = time.perf_counter()
+            res = step(solver)
+            t1 = time.perf_counter()
+            s_setup, s_apply = solver.timings()
+            times.append(t1 - t0)
+            setups.append(s_setup)
+            solves.append(s_apply)
+            log(f"{name}: step {k} {1e3 * (t1 - t0):.2f} ms (setup {1e3 * s_setup:.2f}, solve {1e3 * s_apply:.2f}) "
+                f"it={res.iterations} rel={res.relative_residual:.3e} {res.status.name}")
+            if res.status != SolveStatus.Converged:
+                raise SystemExit(f"{name} solve did not converge: {res}")
+        view = solver.amg()
+        results[name] = dict(total=sum(times), mean=statistics.fmean(times), median=statistics.median(times),
+                             setup=statistics.median(setups), solve=statistics.median(solves), res=res,
+                             x=dx.get(), levels=view.level_count(), stats=view.level_statistics())
+        del solver
+
+    ch, pl = results["chosen"], results["plain"]
+    err = float(np.linalg.norm(ch["x"] - pl["x"]) / np.linalg.norm(pl["x"]))
+
+    # Level-0 FP64 SpMV on the library stream, HIP events around the launches.
+    ctx = Context(0)
+    y = DeviceArray(n, np.float64)
+    xv = DeviceArray.from_host(P.uniform(11, n))
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    check(lib().tf_event_create(C.byref(ev0)))
+    check(lib().tf_event_create(C.byref(ev1)))
+
+    def time_spmv(fn, vals, xx, reps):
+        for _ in range(3):
+            check(fn(ctx.h, n, nnz, A.rp.ptr, A.ci.ptr, vals.ptr, xx.ptr, y.ptr))
+        check(lib().tf_event_record(ctx.h, ev0))
+        for _ in range(reps):
+            check(fn(ctx.h, n, nnz, A.rp.ptr, A.ci.ptr, vals.ptr, xx.ptr, y.ptr))
+        check(lib().tf_event_record(ctx.h, ev1))
+        ms = C.c_float()
+        check(lib().tf_event_elapsed_ms(ev0, ev1, C.byref(ms)))
+        return ms.value / reps * 1e-3
+
+    t64 = time_spmv(lib().tf_spmv_csr_f64, A.v, xv, args.spmv_reps)
+    v32 = DeviceArray.from_host(v.astype(np.float32))
+    x32 = DeviceArray.from_host(P.uniform(11, n).astype(np.float32))
+    y = DeviceArray(n, np.float32)
+    t32 = time_spmv(lib().tf_spmv_csr_f32, v32, x32, args.spmv_reps)
+    bytes64 = P.spmv_bytes(n, nnz, 8)
+    bytes32 = P.spmv_bytes(n, nnz, 4)
+    gbs64 = bytes64 / t64 / 1e9
+    gbs32 = bytes32 / t32 / 1e9
+    log(f"spmv f64 {t64 * 1e6:.1f} us {gbs64:.0f} GB/s; f32 {t32 * 1e6:.1f} us {gbs32:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+        cnx = args.cpu_nx
+        log(f"cpu baseline: oracle FP64 AMG-PCG at {cnx}^3")
+        (crp, cci, cv), cb = P.heat3d_lognormal(cnx, seed=1)
+        t0 = time.perf_counter()
+        host = O.Amg(coarse_rows=256, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0).setup((crp, cci, cv))
+        _, st, it, _, rel = O.krylov("cg", host, (crp, cci, cv), cb)
+        tcpu = time.perf_counter() - t0
+        cpu = {"value": round(tcpu, 4), "unit": "s", "cores": int(O.lib().tfo_threads()), "kind": "port",
+               "sample": f"{cnx}^3 ({cnx ** 3} unknowns) instance of the same log-normal heat problem, FP64 "
+                         f"AMG setup + PCG solve to 1e-8 (status {st}, {it} iterations, rel {rel:.2e}); the full "
+                         f"{nx}^3 workload is {(nx // cnx) ** 3}x larger"}
+        log(f"cpu baseline {tcpu:.2f} s, {it} iterations")
+
+    out = {
+        "metric": METRIC,
+        "value": round(ch["median"], 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ch["total"] / args.steps * 1e3, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"heat3d log-normal {nx}^3 AMG-PCG setup+solve to 1e-8 (FP32 V(2,2) cycle, FP64 CG)",
+            "n": n, "nnz": nnz, "levels": ch["levels"], "coarse_rows": 256,
+        },
+        "plain_fp64_s": round(pl["median"], 6),
+        "speedup_vs_plain": round(pl["median"] / ch["median"], 4),
+        "chosen": {"setup_s": round(ch["setup"], 6), "solve_s": round(ch["solve"], 6),
+                   "iterations": ch["res"].iterations, "relative_residual": ch["res"].relative_residual},
+        "plain": {"setup_s": round(pl["setup"], 6), "solve_s": round(pl["solve"], 6),
+                  "iterations": pl["res"].iterations, "relative_residual": pl["res"].relative_residual},
+        "rel_error_chosen_vs_plain_fp64": err,
+        "spmv": {"f64_us": round(t64 * 1e6, 2), "f64_GBs": round(gbs64, 1), "f32_us": round(t32 * 1e6, 2),
+                 "f32_GBs": round(gbs32, 1), "bytes_f64": bytes64, "bytes_f32": bytes32},
+        "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_csr_rows<8,double,0> level-0 FP64 SpMV"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

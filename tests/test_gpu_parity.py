@@ -1,0 +1,288 @@
+"""GPU parity: every setup stage, transfer, kernel and solve of the HIP
+library against the CPU oracle on the same seeded inputs.
+
+Structure (CSR offsets/columns, maps, members, histograms) and Galerkin /
+strength values are compared bit-exactly: the library merges duplicates in
+original entry order, the order the reference host oracle defines
+(test/support/NativeAggregationOracle.cu:43-75).  Floating-point kernels
+whose summation order differs from the oracle's sequential row sums (SpMV,
+Jacobi, cycles, Krylov) are compared within the tolerances stated per test.
+"""
+import numpy as np
+import pytest
+
+import _gpu as G
+import _inputs as I
+import _oracle as O
+from thermofem_amd import problems as P
+from thermofem_amd.solver import (Amg, AmgOptions, BiCgStab, Cg, Cycle, KrylovOptions, SolveStatus)
+
+pytestmark = pytest.mark.gpu
+
+SAMPLE_NS = [1, 2, 4, 17, 61, 128]
+
+
+def same(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("n", SAMPLE_NS)
+def test_stages_match_oracle_bit_exact(n):
+    """amg_native_test.cpp:66-114 sweep: canonical, strength, aggregate, Galerkin, transfers."""
+    host = I.sample(n)
+    log(f"stages n={n}")
+    a = O.canonical(*host)
+    assert same(G.canonical(host), a)
+    assert same(G.canonical(I.shuffled_with_duplicates(*host)), O.canonical(*I.shuffled_with_duplicates(*host)))
+    exercised = False
+    for theta in (0.0, 0.25, 1.0):
+        g = O.strength(*a, theta)
+        assert same(G.strength(a, theta), g), (n, theta)
+        for rounds in (1, 8):
+            for absorb in (False, True):
+                expected = O.aggregate(g, rounds, absorb)
+                got = G.aggregate(g, rounds, absorb)
+                assert same(got, expected), (n, theta, rounds, absorb)
+                if absorb:
+                    exercised |= len(O.aggregate(g, rounds, False)[1]) != len(expected[1])
+                coarse = G.galerkin(a, got)
+                assert same(coarse, O.galerkin(a, expected[0])), (n, theta, rounds, absorb)
+                assert len(coarse[1]) <= len(a[1])
+                nc = len(expected[1]) - 1
+                sums = G.restrict(got, np.ones(n))
+                assert np.array_equal(sums, np.diff(expected[1]).astype(np.float64))
+                fine = G.prolong(got, sums, np.ones(n))
+                assert np.array_equal(fine, 1 + sums[expected[0]])
+                assert nc >= 1
+    if n == 128:
+        assert exercised
+
+
+def test_canonical_duplicates_kat():
+    """amg_native_test.cpp:115-121: duplicates summed to {2,-1,-1,2}."""
+    a = (np.array([0, 3, 6], np.int32), np.array([1, 0, 0, 1, 0, 1], np.int32),
+         np.array([-1.0, 1, 1, 1, -1, 1]))
+    rp, ci, v = G.canonical(a)
+    assert list(v) == [2, -1, -1, 2] and list(ci) == [0, 1, 0, 1]
+
+
+def test_four_point_galerkin_kat():
+    """multigrid_operations_test.cpp:38-44: 1D Poisson(4) aggregates to one row, Ac = [2]."""
+    a = P.poisson1d(4)
+    g = G.strength(a, 0.25)
+    t = G.aggregate(g, 8, True)
+    c = G.galerkin(a, t)
+    assert len(c[0]) == 2 and list(c[2]) == [2.0]
+
+
+@pytest.mark.parametrize("scale", [1e-100, 1.0, 1e100])
+def test_coarse_lu_scale_invariance(scale):
+    """amg_native_test.cpp:133-153: A = s[1 2;4 3], b = s[5,10] -> x = [1,2]."""
+    import ctypes as C
+    from thermofem_amd._lib import check, lib
+    a = (np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), scale * np.array([1.0, 2, 4, 3]))
+    ok, lu, piv, inv = G.lu_factor(a)
+    assert ok
+    b = G.up(scale * np.array([5.0, 10.0]))
+    x = G.DeviceArray(2, np.float64)
+    check(lib().tf_coarse_lu_solve_f64(G.ctx().h, 2, lu.ptr, piv.ptr, b.ptr, x.ptr))
+    G.ctx().sync()
+    assert np.allclose(x.get(), [1, 2], atol=1e-12)
+    check(lib().tf_coarse_lu_solve_f64(G.ctx().h, 2, lu.ptr, piv.ptr, b.ptr, b.ptr))  # aliasing
+    G.ctx().sync()
+    assert np.array_equal(b.get(), x.get())
+    y = G.DeviceArray(2, np.float64)
+    check(lib().tf_coarse_inv_apply_f64(G.ctx().h, 2, inv.ptr, G.up(scale * np.array([5.0, 10.0])).ptr, y.ptr))
+    G.ctx().sync()
+    assert np.allclose(y.get(), [1, 2], atol=1e-12)
+
+
+def test_coarse_lu_singular():
+    a = (np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), np.array([1.0, 1, 1, 1]))
+    ok, *_ = G.lu_factor(a)
+    assert not ok
+
+
+def test_coarse_lu_256_matches_oracle():
+    """Dense LU at the widened coarse size (n=256) against the oracle's factors and solve."""
+    a = P.poisson2d(16)  # 256 rows
+    ok, lu, piv, inv = G.lu_factor(a)
+    ok_h, lu_h, piv_h = O.lu_factor(a)
+    assert ok and ok_h
+    assert np.array_equal(piv.get(), piv_h)
+    assert np.allclose(lu.get(), lu_h, rtol=1e-12, atol=1e-14)
+    b = np.linspace(-1, 2, 256)
+    import ctypes as C
+    from thermofem_amd._lib import check, lib
+    x = G.DeviceArray(256, np.float64)
+    check(lib().tf_coarse_inv_apply_f64(G.ctx().h, 256, inv.ptr, G.up(b).ptr, x.ptr))
+    G.ctx().sync()
+    xh = O.lu_solve(lu_h, piv_h, b)
+    assert np.allclose(x.get(), xh, rtol=1e-11, atol=1e-12)
+
+
+def _spmv_tol(a, x, dtype):
+    # |y_gpu - y_seq| <= k * eps * sum_j |a_ij x_j| (different summation order)
+    absrow = O.spmv((a[0], a[1], np.abs(a[2])), np.abs(x))
+    eps = np.finfo(dtype).eps
+    k = np.diff(a[0]).max() + 2
+    return k * eps * absrow + np.finfo(dtype).tiny
+
+
+@pytest.mark.parametrize("which", ["c1", "c2"])
+def test_spmv_matches_oracle(which):
+    if which == "c1":
+        a = P.poisson2d(256)
+    else:
+        a, _ = P.heat3d_lognormal(128, seed=1)
+    n = len(a[0]) - 1
+    x = P.uniform(11, n) - 0.5
+    log(f"spmv {which} n={n}")
+    y = G.spmv(a, x)
+    assert np.all(np.abs(y - O.spmv(a, x)) <= _spmv_tol(a, x, np.float64))
+    y32 = G.spmv(a, x, np.float32)
+    y32h = O.spmv_f32(a, x)
+    assert np.all(np.abs(y32 - y32h) <= _spmv_tol((a[0], a[1], a[2].astype(np.float32)), x.astype(np.float32),
+                                                   np.float32) * 2)
+
+
+def test_c1_hierarchy_bit_exact():
+    """Config 1: 256^2 5-point, theta .25, rounds 8, absorb, to <= 256 rows: every level's
+    aggregates and coarse CSR identical to the oracle's."""
+    a = P.poisson2d(256)
+    opts = dict(coarse_rows=256, jacobi_weight=2 / 3)
+    host = O.Amg(**opts).setup(a)
+    dev = Amg(AmgOptions(**opts)).setup(a)
+    assert dev.level_count() == host.levels()
+    for d in range(host.levels()):
+        log(f"c1 level {d}")
+        assert same(dev.level_matrix(d), host.level_matrix(d)), d
+        if d + 1 < host.levels():
+            assert same(dev.level_transfer(d), host.level_transfer(d)), d
+        r, z, h = dev.level_statistics()[d]
+        hr, hz, hh = host.level_info(d)
+        assert (r, z, list(h)) == (hr, hz, list(hh))
+    assert dev.stop_reason() == host.stop_reason() == "coarse size"
+    assert dev.coarse_solver() == host.coarse_solver() == "pivoted GPU LU"
+
+
+@pytest.mark.parametrize("precision,tol", [(64, 2e-10), (32, 2e-3)])
+@pytest.mark.parametrize("cycle", [Cycle.V, Cycle.W, Cycle.F])
+def test_cycles_match_oracle(cycle, precision, tol):
+    """amg_native_test.cpp:324-395: V/W/F on 1D Poisson(127), coarse <= 4, against the host cycle;
+    visit counts; linearity; aliasing is exercised by the oracle's own apply."""
+    a = P.poisson1d(127)
+    i = np.arange(127)
+    b = np.sin(0.13 * i) + 0.4
+    c = np.cos(0.07 * i)
+    if precision == 32:
+        b = b.astype(np.float32).astype(np.float64)
+        c = c.astype(np.float32).astype(np.float64)
+    opts = dict(coarse_rows=4, cycle=int(cycle), level_precision=precision)
+    host = O.Amg(**opts).setup(a)
+    dev = Amg(AmgOptions(coarse_rows=4, cycle=cycle, level_precision=precision)).setup(a)
+    assert dev.level_count() >= 3
+    x = dev.apply(b)
+    assert np.allclose(x, host.apply(b), atol=tol, rtol=0)
+    visits = dev.last_cycle_visits()
+    L = len(visits)
+    if cycle == Cycle.V:
+        assert visits == [1] * L
+    elif cycle == Cycle.W:
+        assert visits == [2 ** d for d in range(L)]
+    else:
+        assert visits == [L if d + 1 == L else d + 2 for d in range(L)]
+    xc = dev.apply(c)
+    xs = dev.apply(b + c)
+    assert np.allclose(xs, x + xc, atol=tol, rtol=0)
+
+
+def test_fallback_matches_fixed_jacobi():
+    """amg_native_test.cpp:398-412: max_levels 1 on Poisson(100) -> Jacobi(8, .75) exactly as the oracle's."""
+    a = P.poisson1d(100)
+    b = np.ones(100)
+    dev = Amg(AmgOptions(max_levels=1)).setup(a)
+    assert dev.coarse_solver() == "Jacobi (coarsening stopped)"
+    x = dev.apply(b)
+    xh = O.jacobi(a, 8, 0.75, b, np.zeros(100))
+    assert np.allclose(x, xh, rtol=0, atol=1e-13)
+    for cyc in (Cycle.V, Cycle.W, Cycle.F):
+        amg = Amg(AmgOptions(coarse_rows=4, pre_sweeps=0, post_sweeps=0, cycle=cyc)).setup(a)
+        assert np.array_equal(amg.apply(b), amg.apply(b))
+
+
+def test_singular_coarse_reports_jacobi():
+    """solver_test.cpp:166-169: singular 2x2 -> "Jacobi (singular LU)"."""
+    a = (np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), np.array([1.0, 1, 1, 1]))
+    amg = Amg(AmgOptions()).setup(a)
+    assert amg.coarse_solver() == "Jacobi (singular LU)"
+
+
+def test_zero_diagonal_throws():
+    a = (np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), np.array([0.0, 1, 1, 1]))
+    with pytest.raises(ValueError):
+        Amg(AmgOptions()).setup(a)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_cg_amg_heat3d_converges(precision):
+    """AMG-PCG (V(2,2), omega 2/3) on the log-normal heat problem at 32^3 reaches 1e-8 in
+    FP64 recomputed residual; the solution agrees with the oracle's FP64 solve."""
+    a, b = P.heat3d_lognormal(32, seed=1)
+    amg_opts = AmgOptions(coarse_rows=256, pre_sweeps=2, post_sweeps=2, jacobi_weight=2 / 3,
+                          level_precision=precision)
+    s = Cg(KrylovOptions(relative_tolerance=1e-8), amg_opts).setup(a)
+    x, res = s.apply(b)
+    log(f"cg prec={precision} it={res.iterations} rel={res.relative_residual}")
+    assert res.status == SolveStatus.Converged
+    r = b - O.spmv(a, x)
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b) * 1.0001
+    host = O.Amg(coarse_rows=256, pre_sweeps=2, post_sweeps=2, jacobi_weight=2 / 3).setup(a)
+    xh, st, it, _, _ = O.krylov("cg", host, a, b)
+    assert st == 0
+    assert abs(it - res.iterations) <= 2 + (3 if precision == 32 else 0)
+    assert np.linalg.norm(x - xh) <= 1e-6 * np.linalg.norm(xh)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_bicgstab_amg_c1_converges(graph):
+    """Config 1: BiCGStab + AMG V(1,1), omega 2/3, to 1e-8."""
+    a, b = P.poisson2d_config(256, seed=0)
+    s = BiCgStab(KrylovOptions(relative_tolerance=1e-8, use_graph=graph),
+                 AmgOptions(coarse_rows=256, jacobi_weight=2 / 3)).setup(a)
+    x, res = s.apply(b)
+    log(f"bicgstab it={res.iterations} rel={res.relative_residual}")
+    assert res.status == SolveStatus.Converged
+    assert np.linalg.norm(b - O.spmv(a, x)) <= 1.0001e-8 * np.linalg.norm(b)
+    host = O.Amg(coarse_rows=256, jacobi_weight=2 / 3).setup(a)
+    xh, st, it, _, _ = O.krylov("bicgstab", host, a, b)
+    assert st == 0 and abs(it - res.iterations) <= 2
+
+
+def test_krylov_failure_paths():
+    """solver_test.cpp:255-287 / cg_test.cpp:157-170: MaxIterations, Breakdown, NonFinite."""
+    a = P.poisson1d(8)
+    b = np.ones(8)
+    _, r = Cg(KrylovOptions(max_iterations=0)).setup(a).apply(b)
+    assert r.status == SolveStatus.MaxIterations and r.iterations == 0
+    z = (np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.array([0.0, 0.0]))
+    _, r = Cg(KrylovOptions(), None).setup(z).apply(np.array([1.0, 1.0]))
+    assert r.status == SolveStatus.Breakdown
+    bad = np.ones(8)
+    bad[3] = np.nan
+    _, r = Cg(KrylovOptions()).setup(a).apply(bad)
+    assert r.status == SolveStatus.NonFinite
+
+
+def test_cg_closed_form_poisson():
+    """cg_test.cpp:66: 1D Poisson, b = 1 -> x_i = (i+1)(n-i)/2 (identity preconditioner)."""
+    n = 63
+    a = P.poisson1d(n)
+    x, r = Cg(KrylovOptions(relative_tolerance=1e-12)).setup(a).apply(np.ones(n))
+    i = np.arange(n)
+    assert r.status == SolveStatus.Converged
+    assert np.allclose(x, (i + 1) * (n - i) / 2, rtol=1e-9)

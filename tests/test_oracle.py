@@ -1,0 +1,239 @@
+"""Pins of the CPU oracle (oracle/tf_oracle.cpp).
+
+1. Against the reference's own host oracle, compiled from the reference
+   sources into oracle/_ref (test/support/NativeAggregationOracle.cu), on the
+   reference test sweep and on larger random / structured inputs.
+2. Against the committed golden fixtures written by that build
+   (tests/golden/make_golden.py), so the pin holds where the reference tree
+   is absent.
+3. Against closed forms and the reference tests' known answers.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import _inputs as I
+import _oracle as O
+from thermofem_amd import problems as P
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "aggregation_golden.json")
+
+
+def same(a, b):
+    return all(np.array_equal(np.asarray(x), np.asarray(y)) for x, y in zip(a, b))
+
+
+def test_mt19937_matches_standard():
+    """The C++ standard fixes the 10000th output of a default-seeded mt19937 at 4123659995."""
+    r = I.MT19937()
+    for _ in range(9999):
+        r()
+    assert r() == 4123659995
+
+
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="reference-built checker not present")
+
+
+@needs_ref
+@pytest.mark.parametrize("n", [1, 2, 4, 17, 61, 128])
+def test_oracle_matches_reference_host_oracle(n):
+    a = I.sample(n)
+    assert same(O.canonical(*a), O.ref_canonical(*a))
+    dup = I.shuffled_with_duplicates(*a)
+    assert same(O.canonical(*dup), O.ref_canonical(*dup))
+    for theta in (0.0, 0.25, 1.0):
+        g = O.strength(*a, theta)
+        assert same(g, O.ref_strength(*a, theta))
+        for rounds in (1, 8):
+            for absorb in (False, True):
+                t = O.aggregate(g, rounds, absorb)
+                assert same(t, O.ref_aggregate(g, rounds, absorb))
+                assert same(O.galerkin(a, t[0]), O.ref_galerkin(a, t))
+
+
+@needs_ref
+def test_oracle_matches_reference_on_structured_and_random():
+    cases = [P.poisson2d(40), P.heat3d_lognormal(10, seed=5)[0]]
+    rng = np.random.default_rng(3)
+    for n in (300, 1000):
+        # random nonsymmetric sparse rows with ties in weight
+        rows = []
+        for i in range(n):
+            k = rng.integers(1, 9)
+            cols = np.unique(np.concatenate([[i], rng.integers(0, n, k)]))
+            vals = -rng.integers(1, 4, len(cols)).astype(float)
+            vals[cols == i] = 20.0
+            rows.append((cols, vals))
+        rp = np.zeros(n + 1, np.int32)
+        rp[1:] = np.cumsum([len(c) for c, _ in rows])
+        cases.append((rp, np.concatenate([c for c, _ in rows]).astype(np.int32),
+                      np.concatenate([v for _, v in rows])))
+    for a in cases:
+        a = O.canonical(*a)
+        for theta in (0.0, 0.25):
+            g = O.strength(*a, theta)
+            assert same(g, O.ref_strength(*a, theta))
+            t = O.aggregate(g, 8, True)
+            assert same(t, O.ref_aggregate(g, 8, True))
+            assert same(O.galerkin(a, t[0]), O.ref_galerkin(a, t))
+
+
+def test_oracle_matches_golden_fixtures():
+    data = json.load(open(GOLDEN))
+    assert len(data["cases"]) >= 30
+    for case in data["cases"]:
+        a = I.sample(case["n"]) if case["input"] == "sample" else P.poisson2d(case["n"])
+        a = O.canonical(*a)
+        g = O.strength(*a, case["theta"])
+        assert [list(map(int, g[0])), list(map(int, g[1]))] == [case["g_rp"], case["g_ci"]]
+        assert np.array_equal(g[2], np.array(case["g_w"]))
+        t = O.aggregate(g, case["rounds"], case["absorb"])
+        assert list(map(int, t[0])) == case["map"]
+        assert list(map(int, t[1])) == case["offsets"]
+        assert list(map(int, t[2])) == case["members"]
+        assert list(map(int, t[3])) == case["histogram"]
+        c = O.galerkin(a, t[0])
+        assert list(map(int, c[0])) == case["c_rp"] and list(map(int, c[1])) == case["c_ci"]
+        assert np.array_equal(c[2], np.array(case["c_v"]))
+
+
+def test_known_answers():
+    # canonical duplicates (amg_native_test.cpp:115-121)
+    a = (np.array([0, 3, 6], np.int32), np.array([1, 0, 0, 1, 0, 1], np.int32), np.array([-1.0, 1, 1, 1, -1, 1]))
+    assert list(O.canonical(*a)[2]) == [2, -1, -1, 2]
+    # 4-point Poisson aggregates to one row with Ac = [2] (multigrid_operations_test.cpp:38-44)
+    p4 = P.poisson1d(4)
+    t = O.aggregate(O.strength(*p4, 0.25), 8, True)
+    assert list(O.galerkin(p4, t[0])[2]) == [2.0]
+    # residual {-1,0,0,-1} for x = 1, b = 0 (multigrid_operations_test.cpp:47-50)
+    assert list(-O.spmv(p4, np.ones(4))) == [-1, 0, 0, -1]
+    # Jacobi KAT: poisson(2), omega 1, b = {1,0} -> {0.5, 0.25} (solver_test.cpp:56-74)
+    p2 = P.poisson1d(2)
+    assert list(O.jacobi(p2, 1, 1.0, np.array([1.0, 0.0]), np.zeros(2))) == [0.5, 0.0]
+    assert list(O.jacobi(p2, 2, 1.0, np.array([1.0, 0.0]), np.zeros(2))) == [0.5, 0.25]
+
+
+@pytest.mark.parametrize("scale", [1e-100, 1.0, 1e100])
+def test_lu_scale_invariance(scale):
+    a = (np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), scale * np.array([1.0, 2, 4, 3]))
+    ok, lu, piv = O.lu_factor(a)
+    assert ok and list(piv) == [1, 1]
+    assert np.allclose(O.lu_solve(lu, piv, scale * np.array([5.0, 10.0])), [1, 2], atol=1e-12)
+    assert not O.lu_factor((a[0], a[1], np.array([1.0, 1, 1, 1])))[0]
+
+
+def test_lu_matches_numpy_solve():
+    a = P.poisson2d(9)
+    a = (a[0], a[1], a[2] + 0.1 * np.sin(np.arange(len(a[2]))))  # nonsymmetric
+    n = len(a[0]) - 1
+    dense = np.zeros((n, n))
+    for i in range(n):
+        dense[i, a[1][a[0][i]:a[0][i + 1]]] = a[2][a[0][i]:a[0][i + 1]]
+    ok, lu, piv = O.lu_factor(a)
+    b = np.linspace(-1, 1, n)
+    assert ok and np.allclose(O.lu_solve(lu, piv, b), np.linalg.solve(dense, b), rtol=1e-12)
+
+
+def _host_cycle(a, kind, b):
+    """Independent restatement of HostCycle (amg_native_test.cpp:157-300) in numpy:
+    omega .75, one sweep each side, exact solve at <= 4 rows."""
+    mats, maps = [], []
+    cur = a
+    while True:
+        mats.append(cur)
+        if len(cur[0]) - 1 <= 4:
+            break
+        t = O.aggregate(O.strength(*cur, 0.25), 8, True)
+        maps.append(t[0])
+        cur = O.galerkin(cur, t[0])
+
+    def dense(m):
+        n = len(m[0]) - 1
+        d = np.zeros((n, n))
+        for i in range(n):
+            d[i, m[1][m[0][i]:m[0][i + 1]]] += m[2][m[0][i]:m[0][i + 1]]
+        return d
+
+    D = [dense(m) for m in mats]
+    rhs = [np.zeros(len(m[0]) - 1) for m in mats]
+    xs = [np.zeros(len(m[0]) - 1) for m in mats]
+
+    def smooth(d):
+        xs[d] = xs[d] + 0.75 * (rhs[d] - D[d] @ xs[d]) / np.diag(D[d])
+
+    def restrict(d, residual):
+        fine = rhs[d] - D[d] @ xs[d] if residual else rhs[d]
+        rhs[d + 1] = np.bincount(maps[d], weights=fine, minlength=len(rhs[d + 1]))
+
+    def prolong(d):
+        xs[d] = xs[d] + xs[d + 1][maps[d]]
+
+    def vw(d, w, zero):
+        if zero:
+            xs[d] = np.zeros_like(xs[d])
+        if d + 1 == len(mats):
+            xs[d] = np.linalg.solve(D[d], rhs[d])
+            return
+        smooth(d); restrict(d, True); vw(d + 1, w, True)
+        if w:
+            vw(d + 1, w, False)
+        prolong(d); smooth(d)
+
+    def f(d):
+        xs[d] = np.zeros_like(xs[d])
+        if d + 1 == len(mats):
+            xs[d] = np.linalg.solve(D[d], rhs[d])
+            return
+        restrict(d, False); f(d + 1); prolong(d); vw(d, False, False)
+
+    rhs[0] = b
+    if kind == 2:
+        f(0)
+    else:
+        vw(0, kind == 1, True)
+    return xs[0]
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_oracle_cycle_matches_independent_host_cycle(kind):
+    a = P.poisson1d(127)
+    b = np.sin(0.13 * np.arange(127)) + 0.4
+    amg = O.Amg(coarse_rows=4, cycle=kind).setup(a)
+    assert np.allclose(amg.apply(b), _host_cycle(a, kind, b), atol=2e-10, rtol=0)
+    v = list(amg.visits())
+    L = len(v)
+    if kind == 0:
+        assert v == [1] * L
+    elif kind == 1:
+        assert v == [2 ** d for d in range(L)]
+    else:
+        assert v == [L if d + 1 == L else d + 2 for d in range(L)]
+
+
+def test_oracle_krylov_closed_forms():
+    n = 63
+    a = P.poisson1d(n)
+    i = np.arange(n)
+    exact = (i + 1) * (n - i) / 2
+    x, st, it, _, _ = O.krylov("cg", None, a, np.ones(n), rtol=1e-12)
+    assert st == 0 and np.allclose(x, exact, rtol=1e-9)
+    x, st, it, _, _ = O.krylov("bicgstab", O.Amg(coarse_rows=4).setup(a), a, np.ones(n), rtol=1e-10)
+    assert st == 0 and it < 100 and np.allclose(x, exact, rtol=1e-7)
+    # CG iteration limits: residual norms 1 and 0.5 (cg_test.cpp:100-133) are on 2x2 systems
+    x, st, it, res, _ = O.krylov("cg", None, a, np.ones(n), maxit=0)
+    assert st == 1 and it == 0 and res == pytest.approx(np.sqrt(n))
+    # refresh = 1 on tridiag(5,4) with b = {3,2,2,2,3} -> ones (cg_test.cpp:135-155)
+    t = (np.array([0, 2, 5, 8, 11, 13], np.int32), np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4], np.int32),
+         np.array([5.0, -1, -1, 5, -1, -1, 5, -1, -1, 5, -1, -1, 5]))
+    b5 = np.array([3.0, 2, 2, 2, 3]) + 1 - 1
+    x, st, _, _, _ = O.krylov("cg", None, t, (np.array([5.0, -1, 0, 0, 0]) + np.array([-1.0, 5, -1, 0, 0]) * 0) * 0
+                              + np.array([4.0, 3, 3, 3, 4]), refresh=1)
+    assert st == 0 and np.allclose(x, np.ones(5))
+    _ = b5
+    # breakdown on a zero matrix, non-finite on a NaN right-hand side
+    z = (np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.zeros(2))
+    assert O.krylov("cg", None, z, np.ones(2))[1] == 2
+    nanb = np.ones(n); nanb[5] = np.nan
+    assert O.krylov("cg", None, a, nanb)[1] == 3

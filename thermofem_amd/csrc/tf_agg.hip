@@ -1,0 +1,809 @@
+// Aggregation-AMG setup on gfx950: CSR checks, row validation with the row
+// maximum, the symmetric strength graph, synchronous pairwise matching,
+// graph contraction, singleton absorption, member tables and the Galerkin
+// product for piecewise-constant prolongation.
+//
+// Coordinate merges do not use a global sort.  Every merge here is local to
+// one output row (a CSR row, or the union of the <= 4 member rows of an
+// aggregate), so entries are staged per output row with key
+// (column << 32 | position-in-row), each row segment is sorted on its own
+// (insertion sort for short rows, LDS bitonic for long ones) and duplicate
+// columns are merged sequentially.  Staging in encounter order (members
+// ascending, entries in stored order) makes every sum run in original entry
+// order, which is the order of the stable sort + sequential merge the
+// reference's host oracle defines (test/support/NativeAggregationOracle.cu:43-75),
+// so structures and values are bit-identical to it.
+#include "tf_device.hpp"
+#include "tf_internal.hpp"
+
+namespace tf {
+namespace {
+
+using u64 = unsigned long long;
+
+enum Slot {
+    kKeys = 0,
+    kVals = 1,
+    kCount = 2,
+    kStart = 3,
+    kFill = 4,
+    kMate = 5,
+    kProposal = 6,
+    kFlags = 7,
+    kPrefix = 8,
+    kSizes = 9,
+    kSlots = 10,
+    kNextMap = 11,
+    kPropWeight = 12,
+    kRowmax = 13,
+    kLarge = 14,
+    kSecond = 15,
+    kMemOffsets = 16,
+    kMembers = 17,
+    kHist = 18,
+    kUnique = 19,
+};
+
+inline dim3 grid1(int64_t n) { return dim3(unsigned((n + kBlock - 1) / kBlock > 0 ? (n + kBlock - 1) / kBlock : 1)); }
+
+__device__ __forceinline__ u64 pack_key(int col, int pos) { return (u64(unsigned(col)) << 32) | unsigned(pos); }
+__device__ __forceinline__ int key_col(u64 k) { return int(k >> 32); }
+
+// SplitMix64 finalizer over the ordered pair (GpuAggregation.cu:49-54).
+__device__ __forceinline__ u64 edge_priority(int i, int j) {
+    u64 h = (u64(unsigned(min(i, j))) << 32) | unsigned(max(i, j));
+    h = (h ^ (h >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    h = (h ^ (h >> 27)) * 0x94d049bb133111ebULL;
+    return h ^ (h >> 31);
+}
+
+__global__ void k_fill_i32(int n, int value, int* p) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i < n) p[i] = value;
+}
+
+// checkCanonical (GpuAggregation.cu:283-304): bit 0 unsorted/duplicate, bit 1 invalid.
+__global__ void k_csr_check(int n, int nnz, const int* __restrict__ rp, const int* __restrict__ ci,
+                            const double* __restrict__ v, int* flags) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    if (rp[0] != 0 || rp[n] != nnz || rp[i] < 0 || rp[i] > rp[i + 1] || rp[i + 1] > nnz) {
+        atomicOr(flags, 2);
+        return;
+    }
+    int previous = -1;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        if (ci[j] < 0 || ci[j] >= n || !isfinite(v[j])) atomicOr(flags, 2);
+        if (ci[j] <= previous) atomicOr(flags, 1);
+        previous = ci[j];
+    }
+}
+
+// Device CSR structure check of CsrMatrix::fromBuffers (CsrMatrix.cu:12-31).
+__global__ void k_csr_structure(int rows, int cols, int64_t count, const int* __restrict__ rp,
+                                const int* __restrict__ ci, int* bad) {
+    const int r = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (r == 0 && (rp[0] != 0 || int64_t(rp[rows]) != count)) atomicOr(bad, 1);
+    if (r >= rows) return;
+    if (rp[r] < 0 || rp[r + 1] < rp[r] || int64_t(rp[r + 1]) > count) {
+        atomicOr(bad, 1);
+        return;
+    }
+    for (int j = rp[r]; j < rp[r + 1]; ++j)
+        if (ci[j] < 0 || ci[j] >= cols) atomicOr(bad, 1);
+}
+
+// validate (GpuAggregation.cu:247-271).
+__global__ void k_validate(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ a,
+                           int* bad, double* __restrict__ rowmax) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    double diag = 0, mx = 0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        if (!isfinite(a[j])) atomicOr(bad, 1);
+        if (ci[j] == i)
+            diag = a[j];
+        else
+            mx = fmax(mx, fabs(a[j]));
+    }
+    if (diag == 0 || !isfinite(diag) || !isfinite(1.0 / diag)) atomicOr(bad, 1);
+    if (rowmax) rowmax[i] = mx;
+}
+
+// Strength edges (GpuAggregation.cu:339-364): w = |a_ij| / rowmax_i for
+// j != i when rowmax_i > 0, w > 0 and w >= theta; both directions.
+__global__ void k_strength_count(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                 const double* __restrict__ a, const double* __restrict__ rowmax, double theta,
+                                 int* cnt) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    const double mx = rowmax[i];
+    if (!(mx > 0)) return;
+    int out = 0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const int c = ci[j];
+        if (c == i) continue;
+        const double w = fabs(a[j]) / mx;
+        if (w >= theta && w > 0) {
+            ++out;
+            atomicAdd(cnt + c, 1);
+        }
+    }
+    if (out) atomicAdd(cnt + i, out);
+}
+
+__global__ void k_strength_fill(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                const double* __restrict__ a, const double* __restrict__ rowmax, double theta,
+                                const int* __restrict__ start, int* fill, u64* __restrict__ keys,
+                                double* __restrict__ vals) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    const double mx = rowmax[i];
+    if (!(mx > 0)) return;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const int c = ci[j];
+        if (c == i) continue;
+        const double w = fabs(a[j]) / mx;
+        if (w >= theta && w > 0) {
+            const int p = atomicAdd(fill + i, 1);
+            keys[start[i] + p] = pack_key(c, p);
+            vals[start[i] + p] = w;
+            const int q = atomicAdd(fill + c, 1);
+            keys[start[c] + q] = pack_key(i, q);
+            vals[start[c] + q] = w;
+        }
+    }
+}
+
+// Output row I gathers the rows of its members (all rows of `offsets`
+// order; offsets == nullptr means the identity aggregate {I}); columns go
+// through map (nullptr: identity).  With `graph` set, intra-aggregate and
+// non-positive entries are dropped (mapGraph, GpuAggregation.cu:534-553).
+__device__ __forceinline__ void member_range(int I, const int* offsets, int& b, int& e) {
+    if (offsets) {
+        b = offsets[I];
+        e = offsets[I + 1];
+    }
+    else {
+        b = I;
+        e = I + 1;
+    }
+}
+
+__global__ void k_gather_count(int nc, const int* __restrict__ offsets, const int* __restrict__ members,
+                               const int* __restrict__ map, const int* __restrict__ rp, const int* __restrict__ ci,
+                               const double* __restrict__ v, int graph, int* cnt) {
+    const int I = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (I == nc) cnt[nc] = 0;
+    if (I >= nc) return;
+    int mb, me;
+    member_range(I, offsets, mb, me);
+    int total = 0;
+    for (int k = mb; k < me; ++k) {
+        const int m = offsets ? members[k] : k;
+        if (!graph) {
+            total += rp[m + 1] - rp[m];
+            continue;
+        }
+        for (int j = rp[m]; j < rp[m + 1]; ++j) {
+            const int J = map ? map[ci[j]] : ci[j];
+            if (J != I && v[j] > 0) ++total;
+        }
+    }
+    cnt[I] = total;
+}
+
+__global__ void k_gather_fill(int nc, const int* __restrict__ offsets, const int* __restrict__ members,
+                              const int* __restrict__ map, const int* __restrict__ rp, const int* __restrict__ ci,
+                              const double* __restrict__ v, int graph, const int* __restrict__ start,
+                              u64* __restrict__ keys, double* __restrict__ vals) {
+    const int I = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (I >= nc) return;
+    int mb, me;
+    member_range(I, offsets, mb, me);
+    int pos = 0;
+    const int base = start[I];
+    for (int k = mb; k < me; ++k) {
+        const int m = offsets ? members[k] : k;
+        for (int j = rp[m]; j < rp[m + 1]; ++j) {
+            const int J = map ? map[ci[j]] : ci[j];
+            if (graph && (J == I || !(v[j] > 0))) continue;
+            keys[base + pos] = pack_key(J, pos);
+            vals[base + pos] = v[j];
+            ++pos;
+        }
+    }
+}
+
+constexpr int kSmallSegment = 32;
+constexpr int kLdsSegment = 2048;
+
+// Short segments: stable insertion sort by key in place; long ones are
+// queued for the block-wide sort.
+__global__ void k_segsort_small(int rows, const int* __restrict__ start, u64* keys, double* vals, int* large,
+                                int* largeCount) {
+    const int r = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (r >= rows) return;
+    const int b = start[r], len = start[r + 1] - b;
+    if (len > kSmallSegment) {
+        large[atomicAdd(largeCount, 1)] = r;
+        return;
+    }
+    u64* k = keys + b;
+    double* x = vals + b;
+    for (int i = 1; i < len; ++i) {
+        const u64 kk = k[i];
+        const double xx = x[i];
+        int j = i;
+        while (j > 0 && k[j - 1] > kk) {
+            k[j] = k[j - 1];
+            x[j] = x[j - 1];
+            --j;
+        }
+        k[j] = kk;
+        x[j] = xx;
+    }
+}
+
+// Bitonic network whose compare-exchanges all move the minimum to the lower
+// index (mirror step, then half-cleaners); indices >= len act as +inf and
+// never move, so any len sorts without padding storage.
+template <class KeyAt, class Swap>
+__device__ void bitonic(int len, KeyAt keyAt, Swap swapAt) {
+    int N = 1;
+    while (N < len) N <<= 1;
+    for (int k = 2; k <= N; k <<= 1) {
+        const int half = k >> 1;
+        for (int t = int(threadIdx.x); t < N / 2; t += int(blockDim.x)) {
+            const int blk = t / half, off = t % half;
+            const int i = blk * k + off, p = blk * k + k - 1 - off;
+            if (p < len && keyAt(i) > keyAt(p)) swapAt(i, p);
+        }
+        __syncthreads();
+        for (int j = half >> 1; j > 0; j >>= 1) {
+            for (int t = int(threadIdx.x); t < N / 2; t += int(blockDim.x)) {
+                const int i = (t / j) * 2 * j + (t % j), p = i + j;
+                if (p < len && keyAt(i) > keyAt(p)) swapAt(i, p);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_segsort_large(const int* __restrict__ start, u64* keys, double* vals,
+                                                          const int* __restrict__ large,
+                                                          const int* __restrict__ largeCount) {
+    __shared__ u64 sk[kLdsSegment];
+    __shared__ double sv[kLdsSegment];
+    const int count = *largeCount;
+    for (int q = int(blockIdx.x); q < count; q += int(gridDim.x)) {
+        const int r = large[q];
+        const int b = start[r], len = start[r + 1] - b;
+        u64* gk = keys + b;
+        double* gv = vals + b;
+        if (len <= kLdsSegment) {
+            for (int i = int(threadIdx.x); i < len; i += kBlock) {
+                sk[i] = gk[i];
+                sv[i] = gv[i];
+            }
+            __syncthreads();
+            bitonic(
+                len, [&](int i) { return sk[i]; },
+                [&](int i, int p) {
+                    const u64 tk = sk[i];
+                    sk[i] = sk[p];
+                    sk[p] = tk;
+                    const double tv = sv[i];
+                    sv[i] = sv[p];
+                    sv[p] = tv;
+                });
+            for (int i = int(threadIdx.x); i < len; i += kBlock) {
+                gk[i] = sk[i];
+                gv[i] = sv[i];
+            }
+            __syncthreads();
+        }
+        else {
+            bitonic(
+                len, [&](int i) { return gk[i]; },
+                [&](int i, int p) {
+                    const u64 tk = gk[i];
+                    gk[i] = gk[p];
+                    gk[p] = tk;
+                    const double tv = gv[i];
+                    gv[i] = gv[p];
+                    gv[p] = tv;
+                });
+        }
+    }
+}
+
+__global__ void k_unique_count(int rows, const int* __restrict__ start, const u64* __restrict__ keys, int* ucnt) {
+    const int r = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (r == rows) ucnt[rows] = 0;
+    if (r >= rows) return;
+    int u = 0, prev = -1;
+    for (int j = start[r]; j < start[r + 1]; ++j) {
+        const int c = key_col(keys[j]);
+        if (u == 0 || c != prev) {
+            ++u;
+            prev = c;
+        }
+    }
+    ucnt[r] = u;
+}
+
+// Sequential merge of equal columns: sum (a + b in order) or max
+// (a > b ? a : b, GpuAggregation.cu:17-23).
+template <bool Max>
+__global__ void k_merge_write(int rows, const int* __restrict__ start, const u64* __restrict__ keys,
+                              const double* __restrict__ vals, const int* __restrict__ orp, int* __restrict__ oci,
+                              double* __restrict__ ov, int* bad) {
+    const int r = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (r >= rows) return;
+    int o = orp[r] - 1, prev = -1;
+    bool first = true;
+    for (int j = start[r]; j < start[r + 1]; ++j) {
+        const int c = key_col(keys[j]);
+        const double x = vals[j];
+        if (first || c != prev) {
+            ++o;
+            oci[o] = c;
+            ov[o] = x;
+            prev = c;
+            first = false;
+        }
+        else {
+            const double acc = ov[o];
+            const double m = Max ? (acc > x ? acc : x) : acc + x;
+            if (!isfinite(m)) atomicOr(bad, 1);
+            ov[o] = m;
+        }
+    }
+}
+
+// ---- matching (GpuAggregation.cu:402-505) ----
+__global__ void k_propose(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ w,
+                          const int* __restrict__ mate, int* __restrict__ proposal) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    int best = -1;
+    double score = 0;
+    u64 priority = 0;
+    if (mate[i] < 0) {
+        for (int j = rp[i]; j < rp[i + 1]; ++j) {
+            const int c = ci[j];
+            const double s = w[j];
+            if (c == i || mate[c] >= 0 || s <= 0) continue;
+            if (s < score) continue;
+            const u64 cand = edge_priority(i, c);
+            if (s > score || (s == score && (best < 0 || cand > priority || (cand == priority && c < best)))) {
+                best = c;
+                score = s;
+                priority = cand;
+            }
+        }
+    }
+    proposal[i] = best;
+}
+
+__global__ void k_accept(int n, const int* __restrict__ proposal, int* __restrict__ mate) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    const int p = proposal[i];
+    if (p >= 0 && proposal[p] == i) mate[i] = p;
+}
+
+__global__ void k_roots(int n, const int* __restrict__ mate, int* __restrict__ flags) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i <= n) flags[i] = (i < n && (mate[i] < 0 || i < mate[i])) ? 1 : 0;
+}
+
+__global__ void k_assign(int n, const int* __restrict__ mate, const int* __restrict__ prefix, int* __restrict__ map) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i < n) map[i] = prefix[mate[i] < 0 ? i : min(i, mate[i])];
+}
+
+__global__ void k_compose(int n, int* __restrict__ map, const int* __restrict__ second) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i < n) map[i] = second[map[i]];
+}
+
+// ---- members (GpuAggregation.cu:586-632) ----
+__global__ void k_slot_members(int n, const int* __restrict__ map, int* sizes, int* __restrict__ slots) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    const int c = map[i];
+    const int slot = atomicAdd(sizes + c, 1);
+    if (slot < 4) slots[size_t(c) * 4 + slot] = i;
+}
+
+__global__ void k_sort_members(int nc, const int* __restrict__ sizes, int* __restrict__ slots, int* overflow) {
+    const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (c >= nc) return;
+    const int sz = sizes[c];
+    if (sz > 4) {
+        atomicOr(overflow, 1);
+        return;
+    }
+    int* m = slots + size_t(c) * 4;
+    for (int i = 1; i < sz; ++i) {
+        const int x = m[i];
+        int j = i;
+        while (j > 0 && m[j - 1] > x) {
+            m[j] = m[j - 1];
+            --j;
+        }
+        m[j] = x;
+    }
+}
+
+// ---- absorption (GpuAggregation.cu:646-720) ----
+__global__ void k_propose_absorption(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                     const double* __restrict__ w, const int* __restrict__ map,
+                                     const int* __restrict__ sizes, int* __restrict__ proposal,
+                                     double* __restrict__ scores) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    int best = -1;
+    double score = 0;
+    if (sizes[map[i]] == 1) {
+        for (int j = rp[i]; j < rp[i + 1]; ++j) {
+            const int c = map[ci[j]];
+            const double s = w[j];
+            if (sizes[c] < 2 || sizes[c] >= 4 || s <= 0) continue;
+            if (s > score || (s == score && (best < 0 || c < best))) {
+                best = c;
+                score = s;
+            }
+        }
+    }
+    proposal[i] = best;
+    scores[i] = score;
+}
+
+__global__ void k_accept_absorption(int nc, const int* __restrict__ rp, const int* __restrict__ ci,
+                                    const int* __restrict__ sizes, const int* __restrict__ slots,
+                                    const int* __restrict__ proposal, const double* __restrict__ scores,
+                                    int* __restrict__ nextMap) {
+    const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (c >= nc || sizes[c] < 2 || sizes[c] >= 4) return;
+    int selected[2] = {-1, -1};
+    double weights[2] = {-1, -1};
+    const int capacity = 4 - sizes[c];
+    for (int k = 0; k < sizes[c]; ++k) {
+        const int row = slots[size_t(c) * 4 + k];
+        for (int j = rp[row]; j < rp[row + 1]; ++j) {
+            const int i = ci[j];
+            if (proposal[i] != c || i == selected[0] || i == selected[1]) continue;
+            const double s = scores[i];
+            for (int q = 0; q < capacity; ++q) {
+                if (s > weights[q] || (s == weights[q] && i < selected[q])) {
+                    for (int r = capacity - 1; r > q; --r) {
+                        selected[r] = selected[r - 1];
+                        weights[r] = weights[r - 1];
+                    }
+                    selected[q] = i;
+                    weights[q] = s;
+                    break;
+                }
+            }
+        }
+    }
+    for (int q = 0; q < capacity; ++q)
+        if (selected[q] >= 0) nextMap[selected[q]] = c;
+}
+
+__global__ void k_occupied(int nc, const int* __restrict__ sizes, int* __restrict__ flags) {
+    const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (c <= nc) flags[c] = (c < nc && sizes[c] > 0) ? 1 : 0;
+}
+
+__global__ void k_pack_members(int nc, const int* __restrict__ sizes, const int* __restrict__ slots,
+                               const int* __restrict__ offsets, int* __restrict__ out, int* histogram) {
+    const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (c >= nc) return;
+    const int sz = sizes[c];
+    for (int k = 0; k < sz; ++k) out[offsets[c] + k] = slots[size_t(c) * 4 + k];
+    if (histogram && sz) atomicAdd(histogram + sz - 1, 1);
+}
+
+// ---------------- host helpers ----------------
+
+void fill_i32(tf_ctx& c, int n, int value, int* p) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_i32, grid1(n), dim3(kBlock), 0, c.stream, n, value, p);
+    check(hipGetLastError(), "fill launch");
+}
+
+// Sort each row segment of (keys, vals) by key.
+void segsort(tf_ctx& c, int rows, const int* start, u64* keys, double* vals) {
+    if (rows <= 0) return;
+    int* large = c.wst<int>(kLarge, size_t(rows) + 1);
+    int* largeCount = large + rows;
+    fill_i32(c, 1, 0, largeCount);
+    hipLaunchKernelGGL(k_segsort_small, grid1(rows), dim3(kBlock), 0, c.stream, rows, start, keys, vals, large,
+                       largeCount);
+    check(hipGetLastError(), "segsort small launch");
+    const int blocks = rows < 512 ? rows : 512;
+    hipLaunchKernelGGL(k_segsort_large, dim3(blocks), dim3(kBlock), 0, c.stream, start, keys, vals, large, largeCount);
+    check(hipGetLastError(), "segsort large launch");
+}
+
+// Merge equal columns of sorted row segments into a new CSR.
+DevCsr merge_rows(tf_ctx& c, int rows, const int* start, const u64* keys, const double* vals, bool maximum) {
+    DevCsr out;
+    out.n = rows;
+    int* ucnt = c.wst<int>(kUnique, size_t(rows) + 1);
+    hipLaunchKernelGGL(k_unique_count, grid1(int64_t(rows) + 1), dim3(kBlock), 0, c.stream, rows, start, keys, ucnt);
+    check(hipGetLastError(), "unique count launch");
+    out.rp = dev_alloc<int>(size_t(rows) + 1);
+    scan_excl(c, rows + 1, ucnt, out.rp);
+    out.nnz = c.read_int(out.rp + rows);
+    out.ci = dev_alloc<int>(size_t(out.nnz));
+    out.v = dev_alloc<double>(size_t(out.nnz));
+    c.clear_flags(1);
+    if (maximum)
+        hipLaunchKernelGGL(k_merge_write<true>, grid1(rows), dim3(kBlock), 0, c.stream, rows, start, keys, vals,
+                           out.rp, out.ci, out.v, c.dflags);
+    else
+        hipLaunchKernelGGL(k_merge_write<false>, grid1(rows), dim3(kBlock), 0, c.stream, rows, start, keys, vals,
+                           out.rp, out.ci, out.v, c.dflags);
+    check(hipGetLastError(), "merge launch");
+    c.read_flags(1);
+    if (c.hflags[0]) {
+        free_csr(out);
+        throw Error(TF_E_NUMERIC, "Nonfinite aggregate reduction");
+    }
+    return out;
+}
+
+// Gather member rows into per-output-row segments, sort, merge.
+DevCsr gather_merge(tf_ctx& c, int nc, const int* offsets, const int* members, const int* map, const int* rp,
+                    const int* ci, const double* v, bool graph, bool maximum) {
+    int* cnt = c.wst<int>(kCount, size_t(nc) + 1);
+    hipLaunchKernelGGL(k_gather_count, grid1(int64_t(nc) + 1), dim3(kBlock), 0, c.stream, nc, offsets, members, map,
+                       rp, ci, v, int(graph), cnt);
+    check(hipGetLastError(), "gather count launch");
+    int* start = c.wst<int>(kStart, size_t(nc) + 1);
+    scan_excl(c, nc + 1, cnt, start);
+    const int total = c.read_int(start + nc);
+    u64* keys = c.wst<u64>(kKeys, size_t(total));
+    double* vals = c.wst<double>(kVals, size_t(total));
+    hipLaunchKernelGGL(k_gather_fill, grid1(nc), dim3(kBlock), 0, c.stream, nc, offsets, members, map, rp, ci, v,
+                       int(graph), start, keys, vals);
+    check(hipGetLastError(), "gather fill launch");
+    segsort(c, nc, start, keys, vals);
+    return merge_rows(c, nc, start, keys, vals, maximum);
+}
+
+// Member table: sizes (nc+1, last entry 0) and 4-slot sorted members.
+void build_members(tf_ctx& c, int n, int nc, const int* map, int** sizesOut, int** slotsOut) {
+    int* sizes = c.wst<int>(kSizes, size_t(nc) + 1);
+    int* slots = c.wst<int>(kSlots, size_t(nc) * 4 + 1);
+    fill_i32(c, nc + 1, 0, sizes);
+    hipLaunchKernelGGL(k_slot_members, grid1(n), dim3(kBlock), 0, c.stream, n, map, sizes, slots);
+    check(hipGetLastError(), "slot members launch");
+    c.clear_flags(1);
+    hipLaunchKernelGGL(k_sort_members, grid1(nc), dim3(kBlock), 0, c.stream, nc, sizes, slots, c.dflags);
+    check(hipGetLastError(), "sort members launch");
+    *sizesOut = sizes;
+    *slotsOut = slots;
+}
+
+}  // namespace
+
+void free_csr(DevCsr& a) {
+    if (a.rp) (void)hipFree(a.rp);
+    if (a.ci) (void)hipFree(a.ci);
+    if (a.v) (void)hipFree(a.v);
+    a.rp = a.ci = nullptr;
+    a.v = nullptr;
+}
+
+void free_agg(DevAgg& t) {
+    if (t.map) (void)hipFree(t.map);
+    if (t.offsets) (void)hipFree(t.offsets);
+    if (t.members) (void)hipFree(t.members);
+    t.map = t.offsets = t.members = nullptr;
+}
+
+int csr_check(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v) {
+    c.clear_flags(1);
+    hipLaunchKernelGGL(k_csr_check, grid1(n), dim3(kBlock), 0, c.stream, n, int(nnz), rp, ci, v, c.dflags);
+    check(hipGetLastError(), "csr check launch");
+    c.read_flags(1);
+    return c.hflags[0];
+}
+
+bool csr_structure_ok(tf_ctx& c, int rows, int cols, int64_t nnz, const int* rp, const int* ci) {
+    c.clear_flags(1);
+    hipLaunchKernelGGL(k_csr_structure, grid1(rows > 0 ? rows : 1), dim3(kBlock), 0, c.stream, rows, cols, nnz, rp,
+                       ci, c.dflags);
+    check(hipGetLastError(), "csr structure launch");
+    c.read_flags(1);
+    return c.hflags[0] == 0;
+}
+
+DevCsr copy_csr(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v) {
+    DevCsr out;
+    out.n = n;
+    out.nnz = nnz;
+    out.rp = dev_alloc<int>(size_t(n) + 1);
+    out.ci = dev_alloc<int>(size_t(nnz));
+    out.v = dev_alloc<double>(size_t(nnz));
+    check(hipMemcpyAsync(out.rp, rp, (size_t(n) + 1) * sizeof(int), hipMemcpyDefault, c.stream), "copy rp");
+    if (nnz) {
+        check(hipMemcpyAsync(out.ci, ci, size_t(nnz) * sizeof(int), hipMemcpyDefault, c.stream), "copy ci");
+        check(hipMemcpyAsync(out.v, v, size_t(nnz) * sizeof(double), hipMemcpyDefault, c.stream), "copy v");
+    }
+    return out;
+}
+
+DevCsr canonicalize(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v) {
+    require(n >= 1 && n < 0x7fffffff && nnz <= 0x3fffffff,
+            "AMG requires nonempty square int32-capacity CSR on workspace device");
+    const int state = csr_check(c, n, nnz, rp, ci, v);
+    if (state & 2) throw Error(TF_E_INVALID, "AMG invalid CSR or nonfinite coefficient");
+    if (state == 0) return copy_csr(c, n, nnz, rp, ci, v);
+    return gather_merge(c, n, nullptr, nullptr, nullptr, rp, ci, v, false, false);
+}
+
+bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax) {
+    c.clear_flags(1);
+    hipLaunchKernelGGL(k_validate, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, v, c.dflags, rowmax);
+    check(hipGetLastError(), "validate launch");
+    c.read_flags(1);
+    return c.hflags[0] == 0;
+}
+
+DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta) {
+    require(std::isfinite(theta) && theta >= 0 && theta <= 1, "Invalid strength threshold");
+    if (nnz > int64_t(0x3fffffff)) throw Error(TF_E_CAPACITY, "Native AMG: strength graph exceeds 32-bit capacity");
+    double* rowmax = c.wst<double>(kRowmax, size_t(n));
+    if (!validate_rowmax(c, n, rp, ci, v, rowmax))
+        throw Error(TF_E_NUMERIC, "Native AMG: missing/zero diagonal, non-finite value or reciprocal in hierarchy");
+    int* cnt = c.wst<int>(kCount, size_t(n) + 1);
+    fill_i32(c, n + 1, 0, cnt);
+    hipLaunchKernelGGL(k_strength_count, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, v, rowmax, theta, cnt);
+    check(hipGetLastError(), "strength count launch");
+    int* start = c.wst<int>(kStart, size_t(n) + 1);
+    scan_excl(c, n + 1, cnt, start);
+    const int total = c.read_int(start + n);
+    if (total == 0) {
+        DevCsr g;
+        g.n = n;
+        g.nnz = 0;
+        g.rp = dev_alloc<int>(size_t(n) + 1);
+        fill_i32(c, n + 1, 0, g.rp);
+        g.ci = dev_alloc<int>(0);
+        g.v = dev_alloc<double>(0);
+        return g;
+    }
+    int* fill = c.wst<int>(kFill, size_t(n));
+    fill_i32(c, n, 0, fill);
+    u64* keys = c.wst<u64>(kKeys, size_t(total));
+    double* vals = c.wst<double>(kVals, size_t(total));
+    hipLaunchKernelGGL(k_strength_fill, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, v, rowmax, theta, start, fill,
+                       keys, vals);
+    check(hipGetLastError(), "strength fill launch");
+    segsort(c, n, start, keys, vals);
+    return merge_rows(c, n, start, keys, vals, true);
+}
+
+void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal) {
+    hipLaunchKernelGGL(k_propose, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, w, mate, proposal);
+    check(hipGetLastError(), "propose launch");
+    hipLaunchKernelGGL(k_accept, grid1(n), dim3(kBlock), 0, c.stream, n, proposal, mate);
+    check(hipGetLastError(), "accept launch");
+}
+
+int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map) {
+    int* mate = c.wst<int>(kMate, size_t(n));
+    int* proposal = c.wst<int>(kProposal, size_t(n));
+    fill_i32(c, n, -1, mate);
+    for (int r = 0; r < rounds; ++r) match_round(c, n, rp, ci, w, mate, proposal);
+    int* flags = c.wst<int>(kFlags, size_t(n) + 1);
+    hipLaunchKernelGGL(k_roots, grid1(int64_t(n) + 1), dim3(kBlock), 0, c.stream, n, mate, flags);
+    check(hipGetLastError(), "roots launch");
+    int* prefix = c.wst<int>(kPrefix, size_t(n) + 1);
+    scan_excl(c, n + 1, flags, prefix);
+    hipLaunchKernelGGL(k_assign, grid1(n), dim3(kBlock), 0, c.stream, n, mate, prefix, map);
+    check(hipGetLastError(), "assign launch");
+    return c.read_int(prefix + n);
+}
+
+DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc) {
+    int *sizes, *slots;
+    build_members(c, n, nc, map, &sizes, &slots);
+    int* moff = c.wst<int>(kMemOffsets, size_t(nc) + 1);
+    scan_excl(c, nc + 1, sizes, moff);
+    int* mem = c.wst<int>(kMembers, size_t(n));
+    hipLaunchKernelGGL(k_pack_members, grid1(nc), dim3(kBlock), 0, c.stream, nc, sizes, slots, moff, mem,
+                       static_cast<int*>(nullptr));
+    check(hipGetLastError(), "pack members launch");
+    return gather_merge(c, nc, moff, mem, map, rp, ci, w, true, true);
+}
+
+DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb) {
+    require(rounds >= 1 && rounds <= 8 && g.n >= 1, "Invalid matching configuration");
+    const int n = g.n;
+    DevAgg t;
+    t.n = n;
+    t.map = dev_alloc<int>(size_t(n));
+    try {
+        const int first = pair(c, n, g.rp, g.ci, g.v, rounds, t.map);
+        DevCsr contracted = contract_graph(c, n, g.rp, g.ci, g.v, t.map, first);
+        int* second = c.wst<int>(kSecond, size_t(first));
+        int coarse = 0;
+        try {
+            coarse = pair(c, first, contracted.rp, contracted.ci, contracted.v, rounds, second);
+        }
+        catch (...) {
+            free_csr(contracted);
+            throw;
+        }
+        free_csr(contracted);
+        hipLaunchKernelGGL(k_compose, grid1(n), dim3(kBlock), 0, c.stream, n, t.map, second);
+        check(hipGetLastError(), "compose launch");
+        int *sizes, *slots;
+        build_members(c, n, coarse, t.map, &sizes, &slots);
+        if (absorb) {
+            int* proposal = c.wst<int>(kProposal, size_t(n));
+            double* scores = c.wst<double>(kPropWeight, size_t(n));
+            int* next = c.wst<int>(kNextMap, size_t(n));
+            int* cur = t.map;
+            for (int round = 0; round < 2; ++round) {
+                hipLaunchKernelGGL(k_propose_absorption, grid1(n), dim3(kBlock), 0, c.stream, n, g.rp, g.ci, g.v, cur,
+                                   sizes, proposal, scores);
+                check(hipGetLastError(), "propose absorption launch");
+                check(hipMemcpyAsync(next, cur, size_t(n) * sizeof(int), hipMemcpyDeviceToDevice, c.stream),
+                      "absorption snapshot");
+                hipLaunchKernelGGL(k_accept_absorption, grid1(coarse), dim3(kBlock), 0, c.stream, coarse, g.rp, g.ci,
+                                   sizes, slots, proposal, scores, next);
+                check(hipGetLastError(), "accept absorption launch");
+                std::swap(cur, next);
+                build_members(c, n, coarse, cur, &sizes, &slots);
+            }
+            if (cur != t.map)
+                check(hipMemcpyAsync(t.map, cur, size_t(n) * sizeof(int), hipMemcpyDeviceToDevice, c.stream),
+                      "absorbed map copy");
+            int* flags = c.wst<int>(kFlags, size_t(coarse) + 1);
+            hipLaunchKernelGGL(k_occupied, grid1(int64_t(coarse) + 1), dim3(kBlock), 0, c.stream, coarse, sizes,
+                               flags);
+            check(hipGetLastError(), "occupied launch");
+            int* prefix = c.wst<int>(kPrefix, size_t(coarse) + 1);
+            scan_excl(c, coarse + 1, flags, prefix);
+            hipLaunchKernelGGL(k_compose, grid1(n), dim3(kBlock), 0, c.stream, n, t.map, prefix);
+            check(hipGetLastError(), "compaction launch");
+            coarse = c.read_int(prefix + coarse);
+            build_members(c, n, coarse, t.map, &sizes, &slots);
+        }
+        c.read_flags(1);
+        if (c.hflags[0]) throw Error(TF_E_INVALID, "Aggregate capacity exceeded");
+        t.nc = coarse;
+        t.offsets = dev_alloc<int>(size_t(coarse) + 1);
+        t.members = dev_alloc<int>(size_t(n));
+        scan_excl(c, coarse + 1, sizes, t.offsets);
+        int* hist = c.wst<int>(kHist, 4);
+        fill_i32(c, 4, 0, hist);
+        hipLaunchKernelGGL(k_pack_members, grid1(coarse), dim3(kBlock), 0, c.stream, coarse, sizes, slots, t.offsets,
+                           t.members, hist);
+        check(hipGetLastError(), "pack members launch");
+        check(hipMemcpyAsync(c.hflags, hist, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream), "histogram download");
+        c.sync();
+        for (int k = 0; k < 4; ++k) t.hist[k] = c.hflags[k];
+    }
+    catch (...) {
+        free_agg(t);
+        throw;
+    }
+    return t;
+}
+
+DevCsr galerkin(tf_ctx& c, const DevCsr& a, const int* map, int nc, const int* offsets, const int* members) {
+    require(nc >= 1, "AMG transfer dimension mismatch");
+    return gather_merge(c, nc, offsets, members, map, a.rp, a.ci, a.v, false, false);
+}
+
+}  // namespace tf

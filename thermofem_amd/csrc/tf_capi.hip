@@ -1,0 +1,638 @@
+// C ABI of the library (include/thermofem_hip.h): contexts, memory, the
+// kernel-level entry points and the AMG / Krylov handles.  Exceptions never
+// cross this boundary; they become negative status codes with a message in
+// tf_last_error().
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "tf_engine.hpp"
+#include "tf_internal.hpp"
+
+namespace tf {
+namespace {
+thread_local std::string g_error;
+
+// Device copies of host CSR arrays for the host_memory entry points.
+struct Staged {
+    int* rp = nullptr;
+    int* ci = nullptr;
+    double* v = nullptr;
+    ~Staged() {
+        if (rp) (void)hipFree(rp);
+        if (ci) (void)hipFree(ci);
+        if (v) (void)hipFree(v);
+    }
+};
+
+void stage(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, Staged& s) {
+    s.rp = dev_alloc<int>(size_t(n) + 1);
+    s.ci = dev_alloc<int>(size_t(nnz));
+    s.v = dev_alloc<double>(size_t(nnz));
+    check(hipMemcpyAsync(s.rp, rp, (size_t(n) + 1) * sizeof(int), hipMemcpyHostToDevice, c.stream), "upload rp");
+    if (nnz) {
+        check(hipMemcpyAsync(s.ci, ci, size_t(nnz) * sizeof(int), hipMemcpyHostToDevice, c.stream), "upload ci");
+        check(hipMemcpyAsync(s.v, v, size_t(nnz) * sizeof(double), hipMemcpyHostToDevice, c.stream), "upload v");
+    }
+}
+
+tf_ctx* make_ctx(int device) {
+    int count = 0;
+    check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    require(device >= 0 && device < count, "invalid device id");
+    DeviceScope scope(device);
+    auto c = std::make_unique<tf_ctx>();
+    c->device = device;
+    check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipMalloc(&c->partials, size_t(kMaxReduceBlocks) * kTickets * sizeof(double)), "hipMalloc partials");
+    check(hipMalloc(&c->tickets, kTickets * sizeof(unsigned)), "hipMalloc tickets");
+    check(hipMemset(c->tickets, 0, kTickets * sizeof(unsigned)), "tickets clear");
+    check(hipMalloc(&c->dscalars, 64 * sizeof(double)), "hipMalloc scalars");
+    check(hipHostMalloc(&c->hscalars, 64 * sizeof(double), hipHostMallocDefault), "hipHostMalloc scalars");
+    check(hipMalloc(&c->dflags, 64 * sizeof(int)), "hipMalloc flags");
+    check(hipHostMalloc(&c->hflags, 64 * sizeof(int), hipHostMallocDefault), "hipHostMalloc flags");
+    check(hipDeviceSynchronize(), "context init");
+    return c.release();
+}
+
+void destroy_ctx(tf_ctx* c) {
+    if (!c) return;
+    DeviceScope scope(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 32; ++i)
+        if (c->ws_ptr[i]) (void)hipFree(c->ws_ptr[i]);
+    if (c->scan_tmp) (void)hipFree(c->scan_tmp);
+    if (c->partials) (void)hipFree(c->partials);
+    if (c->tickets) (void)hipFree(c->tickets);
+    if (c->dscalars) (void)hipFree(c->dscalars);
+    if (c->hscalars) (void)hipHostFree(c->hscalars);
+    if (c->dflags) (void)hipFree(c->dflags);
+    if (c->hflags) (void)hipHostFree(c->hflags);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+template <class T>
+double host_dot(tf_ctx& c, int n, const T* x, const T* y) {
+    dot_device<T>(c, n, x, y, c.dscalars, 0);
+    check(hipMemcpyAsync(c.hscalars, c.dscalars, sizeof(double), hipMemcpyDeviceToHost, c.stream), "dot download");
+    c.sync();
+    return c.hscalars[0];
+}
+
+}  // namespace
+
+void set_error(const char* msg) { g_error = msg ? msg : ""; }
+
+}  // namespace tf
+
+struct tf_amg {
+    tf_ctx* ctx = nullptr;
+    tf::AmgEngine* eng = nullptr;
+};
+
+struct tf_solver {
+    tf_ctx* ctx = nullptr;
+    tf::AmgEngine* amg = nullptr;
+    tf::SolverEngine* eng = nullptr;
+    tf_amg view;
+};
+
+using namespace tf;
+
+#define TF_CTX_GUARD(ctx, body)                                      \
+    return guard([&] {                                              \
+        require((ctx) != nullptr, "null context");                  \
+        DeviceScope scope_((ctx)->device);                          \
+        body;                                                       \
+    })
+
+extern "C" {
+
+const char* tf_last_error(void) { return g_error.c_str(); }
+
+int tf_device_count(int* count) {
+    return guard([&] {
+        require(count != nullptr, "null count");
+        check(hipGetDeviceCount(count), "hipGetDeviceCount");
+    });
+}
+
+int tf_ctx_create(int device, tf_ctx** ctx) {
+    return guard([&] {
+        require(ctx != nullptr, "null context pointer");
+        *ctx = make_ctx(device);
+    });
+}
+
+int tf_ctx_destroy(tf_ctx* ctx) {
+    return guard([&] { destroy_ctx(ctx); });
+}
+
+int tf_ctx_synchronize(tf_ctx* ctx) { TF_CTX_GUARD(ctx, ctx->sync()); }
+void* tf_ctx_stream(tf_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+int tf_ctx_device(tf_ctx* ctx) { return ctx ? ctx->device : -1; }
+
+// ---------------------------------------------------------------- events
+int tf_event_create(tf_event** ev) {
+    return guard([&] {
+        require(ev != nullptr, "null event pointer");
+        hipEvent_t e = nullptr;
+        check(hipEventCreate(&e), "hipEventCreate");
+        *ev = reinterpret_cast<tf_event*>(e);
+    });
+}
+int tf_event_destroy(tf_event* ev) {
+    return guard([&] {
+        if (ev) check(hipEventDestroy(reinterpret_cast<hipEvent_t>(ev)), "hipEventDestroy");
+    });
+}
+int tf_event_record(tf_ctx* ctx, tf_event* ev) {
+    TF_CTX_GUARD(ctx, check(hipEventRecord(reinterpret_cast<hipEvent_t>(ev), ctx->stream), "hipEventRecord"));
+}
+int tf_event_elapsed_ms(tf_event* start, tf_event* stop, float* ms) {
+    return guard([&] {
+        check(hipEventSynchronize(reinterpret_cast<hipEvent_t>(stop)), "hipEventSynchronize");
+        check(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start), reinterpret_cast<hipEvent_t>(stop)),
+              "hipEventElapsedTime");
+    });
+}
+
+// ---------------------------------------------------------------- memory
+int tf_alloc(int device, int strategy, size_t bytes, void** ptr) {
+    return guard([&] {
+        require(ptr != nullptr, "null pointer");
+        *ptr = nullptr;
+        if (!bytes) return;
+        DeviceScope scope(device);
+        switch (strategy) {
+        case TF_MEM_GLOBAL: check(hipMalloc(ptr, bytes), "hipMalloc"); break;
+        case TF_MEM_PINNED: check(hipHostMalloc(ptr, bytes, hipHostMallocDefault), "hipHostMalloc"); break;
+        case TF_MEM_MAPPED: check(hipHostMalloc(ptr, bytes, hipHostMallocMapped), "hipHostMalloc mapped"); break;
+        case TF_MEM_MANAGED: check(hipMallocManaged(ptr, bytes, hipMemAttachGlobal), "hipMallocManaged"); break;
+        default: throw Error(TF_E_INVALID, "unknown memory strategy");
+        }
+    });
+}
+
+int tf_free(int strategy, void* ptr) {
+    return guard([&] {
+        if (!ptr) return;
+        if (strategy == TF_MEM_PINNED || strategy == TF_MEM_MAPPED)
+            check(hipHostFree(ptr), "hipHostFree");
+        else
+            check(hipFree(ptr), "hipFree");
+    });
+}
+
+int tf_device_view(int strategy, void* ptr, void** device_ptr) {
+    return guard([&] {
+        require(device_ptr != nullptr, "null pointer");
+        switch (strategy) {
+        case TF_MEM_GLOBAL:
+        case TF_MEM_MANAGED: *device_ptr = ptr; break;
+        case TF_MEM_MAPPED:
+            *device_ptr = nullptr;
+            if (ptr) check(hipHostGetDevicePointer(device_ptr, ptr, 0), "hipHostGetDevicePointer");
+            break;
+        case TF_MEM_PINNED: throw Error(TF_E_INVALID, "Pinned memory has no device view");
+        default: throw Error(TF_E_INVALID, "unknown memory strategy");
+        }
+    });
+}
+
+int tf_memcpy(void* dst, const void* src, size_t bytes) {
+    return guard([&] {
+        if (bytes) check(hipMemcpy(dst, src, bytes, hipMemcpyDefault), "hipMemcpy");
+    });
+}
+
+int tf_memcpy_async(tf_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    TF_CTX_GUARD(ctx, if (bytes) check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream), "hipMemcpyAsync"));
+}
+
+int tf_fill_zero(tf_ctx* ctx, void* dst, size_t bytes) { TF_CTX_GUARD(ctx, fill_zero(*ctx, dst, bytes)); }
+
+// ---------------------------------------------------------------- kernels
+#define TF_TYPED(SUF, T)                                                                                             \
+    int tf_spmv_csr_##SUF(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const T* x,     \
+                          T* y) {                                                                                    \
+        TF_CTX_GUARD(ctx, require(x != y || n == 0, "SpMV shape or alias mismatch");                                \
+                     spmv<T>(*ctx, n, nnz, rp, ci, v, x, y));                                                        \
+    }                                                                                                                \
+    int tf_residual_csr_##SUF(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const T* b, \
+                              const T* x, T* r) {                                                                    \
+        TF_CTX_GUARD(ctx, require((r != b && r != x) || n == 0, "Residual shape or alias mismatch");                \
+                     residual<T>(*ctx, n, nnz, rp, ci, v, b, x, r));                                                 \
+    }                                                                                                                \
+    int tf_spmm_csr_##SUF(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const T* v, int k,          \
+                          const T* X, T* Y) {                                                                        \
+        TF_CTX_GUARD(ctx, require(k >= 0 && (X != Y || n == 0), "SpMM shape or alias mismatch");                    \
+                     spmm<T>(*ctx, n, nnz, rp, ci, v, k, X, Y));                                                     \
+    }                                                                                                                \
+    int tf_jacobi_update_##SUF(tf_ctx* ctx, int n, T w, const T* inv, const T* b, const T* product, T* x) {          \
+        TF_CTX_GUARD(ctx, jacobi_update<T>(*ctx, n, w, inv, b, product, x));                                        \
+    }                                                                                                                \
+    int tf_jacobi_fused_##SUF(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const T* v,             \
+                              const T* inv, T w, const T* b, const T* x_in, T* x_out) {                              \
+        TF_CTX_GUARD(ctx, require(x_in != x_out, "fused Jacobi needs distinct input and output iterates");          \
+                     jacobi_fused<T>(*ctx, n, nnz, rp, ci, v, inv, w, b, x_in, x_out));                             \
+    }                                                                                                                \
+    int tf_inverse_diagonal_##SUF(tf_ctx* ctx, int n, const int* rp, const int* ci, const T* v, T* inv,            \
+                                  int* host_bad) {                                                                   \
+        TF_CTX_GUARD(ctx, ctx->clear_flags(1); inverse_diagonal<T>(*ctx, n, rp, ci, v, inv, ctx->dflags);          \
+                     ctx->read_flags(1); if (host_bad) *host_bad = ctx->hflags[0]);                                  \
+    }                                                                                                                \
+    int tf_axpy_##SUF(tf_ctx* ctx, int n, T alpha, const T* x, T* y) {                                               \
+        TF_CTX_GUARD(ctx, axpy<T>(*ctx, n, alpha, x, y));                                                           \
+    }                                                                                                                \
+    int tf_scal_##SUF(tf_ctx* ctx, int n, T alpha, T* x) { TF_CTX_GUARD(ctx, scal<T>(*ctx, n, alpha, x)); }          \
+    int tf_dot_##SUF(tf_ctx* ctx, int n, const T* x, const T* y, T* host_result) {                                   \
+        TF_CTX_GUARD(ctx, *host_result = T(host_dot<T>(*ctx, n, x, y)));                                            \
+    }                                                                                                                \
+    int tf_nrm2_##SUF(tf_ctx* ctx, int n, const T* x, T* host_result) {                                             \
+        TF_CTX_GUARD(ctx, *host_result = T(std::sqrt(host_dot<T>(*ctx, n, x, x))));                                 \
+    }                                                                                                                \
+    int tf_restrict_##SUF(tf_ctx* ctx, int nc, const int* offsets, const int* members, const T* fine, T* coarse) {   \
+        TF_CTX_GUARD(ctx, require(fine != coarse, "Invalid aggregate transfer vectors");                            \
+                     restrict_agg<T>(*ctx, nc, offsets, members, fine, coarse));                                     \
+    }                                                                                                                \
+    int tf_prolong_##SUF(tf_ctx* ctx, int n, const int* map, const T* coarse, T* fine) {                             \
+        TF_CTX_GUARD(ctx, require(fine != coarse, "Invalid aggregate transfer vectors");                            \
+                     prolong_agg<T>(*ctx, n, map, coarse, fine, false));                                             \
+    }                                                                                                                \
+    int tf_coarse_lu_solve_##SUF(tf_ctx* ctx, int n, const double* lu, const int* pivots, const T* b, T* x) {        \
+        TF_CTX_GUARD(ctx, coarse_lu_solve<T>(*ctx, n, lu, pivots, b, x));                                           \
+    }                                                                                                                \
+    int tf_coarse_inv_apply_##SUF(tf_ctx* ctx, int n, const double* inverse, const T* b, T* x) {                     \
+        TF_CTX_GUARD(ctx, coarse_inv_apply<T>(*ctx, n, inverse, b, x));                                             \
+    }
+
+TF_TYPED(f64, double)
+TF_TYPED(f32, float)
+#undef TF_TYPED
+
+int tf_convert_f64_f32(tf_ctx* ctx, size_t n, const double* in, float* out) {
+    TF_CTX_GUARD(ctx, (convert<double, float>(*ctx, n, in, out)));
+}
+int tf_convert_f32_f64(tf_ctx* ctx, size_t n, const float* in, double* out) {
+    TF_CTX_GUARD(ctx, (convert<float, double>(*ctx, n, in, out)));
+}
+int tf_convert_i64_i32(tf_ctx* ctx, size_t n, const int64_t* in, int* out) {
+    TF_CTX_GUARD(ctx, (convert<int64_t, int>(*ctx, n, in, out)));
+}
+int tf_convert_i32_i64(tf_ctx* ctx, size_t n, const int* in, int64_t* out) {
+    TF_CTX_GUARD(ctx, (convert<int, int64_t>(*ctx, n, in, out)));
+}
+
+int tf_scan_excl_i32(tf_ctx* ctx, int n, const int* in, int* out) { TF_CTX_GUARD(ctx, scan_excl(*ctx, n, in, out)); }
+
+int tf_csr_check(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v, int* host_flags) {
+    TF_CTX_GUARD(ctx, *host_flags = csr_check(*ctx, n, nnz, rp, ci, v));
+}
+
+int tf_csr_validate_structure(tf_ctx* ctx, int rows, int cols, int64_t nnz, const int* rp, const int* ci,
+                              int* host_bad) {
+    TF_CTX_GUARD(ctx, *host_bad = csr_structure_ok(*ctx, rows, cols, nnz, rp, ci) ? 0 : 1);
+}
+
+int tf_csr_canonicalize(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v, int* out_rp,
+                        int** out_ci, double** out_v, int64_t* out_nnz) {
+    TF_CTX_GUARD(ctx, {
+        DevCsr a = canonicalize(*ctx, n, nnz, rp, ci, v);
+        check(hipMemcpyAsync(out_rp, a.rp, (size_t(n) + 1) * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream),
+              "canonical offsets");
+        ctx->sync();
+        (void)hipFree(a.rp);
+        *out_ci = a.ci;
+        *out_v = a.v;
+        *out_nnz = a.nnz;
+    });
+}
+
+int tf_agg_validate_rowmax(tf_ctx* ctx, int n, const int* rp, const int* ci, const double* v, double* rowmax,
+                           int* host_bad) {
+    TF_CTX_GUARD(ctx, *host_bad = validate_rowmax(*ctx, n, rp, ci, v, rowmax) ? 0 : 1);
+}
+
+int tf_agg_strength(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta,
+                    int** g_rp, int** g_ci, double** g_w, int64_t* g_nnz) {
+    TF_CTX_GUARD(ctx, {
+        DevCsr g = strength_graph(*ctx, n, nnz, rp, ci, v, theta);
+        *g_rp = g.rp;
+        *g_ci = g.ci;
+        *g_w = g.v;
+        *g_nnz = g.nnz;
+    });
+}
+
+int tf_agg_match_round(tf_ctx* ctx, int n, const int* g_rp, const int* g_ci, const double* g_w, int* mate,
+                       int* proposal) {
+    TF_CTX_GUARD(ctx, match_round(*ctx, n, g_rp, g_ci, g_w, mate, proposal));
+}
+
+int tf_agg_pair(tf_ctx* ctx, int n, const int* g_rp, const int* g_ci, const double* g_w, int rounds, int* map,
+                int* host_count) {
+    TF_CTX_GUARD(ctx, require(rounds >= 1 && rounds <= 8 && n >= 1, "Invalid matching configuration");
+                 *host_count = pair(*ctx, n, g_rp, g_ci, g_w, rounds, map));
+}
+
+int tf_agg_contract(tf_ctx* ctx, int n, const int* g_rp, const int* g_ci, const double* g_w, const int* map, int nc,
+                    int** c_rp, int** c_ci, double** c_w, int64_t* c_nnz) {
+    TF_CTX_GUARD(ctx, {
+        DevCsr g = contract_graph(*ctx, n, g_rp, g_ci, g_w, map, nc);
+        *c_rp = g.rp;
+        *c_ci = g.ci;
+        *c_w = g.v;
+        *c_nnz = g.nnz;
+    });
+}
+
+int tf_agg_aggregate(tf_ctx* ctx, int n, const int* g_rp, const int* g_ci, const double* g_w, int rounds, int absorb,
+                     int** map, int** offsets, int** members, int* host_coarse_rows, int64_t* host_histogram) {
+    TF_CTX_GUARD(ctx, {
+        DevCsr g;
+        g.n = n;
+        g.rp = const_cast<int*>(g_rp);
+        g.ci = const_cast<int*>(g_ci);
+        g.v = const_cast<double*>(g_w);
+        DevAgg t = aggregate(*ctx, g, rounds, absorb != 0);
+        *map = t.map;
+        *offsets = t.offsets;
+        *members = t.members;
+        *host_coarse_rows = t.nc;
+        if (host_histogram)
+            for (int k = 0; k < 4; ++k) host_histogram[k] = t.hist[k];
+    });
+}
+
+int tf_galerkin(tf_ctx* ctx, int n, const int* rp, const int* ci, const double* v, const int* map, int nc,
+                const int* offsets, const int* members, int** c_rp, int** c_ci, double** c_v, int64_t* c_nnz) {
+    TF_CTX_GUARD(ctx, {
+        DevCsr a;
+        a.n = n;
+        a.rp = const_cast<int*>(rp);
+        a.ci = const_cast<int*>(ci);
+        a.v = const_cast<double*>(v);
+        DevCsr out = galerkin(*ctx, a, map, nc, offsets, members);
+        *c_rp = out.rp;
+        *c_ci = out.ci;
+        *c_v = out.v;
+        *c_nnz = out.nnz;
+    });
+}
+
+int tf_coarse_lu_factor_f64(tf_ctx* ctx, int n, const int* rp, const int* ci, const double* v, double* lu,
+                            int* pivots, double* inverse, int* host_singular) {
+    TF_CTX_GUARD(ctx, {
+        const bool ok = coarse_lu_factor(*ctx, n, rp, ci, v, lu, pivots, inverse);
+        ctx->sync();
+        *host_singular = ok ? 0 : 1;
+    });
+}
+
+// ---------------------------------------------------------------- AMG
+int tf_amg_create(int device, const tf_amg_options* options, tf_amg** amg) {
+    return guard([&] {
+        require(amg != nullptr, "null handle pointer");
+        const AmgOptionsC o = amg_options_from(options);
+        auto h = std::make_unique<tf_amg>();
+        h->ctx = make_ctx(device);
+        h->eng = new AmgEngine(h->ctx, o);
+        *amg = h.release();
+    });
+}
+
+int tf_amg_destroy(tf_amg* amg) {
+    return guard([&] {
+        if (!amg) return;
+        {
+            DeviceScope scope(amg->ctx->device);
+            delete amg->eng;
+        }
+        destroy_ctx(amg->ctx);
+        delete amg;
+    });
+}
+
+int tf_amg_setup(tf_amg* amg, int n, int64_t nnz, const int* rp, const int* ci, const double* v, int host_memory) {
+    return guard([&] {
+        require(amg != nullptr, "null handle");
+        require(n >= 1, "Solver requires a nonempty square matrix");
+        DeviceScope scope(amg->ctx->device);
+        if (host_memory) {
+            Staged s;
+            stage(*amg->ctx, n, nnz, rp, ci, v, s);
+            amg->eng->setup(n, nnz, s.rp, s.ci, s.v);
+            amg->ctx->sync();
+        }
+        else {
+            amg->eng->setup(n, nnz, rp, ci, v);
+        }
+    });
+}
+
+int tf_amg_apply(tf_amg* amg, const double* rhs, double* x, int host_memory) {
+    return guard([&] {
+        require(amg != nullptr && amg->eng->ready(), "AMG requires setup");
+        DeviceScope scope(amg->ctx->device);
+        const int n = amg->eng->n();
+        if (host_memory) {
+            double* d = dev_alloc<double>(size_t(n) * 2);
+            std::unique_ptr<double, void (*)(double*)> hold(d, [](double* p) { (void)hipFree(p); });
+            check(hipMemcpyAsync(d, rhs, size_t(n) * sizeof(double), hipMemcpyHostToDevice, amg->ctx->stream),
+                  "upload rhs");
+            amg->eng->apply(d, d + n);
+            check(hipMemcpyAsync(x, d + n, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, amg->ctx->stream),
+                  "download x");
+            amg->ctx->sync();
+        }
+        else {
+            amg->eng->apply(rhs, x);
+            amg->ctx->sync();
+        }
+    });
+}
+
+int tf_amg_level_count(tf_amg* amg) { return amg && amg->eng ? int(amg->eng->levels_.size()) : 0; }
+
+int tf_amg_level_info(tf_amg* amg, int level, int64_t* rows, int64_t* nnz, int64_t* histogram) {
+    return guard([&] {
+        require(amg && level >= 0 && level < int(amg->eng->levels_.size()), "level out of range");
+        const auto& l = amg->eng->levels_[size_t(level)];
+        *rows = l.native.n;
+        *nnz = l.native.nnz;
+        if (histogram)
+            for (int k = 0; k < 4; ++k) histogram[k] = l.hist[k];
+    });
+}
+
+int tf_amg_level_matrix(tf_amg* amg, int level, int* host_rp, int* host_ci, double* host_v) {
+    return guard([&] {
+        require(amg && level >= 0 && level < int(amg->eng->levels_.size()), "level out of range");
+        DeviceScope scope(amg->ctx->device);
+        const auto& a = amg->eng->levels_[size_t(level)].native;
+        check(hipMemcpy(host_rp, a.rp, (size_t(a.n) + 1) * sizeof(int), hipMemcpyDeviceToHost), "download rp");
+        if (a.nnz) {
+            check(hipMemcpy(host_ci, a.ci, size_t(a.nnz) * sizeof(int), hipMemcpyDeviceToHost), "download ci");
+            check(hipMemcpy(host_v, a.v, size_t(a.nnz) * sizeof(double), hipMemcpyDeviceToHost), "download v");
+        }
+    });
+}
+
+int tf_amg_level_transfer(tf_amg* amg, int level, int* host_map, int* host_offsets, int* host_members) {
+    return guard([&] {
+        require(amg && level >= 0 && level + 1 < int(amg->eng->levels_.size()), "level has no transfer");
+        DeviceScope scope(amg->ctx->device);
+        const auto& t = amg->eng->levels_[size_t(level)].transfer;
+        check(hipMemcpy(host_map, t.map, size_t(t.n) * sizeof(int), hipMemcpyDeviceToHost), "download map");
+        check(hipMemcpy(host_offsets, t.offsets, (size_t(t.nc) + 1) * sizeof(int), hipMemcpyDeviceToHost),
+              "download offsets");
+        check(hipMemcpy(host_members, t.members, size_t(t.n) * sizeof(int), hipMemcpyDeviceToHost),
+              "download members");
+    });
+}
+
+const char* tf_amg_stop_reason(tf_amg* amg) { return amg && amg->eng ? amg->eng->stop_.c_str() : ""; }
+const char* tf_amg_coarse_solver(tf_amg* amg) { return amg && amg->eng ? amg->eng->coarseSolver_.c_str() : ""; }
+
+int tf_amg_visits(tf_amg* amg, int64_t* visits) {
+    return guard([&] {
+        require(amg != nullptr, "null handle");
+        for (size_t i = 0; i < amg->eng->visits_.size(); ++i) visits[i] = amg->eng->visits_[i];
+    });
+}
+
+// ---------------------------------------------------------------- solvers
+int tf_solver_create(int device, int kind, const tf_krylov_options* options, const tf_amg_options* amg_options,
+                     tf_solver** solver) {
+    return guard([&] {
+        require(solver != nullptr, "null handle pointer");
+        KrylovOptionsC o;
+        if (options) {
+            o.maxit = options->max_iterations;
+            o.rtol = options->relative_tolerance;
+            o.atol = options->absolute_tolerance;
+            o.refresh = options->residual_refresh_interval;
+            o.graph = options->use_graph != 0;
+        }
+        AmgOptionsC ao;
+        if (amg_options) ao = amg_options_from(amg_options);
+        auto h = std::make_unique<tf_solver>();
+        h->ctx = make_ctx(device);
+        try {
+            DeviceScope scope(device);
+            if (amg_options) h->amg = new AmgEngine(h->ctx, ao);
+            h->eng = new SolverEngine(h->ctx, kind, o, h->amg);
+        }
+        catch (...) {
+            delete h->amg;
+            destroy_ctx(h->ctx);
+            throw;
+        }
+        h->view.ctx = h->ctx;
+        h->view.eng = h->amg;
+        *solver = h.release();
+    });
+}
+
+int tf_solver_destroy(tf_solver* s) {
+    return guard([&] {
+        if (!s) return;
+        {
+            DeviceScope scope(s->ctx->device);
+            delete s->eng;
+            delete s->amg;
+        }
+        destroy_ctx(s->ctx);
+        delete s;
+    });
+}
+
+namespace {
+double elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    return double(ms) * 1e-3;
+}
+struct Events {
+    hipEvent_t a = nullptr, b = nullptr;
+    Events() {
+        check(hipEventCreate(&a), "hipEventCreate");
+        check(hipEventCreate(&b), "hipEventCreate");
+    }
+    ~Events() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+}  // namespace
+
+int tf_solver_setup(tf_solver* s, int n, int64_t nnz, const int* rp, const int* ci, const double* v, int host_memory) {
+    return guard([&] {
+        require(s != nullptr, "null handle");
+        require(n >= 1, "Solver requires a nonempty square matrix");
+        DeviceScope scope(s->ctx->device);
+        Staged st;
+        if (host_memory) {
+            stage(*s->ctx, n, nnz, rp, ci, v, st);
+            rp = st.rp;
+            ci = st.ci;
+            v = st.v;
+            s->ctx->sync();
+        }
+        Events ev;
+        check(hipEventRecord(ev.a, s->ctx->stream), "event record");
+        s->eng->setup(n, nnz, rp, ci, v);
+        check(hipEventRecord(ev.b, s->ctx->stream), "event record");
+        check(hipEventSynchronize(ev.b), "event sync");
+        s->eng->setupSeconds = elapsed(ev.a, ev.b);
+    });
+}
+
+int tf_solver_apply(tf_solver* s, const double* b, double* x, int host_memory, tf_solve_result* result) {
+    return guard([&] {
+        require(s != nullptr && result != nullptr, "null handle");
+        DeviceScope scope(s->ctx->device);
+        const int n = s->eng->n();
+        require(n > 0, "solver requires setup");
+        Events ev;
+        double* hx = x;
+        std::unique_ptr<double, void (*)(double*)> hold(nullptr, [](double* p) {
+            if (p) (void)hipFree(p);
+        });
+        if (host_memory) {
+            double* d = dev_alloc<double>(size_t(n) * 2);
+            hold.reset(d);
+            check(hipMemcpyAsync(d, b, size_t(n) * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream), "upload b");
+            check(hipMemcpyAsync(d + n, x, size_t(n) * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream),
+                  "upload x");
+            b = d;
+            x = d + n;
+        }
+        check(hipEventRecord(ev.a, s->ctx->stream), "event record");
+        *result = s->eng->apply(b, x);
+        check(hipEventRecord(ev.b, s->ctx->stream), "event record");
+        check(hipEventSynchronize(ev.b), "event sync");
+        s->eng->applySeconds = elapsed(ev.a, ev.b);
+        if (host_memory) {
+            check(hipMemcpyAsync(hx, x, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream),
+                  "download x");
+            s->ctx->sync();
+        }
+    });
+}
+
+tf_amg* tf_solver_amg(tf_solver* s) { return s && s->amg ? &s->view : nullptr; }
+
+int tf_solver_timings(tf_solver* s, double* setup_seconds, double* apply_seconds) {
+    return guard([&] {
+        require(s != nullptr, "null handle");
+        if (setup_seconds) *setup_seconds = s->eng->setupSeconds;
+        if (apply_seconds) *apply_seconds = s->eng->applySeconds;
+    });
+}
+
+}  // extern "C"

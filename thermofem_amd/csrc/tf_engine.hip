@@ -1,0 +1,822 @@
+// AMG hierarchy + cycle (Amg.cuh:81-161, Cycle.cuh:35-63) and the CG /
+// BiCGStab iterations (Cg.cuh:123-189, BiCgStab.cuh:144-247) on gfx950.
+//
+// The Krylov scalars (rho, alpha, beta, omega, norms) never leave the
+// device: each reduction kernel's last block evaluates the reference's
+// scalar recurrences and breakdown/non-finite tests on a control block, and
+// later kernels of the iteration read it.  One iteration is therefore a
+// fixed launch sequence that is captured once as a hipGraph and replayed;
+// the host reads the control block once per iteration to stop.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "tf_device.hpp"
+#include "tf_engine.hpp"
+
+namespace tf {
+
+// ---------------------------------------------------------------- control
+
+struct KCtl {
+    double normb, tol, res, rho, rho_prev, alpha, beta, omega, den, ns;
+    int status;  // -1 running, otherwise the status handed to finish()
+    int iter;
+    int fresh;
+    int check;  // true-residual refresh requested
+    int early;  // BiCGStab: ||s|| <= tol in this iteration
+    int refresh;
+};
+
+enum { kRunning = -1, kConverged = 0, kMaxIt = 1, kBreakdown = 2, kNonFinite = 3 };
+
+namespace {
+
+constexpr int kSlotA = 40, kSlotB = 41, kSlotC = 42, kSlotD = 43, kSlotE = 44;
+
+int kgroup(int n, int64_t nnz) {
+    const double avg = n > 0 ? double(nnz) / double(n) : 1.0;
+    int g = 2;
+    while (g < 64 && g < avg) g <<= 1;
+    return g;
+}
+
+template <class T>
+__device__ __forceinline__ T group_row_dot(int s, int e, int lane, int G, const int* __restrict__ ci,
+                                           const T* __restrict__ v, const T* __restrict__ x) {
+    T acc = 0;
+    for (int j = s + lane; j < e; j += G) acc += ld_stream(v + j) * x[ld_stream(ci + j)];
+    return acc;
+}
+
+// Persistent row loop for the fused reduction kernels: the grid is capped at
+// kMaxReduceBlocks (one partial per block) and each XCD walks one contiguous
+// slab of logical row blocks.
+template <int G, class F>
+__device__ __forceinline__ void for_rows_xcd(int n, int nblocks, F&& f) {
+    const int per = int(gridDim.x) / kXcds;
+    const int xcd = int(blockIdx.x % kXcds), idx = int(blockIdx.x / kXcds);
+    const int chunk = (nblocks + kXcds - 1) / kXcds;
+    const int end = min(nblocks, (xcd + 1) * chunk);
+    constexpr int R = kBlock / G;
+    const int lane = int(threadIdx.x) % G;
+    for (int lb = xcd * chunk + idx; lb < end; lb += per) {
+        const int row = lb * R + int(threadIdx.x) / G;
+        if (row < n) f(row, lane);
+    }
+}
+
+// Finalisers run by thread 0 of the last block of a reduction.
+enum Fin { kFinRhoCg, kFinCurvCg, kFinUpdCg, kFinTrue, kFinRhoBi, kFinDenBi, kFinSBi, kFinUpdBi };
+
+__device__ void finalize(int mode, double total, KCtl* c) {
+    switch (mode) {
+    case kFinRhoCg: {  // Cg.cuh:139-160
+        c->iter += 1;
+        const double rho = total;
+        c->rho = rho;
+        if (!isfinite(rho)) c->status = kNonFinite;
+        else if (rho <= 0) c->status = kBreakdown;
+        else if (!c->fresh) {
+            c->beta = rho / c->rho_prev;
+            if (!isfinite(c->beta)) c->status = kNonFinite;
+        }
+        break;
+    }
+    case kFinCurvCg: {  // Cg.cuh:161-172
+        const double curv = total;
+        if (!isfinite(curv)) c->status = kNonFinite;
+        else if (curv <= 0) c->status = kBreakdown;
+        else {
+            c->alpha = c->rho / curv;
+            if (!isfinite(c->alpha)) c->status = kNonFinite;
+        }
+        c->fresh = 0;
+        break;
+    }
+    case kFinUpdCg: {  // Cg.cuh:173-187
+        const double res = sqrt(total);
+        c->res = res;
+        if (!isfinite(res)) c->status = kNonFinite;
+        else {
+            c->check = (res <= c->tol || c->iter % c->refresh == 0) ? 1 : 0;
+            c->rho_prev = c->rho;
+        }
+        break;
+    }
+    case kFinTrue: {  // true-residual refresh, Cg.cuh:178-185 / BiCgStab.cuh:202-211,235-243
+        const double res = sqrt(total);
+        c->res = res;
+        c->check = 0;
+        if (!isfinite(res) || res <= c->tol) c->status = kMaxIt;
+        else c->fresh = 1;
+        break;
+    }
+    case kFinRhoBi: {  // BiCgStab.cuh:160-180
+        c->iter += 1;
+        c->early = 0;
+        const double rho = total;
+        c->rho = rho;
+        if (!isfinite(rho)) c->status = kNonFinite;
+        else if (rho == 0) c->status = kBreakdown;
+        else if (!c->fresh) {
+            c->beta = (rho / c->rho_prev) * (c->alpha / c->omega);
+            if (!isfinite(c->beta)) c->status = kNonFinite;
+        }
+        break;
+    }
+    case kFinDenBi: {  // BiCgStab.cuh:182-195
+        const double den = total;
+        c->den = den;
+        if (!isfinite(den)) c->status = kNonFinite;
+        else if (den == 0) c->status = kBreakdown;
+        else {
+            c->alpha = c->rho / den;
+            if (!isfinite(c->alpha)) c->status = kNonFinite;
+        }
+        c->fresh = 0;
+        break;
+    }
+    case kFinSBi: {  // BiCgStab.cuh:196-212
+        const double ns = sqrt(total);
+        c->ns = ns;
+        if (!isfinite(ns)) c->status = kNonFinite;
+        else c->early = ns <= c->tol ? 1 : 0;
+        break;
+    }
+    case kFinUpdBi: {  // BiCgStab.cuh:229-245
+        const double res = sqrt(total);
+        c->res = res;
+        if (!isfinite(res)) c->status = kNonFinite;
+        else {
+            c->check = (res <= c->tol || c->iter % c->refresh == 0) ? 1 : 0;
+            c->rho_prev = c->rho;
+        }
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+__device__ __forceinline__ bool stopped(const KCtl* c) { return c->status != kRunning; }
+
+// Reduce `contrib` over the grid and run finalizer `mode`.
+__device__ __forceinline__ void reduce_fin(double contrib, double* partials, unsigned* ticket, KCtl* ctl, int mode) {
+    contrib = block_sum(contrib);
+    double total = 0;
+    if (grid_sum(contrib, partials, ticket, &total) && threadIdx.x == 0) finalize(mode, total, ctl);
+}
+
+// dot(x, y) -> finalizer.
+__global__ __launch_bounds__(kBlock) void k_kdot(int n, const double* __restrict__ x, const double* __restrict__ y,
+                                                 double* partials, unsigned* ticket, KCtl* ctl, int mode) {
+    if (stopped(ctl)) return;
+    double s = 0;
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) s += x[i] * y[i];
+    reduce_fin(s, partials, ticket, ctl, mode);
+}
+
+// y = A x fused with sum(w .* y) -> finalizer; skip when `early` is set if skipEarly.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_kspmv_dot(int n, int nblocks, const int* __restrict__ rp,
+                                                      const int* __restrict__ ci, const double* __restrict__ v,
+                                                      const double* __restrict__ x, double* __restrict__ y,
+                                                      const double* __restrict__ w, double* partials,
+                                                      unsigned* ticket, KCtl* ctl, int mode) {
+    if (stopped(ctl)) return;
+    double contrib = 0;
+    for_rows_xcd<G>(n, nblocks, [&](int row, int lane) {
+        double acc = group_row_dot(rp[row], rp[row + 1], lane, G, ci, v, x);
+        acc = group_sum<G>(acc);
+        if (lane == 0) {
+            y[row] = acc;
+            contrib += w[row] * acc;
+        }
+    });
+    reduce_fin(contrib, partials, ticket, ctl, mode);
+}
+
+// t = A x fused with (t.t, t.s): BiCgStab.cuh:214-226 (omega).
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_kspmv_omega(int n, int nblocks, const int* __restrict__ rp,
+                                                        const int* __restrict__ ci, const double* __restrict__ v,
+                                                        const double* __restrict__ x, double* __restrict__ t,
+                                                        const double* __restrict__ s, double* partials,
+                                                        unsigned* tickets, KCtl* ctl) {
+    if (stopped(ctl) || ctl->early) return;
+    double tt = 0, ts = 0;
+    for_rows_xcd<G>(n, nblocks, [&](int row, int lane) {
+        double acc = group_row_dot(rp[row], rp[row + 1], lane, G, ci, v, x);
+        acc = group_sum<G>(acc);
+        if (lane == 0) {
+            t[row] = acc;
+            tt += acc * acc;
+            ts += acc * s[row];
+        }
+    });
+    tt = block_sum(tt);
+    ts = block_sum(ts);
+    // Two reductions share one ticket: partials for ts live after those for tt.
+    __shared__ int isLast;
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = tt;
+        partials[kMaxReduceBlocks + blockIdx.x] = ts;
+        const unsigned k = __hip_atomic_fetch_add(tickets, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        isLast = (k == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!isLast) return;
+    double a = 0, b = 0;
+    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+        a += __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        b += __hip_atomic_load(partials + kMaxReduceBlocks + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    a = block_sum(a);
+    b = block_sum(b);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(tickets, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!isfinite(a) || !isfinite(b)) ctl->status = kNonFinite;
+        else if (a == 0) ctl->status = kBreakdown;
+        else {
+            const double omega = b / a;
+            ctl->omega = omega;
+            if (!isfinite(omega)) ctl->status = kNonFinite;
+            else if (omega == 0) ctl->status = kBreakdown;
+        }
+    }
+}
+
+// r = b - A x and sum(r.^2) when the control block asks for it; with `shadow`
+// the new residual is also copied into the shadow vector (BiCGStab restart).
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_ktrue(int n, int nblocks, const int* __restrict__ rp,
+                                                  const int* __restrict__ ci, const double* __restrict__ v,
+                                                  const double* __restrict__ b, const double* __restrict__ x,
+                                                  double* __restrict__ r, double* __restrict__ shadow,
+                                                  double* partials, unsigned* ticket, KCtl* ctl, int onEarly) {
+    if (stopped(ctl)) return;
+    if (onEarly ? !ctl->early : (!ctl->check || ctl->early)) return;
+    double contrib = 0;
+    for_rows_xcd<G>(n, nblocks, [&](int row, int lane) {
+        double acc = group_row_dot(rp[row], rp[row + 1], lane, G, ci, v, x);
+        acc = group_sum<G>(acc);
+        if (lane == 0) {
+            const double res = b[row] - acc;
+            r[row] = res;
+            if (shadow) shadow[row] = res;
+            contrib += res * res;
+        }
+    });
+    reduce_fin(contrib, partials, ticket, ctl, kFinTrue);
+}
+
+// CG direction: p = z (fresh) or p = z + beta p.
+__global__ void k_cg_p(int n, const double* __restrict__ z, double* __restrict__ p, const KCtl* ctl) {
+    if (stopped(ctl)) return;
+    const int fresh = ctl->fresh;
+    const double beta = ctl->beta;
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock)
+        p[i] = fresh ? z[i] : z[i] + beta * p[i];
+}
+
+// x += alpha p; r -= alpha q; sum(r.^2).
+__global__ __launch_bounds__(kBlock) void k_cg_update(int n, const double* __restrict__ p,
+                                                      const double* __restrict__ q, double* __restrict__ x,
+                                                      double* __restrict__ r, double* partials, unsigned* ticket,
+                                                      KCtl* ctl) {
+    if (stopped(ctl)) return;
+    const double alpha = ctl->alpha;
+    double s = 0;
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
+        x[i] = alpha * p[i] + x[i];
+        const double ri = -alpha * q[i] + r[i];
+        r[i] = ri;
+        s += ri * ri;
+    }
+    reduce_fin(s, partials, ticket, ctl, kFinUpdCg);
+}
+
+// BiCGStab direction: p = r (fresh) or p = r + beta (p - omega v).
+__global__ void k_bi_p(int n, const double* __restrict__ r, const double* __restrict__ v, double* __restrict__ p,
+                       const KCtl* ctl) {
+    if (stopped(ctl)) return;
+    const int fresh = ctl->fresh;
+    const double beta = ctl->beta, omega = ctl->omega;
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock)
+        p[i] = fresh ? r[i] : 1.0 * r[i] + beta * (-omega * v[i] + p[i]);
+}
+
+// s = r - alpha v; x += alpha ph; sum(s.^2).
+__global__ __launch_bounds__(kBlock) void k_bi_s(int n, const double* __restrict__ r, const double* __restrict__ v,
+                                                 const double* __restrict__ ph, double* __restrict__ s,
+                                                 double* __restrict__ x, double* partials, unsigned* ticket,
+                                                 KCtl* ctl) {
+    if (stopped(ctl)) return;
+    const double alpha = ctl->alpha;
+    double acc = 0;
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
+        const double si = -alpha * v[i] + r[i];
+        s[i] = si;
+        x[i] = alpha * ph[i] + x[i];
+        acc += si * si;
+    }
+    reduce_fin(acc, partials, ticket, ctl, kFinSBi);
+}
+
+// x += omega shat; r = s - omega t; sum(r.^2).
+__global__ __launch_bounds__(kBlock) void k_bi_update(int n, const double* __restrict__ shat,
+                                                      const double* __restrict__ s, const double* __restrict__ t,
+                                                      double* __restrict__ x, double* __restrict__ r,
+                                                      double* partials, unsigned* ticket, KCtl* ctl) {
+    if (stopped(ctl) || ctl->early) return;
+    const double omega = ctl->omega;
+    double acc = 0;
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
+        x[i] = omega * shat[i] + x[i];
+        const double ri = -omega * t[i] + s[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    reduce_fin(acc, partials, ticket, ctl, kFinUpdBi);
+}
+
+int vec_blocks(int n) {
+    int b = (n + kBlock - 1) / kBlock;
+    if (b > 1024) b = 1024;
+    return b < 1 ? 1 : b;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ AMG
+
+AmgOptionsC amg_options_from(const tf_amg_options* o) {
+    AmgOptionsC a;
+    if (o) {
+        a.maxLevels = o->max_levels;
+        a.coarseRows = o->coarse_rows;
+        a.rounds = o->matching_rounds;
+        a.pre = o->pre_sweeps;
+        a.post = o->post_sweeps;
+        a.coarseSweeps = o->coarse_sweeps;
+        a.omega = o->jacobi_weight;
+        a.theta = o->strength_threshold;
+        a.absorb = o->absorb_singletons != 0;
+        a.cycle = o->cycle;
+        a.precision = o->level_precision;
+    }
+    // Amg.cuh:66-75 (coarse_rows widened to 1..1024).
+    require(a.maxLevels >= 1 && a.maxLevels <= 20 && a.coarseRows >= 1 && a.coarseRows <= 1024 && a.rounds >= 1 &&
+                a.rounds <= 8 && a.pre >= 0 && a.post >= 0 && a.coarseSweeps >= 1 && std::isfinite(a.theta) &&
+                a.theta >= 0 && a.theta <= 1 && std::isfinite(a.omega) && a.omega > 0 && a.omega < 2 &&
+                a.cycle >= 0 && a.cycle <= 2 && (a.precision == 32 || a.precision == 64),
+            "Invalid AMG options");
+    return a;
+}
+
+AmgEngine::AmgEngine(tf_ctx* ctx, const AmgOptionsC& o) : opt_(o), c_(ctx) {}
+AmgEngine::~AmgEngine() { release(); }
+
+namespace {
+void free_level(AmgLevel& l, bool f32) {
+    if (f32 && l.vals) (void)hipFree(l.vals);
+    l.vals = nullptr;
+    for (void** p : {&l.inv, &l.b, &l.x, &l.x2, &l.r})
+        if (*p) {
+            (void)hipFree(*p);
+            *p = nullptr;
+        }
+    free_csr(l.native);
+    free_agg(l.transfer);
+}
+}  // namespace
+
+void AmgEngine::release() {
+    if (c_) (void)hipStreamSynchronize(c_->stream);
+    for (auto& l : levels_) free_level(l, opt_.precision == 32);
+    levels_.clear();
+    for (void* p : {static_cast<void*>(lu_), static_cast<void*>(piv_), static_cast<void*>(inverse_)})
+        if (p) (void)hipFree(p);
+    lu_ = nullptr;
+    piv_ = nullptr;
+    inverse_ = nullptr;
+    factored_ = singular_ = false;
+}
+
+void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v) {
+    const bool f32 = opt_.precision == 32;
+    const size_t es = f32 ? sizeof(float) : sizeof(double);
+    std::vector<AmgLevel> next;
+    std::string stop;
+    double *lu = nullptr, *inv = nullptr;
+    int* piv = nullptr;
+    bool factored = false, singular = false;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(c_->stream);
+        for (auto& l : next) free_level(l, f32);
+        if (lu) (void)hipFree(lu);
+        if (piv) (void)hipFree(piv);
+        if (inv) (void)hipFree(inv);
+    };
+    try {
+        DevCsr native = canonicalize(*c_, n, nnz, rp, ci, v);
+        for (int depth = 0;; ++depth) {
+            next.emplace_back();
+            AmgLevel& L = next.back();
+            L.native = native;
+            const int ln = native.n;
+            if (!validate_rowmax(*c_, ln, native.rp, native.ci, native.v, nullptr))
+                throw Error(TF_E_NUMERIC,
+                            "Native AMG: missing/zero diagonal, non-finite value or reciprocal in hierarchy");
+            if (f32) {
+                L.vals = dev_alloc<float>(size_t(native.nnz));
+                convert<double, float>(*c_, size_t(native.nnz), native.v, static_cast<float*>(L.vals));
+            }
+            else {
+                L.vals = native.v;
+            }
+            L.inv = dev_alloc<char>(size_t(ln) * es);
+            L.b = dev_alloc<char>(size_t(ln) * es);
+            L.x = dev_alloc<char>(size_t(ln) * es);
+            L.x2 = dev_alloc<char>(size_t(ln) * es);
+            L.r = dev_alloc<char>(size_t(ln) * es);
+            c_->clear_flags(1);
+            if (f32)
+                inverse_diagonal<float>(*c_, ln, native.rp, native.ci, static_cast<const float*>(L.vals),
+                                        static_cast<float*>(L.inv), c_->dflags);
+            else
+                inverse_diagonal<double>(*c_, ln, native.rp, native.ci, native.v, static_cast<double*>(L.inv),
+                                         c_->dflags);
+            c_->read_flags(1);
+            if (c_->hflags[0]) throw Error(TF_E_NUMERIC, "Missing or invalid diagonal");
+            if (ln <= opt_.coarseRows) {
+                stop = "coarse size";
+                break;
+            }
+            if (depth + 1 == opt_.maxLevels) {
+                stop = "level limit";
+                break;
+            }
+            DevCsr g = strength_graph(*c_, ln, native.nnz, native.rp, native.ci, native.v, opt_.theta);
+            if (g.nnz == 0) {
+                free_csr(g);
+                stop = "empty strength graph";
+                break;
+            }
+            DevAgg t;
+            try {
+                t = aggregate(*c_, g, opt_.rounds, opt_.absorb);
+            }
+            catch (...) {
+                free_csr(g);
+                throw;
+            }
+            free_csr(g);
+            if (t.nc >= ln) {
+                free_agg(t);
+                stop = "no contraction";
+                break;
+            }
+            L.transfer = t;
+            for (int k = 0; k < 4; ++k) L.hist[k] = t.hist[k];
+            native = galerkin(*c_, L.native, t.map, t.nc, t.offsets, t.members);
+        }
+        const DevCsr& last = next.back().native;
+        if (last.n <= std::max(64, opt_.coarseRows)) {
+            lu = dev_alloc<double>(size_t(last.n) * size_t(last.n));
+            inv = dev_alloc<double>(size_t(last.n) * size_t(last.n));
+            piv = dev_alloc<int>(size_t(last.n));
+            factored = coarse_lu_factor(*c_, last.n, last.rp, last.ci, last.v, lu, piv, inv);
+            singular = !factored;
+        }
+        c_->sync();
+    }
+    catch (...) {
+        cleanup();
+        throw;
+    }
+    release();
+    levels_ = std::move(next);
+    lu_ = lu;
+    piv_ = piv;
+    inverse_ = inv;
+    factored_ = factored;
+    singular_ = singular;
+    stop_ = stop;
+    coarseSolver_ = factored ? "pivoted GPU LU" : (singular ? "Jacobi (singular LU)" : "Jacobi (coarsening stopped)");
+    visits_.assign(levels_.size(), 0);
+}
+
+template <class L>
+void AmgEngine::smooth(size_t d, int sweeps) {
+    AmgLevel& l = levels_[d];
+    const DevCsr& A = l.native;
+    for (int s = 0; s < sweeps; ++s) {
+        if (l.zero) {
+            jacobi_fused<L>(*c_, A.n, A.nnz, A.rp, A.ci, static_cast<const L*>(l.vals), static_cast<const L*>(l.inv),
+                            L(opt_.omega), static_cast<const L*>(l.b), nullptr, static_cast<L*>(l.x));
+            l.zero = false;
+        }
+        else {
+            jacobi_fused<L>(*c_, A.n, A.nnz, A.rp, A.ci, static_cast<const L*>(l.vals), static_cast<const L*>(l.inv),
+                            L(opt_.omega), static_cast<const L*>(l.b), static_cast<const L*>(l.x),
+                            static_cast<L*>(l.x2));
+            std::swap(l.x, l.x2);
+        }
+    }
+}
+
+template <class L>
+void AmgEngine::coarse(size_t d) {
+    AmgLevel& l = levels_[d];
+    if (factored_) {
+        coarse_inv_apply<L>(*c_, l.native.n, inverse_, static_cast<const L*>(l.b), static_cast<L*>(l.x));
+        l.zero = false;
+    }
+    else {
+        smooth<L>(d, opt_.coarseSweeps);
+    }
+}
+
+template <class L>
+void AmgEngine::cycle(int kind, size_t d, bool zero) {
+    ++visits_[d];
+    AmgLevel& l = levels_[d];
+    if (zero || kind == 2) l.zero = true;
+    if (d + 1 == levels_.size()) {
+        coarse<L>(d);
+        return;
+    }
+    AmgLevel& nx = levels_[d + 1];
+    const DevAgg& t = l.transfer;
+    if (kind == 2) {
+        restrict_agg<L>(*c_, t.nc, t.offsets, t.members, static_cast<const L*>(l.b), static_cast<L*>(nx.b));
+        cycle<L>(2, d + 1, true);
+        prolong_agg<L>(*c_, l.native.n, t.map, static_cast<const L*>(nx.x), static_cast<L*>(l.x), l.zero);
+        l.zero = false;
+        cycle<L>(0, d, false);
+        return;
+    }
+    smooth<L>(d, opt_.pre);
+    if (l.zero) {
+        restrict_agg<L>(*c_, t.nc, t.offsets, t.members, static_cast<const L*>(l.b), static_cast<L*>(nx.b));
+    }
+    else {
+        residual<L>(*c_, l.native.n, l.native.nnz, l.native.rp, l.native.ci, static_cast<const L*>(l.vals),
+                    static_cast<const L*>(l.b), static_cast<const L*>(l.x), static_cast<L*>(l.r));
+        restrict_agg<L>(*c_, t.nc, t.offsets, t.members, static_cast<const L*>(l.r), static_cast<L*>(nx.b));
+    }
+    cycle<L>(kind, d + 1, true);
+    if (kind == 1) cycle<L>(kind, d + 1, false);
+    prolong_agg<L>(*c_, l.native.n, t.map, static_cast<const L*>(nx.x), static_cast<L*>(l.x), l.zero);
+    l.zero = false;
+    smooth<L>(d, opt_.post);
+}
+
+template <class L>
+void AmgEngine::apply_typed(const double* rhs, double* x) {
+    AmgLevel& l0 = levels_[0];
+    const size_t n = size_t(l0.native.n);
+    convert<double, L>(*c_, n, rhs, static_cast<L*>(l0.b));
+    std::fill(visits_.begin(), visits_.end(), 0);
+    cycle<L>(opt_.cycle, 0, true);
+    if (l0.zero) {
+        fill_zero(*c_, x, n * sizeof(double));
+    }
+    else {
+        convert<L, double>(*c_, n, static_cast<const L*>(l0.x), x);
+    }
+}
+
+void AmgEngine::apply(const double* rhs, double* x) {
+    require(ready(), "AMG requires setup", TF_E_INVALID);
+    if (opt_.precision == 32)
+        apply_typed<float>(rhs, x);
+    else
+        apply_typed<double>(rhs, x);
+}
+
+// --------------------------------------------------------------- Krylov
+
+SolverEngine::SolverEngine(tf_ctx* ctx, int kind, const KrylovOptionsC& o, AmgEngine* amg)
+    : c_(ctx), kind_(kind), o_(o), amg_(amg) {
+    require(kind == 0 || kind == 1, "unknown Krylov kind");
+    require(o.maxit >= 0 && o.refresh >= 1 && std::isfinite(o.rtol) && o.rtol >= 0 && o.rtol < 1 &&
+                std::isfinite(o.atol) && o.atol >= 0,
+            kind == 0 ? "Invalid CG options" : "Invalid BiCGStab options");
+    check(hipMalloc(&ctl_, sizeof(KCtl)), "hipMalloc control");
+    check(hipHostMalloc(&hctl_, sizeof(KCtl), hipHostMallocDefault), "hipHostMalloc control");
+}
+
+void SolverEngine::release_graphs() {
+    for (auto& g : graph_)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+SolverEngine::~SolverEngine() {
+    if (c_) (void)hipStreamSynchronize(c_->stream);
+    release_graphs();
+    free_csr(A_);
+    if (vec_) (void)hipFree(vec_);
+    if (ctl_) (void)hipFree(ctl_);
+    if (hctl_) (void)hipHostFree(hctl_);
+}
+
+void SolverEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v) {
+    require(n >= 1, "Solver requires a nonempty square matrix");
+    release_graphs();
+    DevCsr A = canonicalize(*c_, n, nnz, rp, ci, v);
+    if (vec_) {
+        c_->sync();
+        (void)hipFree(vec_);
+        vec_ = nullptr;
+    }
+    free_csr(A_);
+    A_ = A;
+    n_ = n;
+    const int nv = kind_ == 0 ? 6 : 11;
+    vec_ = dev_alloc<double>(size_t(n) * nv);
+    double* p = vec_;
+    auto take = [&]() {
+        double* q = p;
+        p += n;
+        return q;
+    };
+    rhs_ = take();
+    x_ = take();
+    r_ = take();
+    z_ = take();  // CG: M^-1 r
+    p_ = take();
+    q_ = take();  // CG: A p; BiCGStab: v
+    if (kind_ == 1) {
+        sh_ = take();
+        s_ = take();
+        t_ = take();
+        ph_ = take();
+        shat_ = take();
+    }
+    if (amg_) amg_->setup(A_.n, A_.nnz, A_.rp, A_.ci, A_.v);
+    c_->sync();
+}
+
+void SolverEngine::precondition(const double* in, double* out) {
+    if (amg_)
+        amg_->apply(in, out);
+    else
+        convert<double, double>(*c_, size_t(n_), in, out);
+}
+
+double SolverEngine::norm_host(const double* x) {
+    dot_device<double>(*c_, n_, x, x, c_->dscalars, 0);
+    check(hipMemcpyAsync(c_->hscalars, c_->dscalars, sizeof(double), hipMemcpyDeviceToHost, c_->stream),
+          "norm download");
+    c_->sync();
+    return std::sqrt(c_->hscalars[0]);
+}
+
+// One iteration of the solver as a fixed launch sequence (part 0).
+void SolverEngine::launch_part(int part) {
+    (void)part;
+    KCtl* ctl = static_cast<KCtl*>(ctl_);
+    const int g = kgroup(A_.n, A_.nnz);
+    const int rows = kBlock / g;
+    const int nb = (A_.n + rows - 1) / rows;
+    const dim3 grid(std::min(round_xcd(nb), kMaxReduceBlocks)), vb(vec_blocks(n_));
+    double* P = c_->partials;
+    unsigned* T = c_->tickets;
+    auto pslot = [&](int s) { return P + size_t(s) * kMaxReduceBlocks; };
+#define TF_G(NAME, ...)                                                                                 \
+    switch (g) {                                                                                       \
+    case 2: hipLaunchKernelGGL(NAME<2>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;        \
+    case 4: hipLaunchKernelGGL(NAME<4>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;        \
+    case 8: hipLaunchKernelGGL(NAME<8>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;        \
+    case 16: hipLaunchKernelGGL(NAME<16>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;      \
+    case 32: hipLaunchKernelGGL(NAME<32>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;      \
+    default: hipLaunchKernelGGL(NAME<64>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;      \
+    }
+    if (kind_ == 0) {
+        precondition(r_, z_);
+        hipLaunchKernelGGL(k_kdot, vb, dim3(kBlock), 0, c_->stream, n_, r_, z_, pslot(kSlotA), T + kSlotA, ctl,
+                           int(kFinRhoCg));
+        hipLaunchKernelGGL(k_cg_p, vb, dim3(kBlock), 0, c_->stream, n_, z_, p_, ctl);
+        TF_G(k_kspmv_dot, A_.n, nb, A_.rp, A_.ci, A_.v, p_, q_, p_, pslot(kSlotB), T + kSlotB, ctl, int(kFinCurvCg));
+        hipLaunchKernelGGL(k_cg_update, vb, dim3(kBlock), 0, c_->stream, n_, p_, q_, x_, r_, pslot(kSlotC),
+                           T + kSlotC, ctl);
+        TF_G(k_ktrue, A_.n, nb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, static_cast<double*>(nullptr), pslot(kSlotD),
+             T + kSlotD, ctl, 0);
+    }
+    else {
+        double* v = q_;
+        hipLaunchKernelGGL(k_kdot, vb, dim3(kBlock), 0, c_->stream, n_, sh_, r_, pslot(kSlotA), T + kSlotA, ctl,
+                           int(kFinRhoBi));
+        hipLaunchKernelGGL(k_bi_p, vb, dim3(kBlock), 0, c_->stream, n_, r_, v, p_, ctl);
+        precondition(p_, ph_);
+        TF_G(k_kspmv_dot, A_.n, nb, A_.rp, A_.ci, A_.v, ph_, v, sh_, pslot(kSlotB), T + kSlotB, ctl, int(kFinDenBi));
+        hipLaunchKernelGGL(k_bi_s, vb, dim3(kBlock), 0, c_->stream, n_, r_, v, ph_, s_, x_, pslot(kSlotC), T + kSlotC,
+                           ctl);
+        TF_G(k_ktrue, A_.n, nb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 1);
+        precondition(s_, shat_);
+        TF_G(k_kspmv_omega, A_.n, nb, A_.rp, A_.ci, A_.v, shat_, t_, s_, pslot(kSlotE), T + kSlotE, ctl);
+        hipLaunchKernelGGL(k_bi_update, vb, dim3(kBlock), 0, c_->stream, n_, shat_, s_, t_, x_, r_, pslot(kSlotA),
+                           T + kSlotA, ctl);
+        TF_G(k_ktrue, A_.n, nb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 0);
+    }
+#undef TF_G
+    check(hipGetLastError(), "Krylov iteration launch");
+}
+
+void SolverEngine::capture_or_launch(int part) {
+    if (!o_.graph) {
+        launch_part(part);
+        return;
+    }
+    if (!graph_[part]) {
+        hipGraph_t graph = nullptr;
+        check(hipStreamBeginCapture(c_->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            launch_part(part);
+        }
+        catch (...) {
+            (void)hipStreamEndCapture(c_->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        check(hipStreamEndCapture(c_->stream, &graph), "end capture");
+        const hipError_t e = hipGraphInstantiate(&graph_[part], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        check(e, "graph instantiate");
+    }
+    check(hipGraphLaunch(graph_[part], c_->stream), "graph launch");
+}
+
+tf_solve_result SolverEngine::finish(int status, int iterations) {
+    residual<double>(*c_, A_.n, A_.nnz, A_.rp, A_.ci, A_.v, rhs_, x_, r_);
+    const double res = norm_host(r_);
+    if (!std::isfinite(res) || !std::isfinite(normb_))
+        status = kNonFinite;
+    else if (res <= tol_)
+        status = kConverged;
+    else if (status == kConverged)
+        status = kMaxIt;
+    tf_solve_result out;
+    out.status = status;
+    out.iterations = iterations;
+    out.residual_norm = res;
+    out.relative_residual = normb_ > 0 ? res / normb_ : res;
+    return out;
+}
+
+tf_solve_result SolverEngine::apply(const double* b, double* x) {
+    require(n_ > 0, kind_ == 0 ? "CG requires setup" : "BiCGStab requires setup", TF_E_INVALID);
+    convert<double, double>(*c_, size_t(n_), b, rhs_);
+    convert<double, double>(*c_, size_t(n_), x, x_);
+    tf_solve_result out{};
+    normb_ = norm_host(rhs_);
+    tol_ = std::max(o_.atol, o_.rtol * normb_);
+    residual<double>(*c_, A_.n, A_.nnz, A_.rp, A_.ci, A_.v, rhs_, x_, r_);
+    const double res = norm_host(r_);
+    if (!std::isfinite(normb_) || !std::isfinite(res)) {
+        out = finish(kNonFinite, 0);
+    }
+    else if (res <= tol_) {
+        out = finish(kConverged, 0);
+    }
+    else {
+        KCtl init{};
+        init.normb = normb_;
+        init.tol = tol_;
+        init.res = res;
+        init.rho_prev = kind_ == 0 ? 0.0 : 1.0;
+        init.alpha = 1.0;
+        init.omega = 1.0;
+        init.status = kRunning;
+        init.iter = 0;
+        init.fresh = 1;
+        init.refresh = o_.refresh;
+        std::memcpy(hctl_, &init, sizeof(KCtl));
+        check(hipMemcpyAsync(ctl_, hctl_, sizeof(KCtl), hipMemcpyHostToDevice, c_->stream), "control upload");
+        if (kind_ == 1) convert<double, double>(*c_, size_t(n_), r_, sh_);
+        bool done = false;
+        KCtl* h = static_cast<KCtl*>(hctl_);
+        for (int k = 0; k < o_.maxit; ++k) {
+            capture_or_launch(0);
+            check(hipMemcpyAsync(hctl_, ctl_, sizeof(KCtl), hipMemcpyDeviceToHost, c_->stream), "control download");
+            c_->sync();
+            if (h->status != kRunning) {
+                out = finish(h->status, h->iter);
+                done = true;
+                break;
+            }
+        }
+        if (!done) out = finish(kMaxIt, o_.maxit);
+    }
+    convert<double, double>(*c_, size_t(n_), x_, x);
+    c_->sync();
+    return out;
+}
+
+}  // namespace tf

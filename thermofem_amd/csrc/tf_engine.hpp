@@ -1,0 +1,107 @@
+// Preconditioner and Krylov engines behind the C ABI: the AMG hierarchy and
+// V/W/F cycle (Amg.cuh, Cycle.cuh) and CG / BiCGStab (Cg.cuh, BiCgStab.cuh)
+// driven by device-resident scalars.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "tf_internal.hpp"
+
+namespace tf {
+
+struct AmgOptionsC {
+    int maxLevels = 20, coarseRows = 64, rounds = 8, pre = 1, post = 1, coarseSweeps = 8;
+    double omega = 0.75, theta = 0.25;
+    bool absorb = true;
+    int cycle = 0;
+    int precision = 64;
+};
+AmgOptionsC amg_options_from(const tf_amg_options* o);
+
+struct AmgLevel {
+    DevCsr native;          // canonical FP64 operator of this level
+    void* vals = nullptr;   // values in level precision (aliases native.v for FP64)
+    void* inv = nullptr;    // inverse diagonal, level precision
+    void* b = nullptr;      // right-hand side
+    void* x = nullptr;      // iterate
+    void* x2 = nullptr;     // Jacobi ping-pong partner
+    void* r = nullptr;      // residual
+    DevAgg transfer;        // to the next coarser level (empty on the coarsest)
+    bool zero = false;      // iterate is known to be zero (cleared lazily)
+    int64_t hist[4] = {0, 0, 0, 0};
+};
+
+class AmgEngine {
+public:
+    AmgEngine(tf_ctx* ctx, const AmgOptionsC& o);
+    ~AmgEngine();
+    void setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v);  // device arrays
+    // z = M^-1 r (FP64 device vectors; may alias).  Enqueued on the context stream.
+    void apply(const double* rhs, double* x);
+    bool ready() const { return !levels_.empty(); }
+    int n() const { return levels_.empty() ? 0 : levels_[0].native.n; }
+
+    std::vector<AmgLevel> levels_;
+    std::vector<int64_t> visits_;
+    std::string stop_, coarseSolver_;
+    AmgOptionsC opt_;
+
+private:
+    template <class L>
+    void cycle(int kind, size_t d, bool zero);
+    template <class L>
+    void smooth(size_t d, int sweeps);
+    template <class L>
+    void coarse(size_t d);
+    template <class L>
+    void apply_typed(const double* rhs, double* x);
+    void release();
+
+    tf_ctx* c_;
+    bool factored_ = false, singular_ = false;
+    double* lu_ = nullptr;
+    int* piv_ = nullptr;
+    double* inverse_ = nullptr;
+};
+
+struct KrylovOptionsC {
+    int maxit = 1000;
+    double rtol = 1e-8, atol = 0;
+    int refresh = 50;
+    bool graph = true;
+};
+
+class SolverEngine {
+public:
+    SolverEngine(tf_ctx* ctx, int kind, const KrylovOptionsC& o, AmgEngine* amg);
+    ~SolverEngine();
+    void setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v);
+    tf_solve_result apply(const double* b, double* x);  // device vectors
+    AmgEngine* amg() { return amg_; }
+    int n() const { return n_; }
+    double setupSeconds = 0, applySeconds = 0;
+
+private:
+    void precondition(const double* in, double* out);
+    void capture_or_launch(int part);
+    void launch_part(int part);
+    tf_solve_result finish(int status, int iterations);
+    double norm_host(const double* x);
+    void release_graphs();
+
+    tf_ctx* c_;
+    int kind_;
+    KrylovOptionsC o_;
+    AmgEngine* amg_;
+    DevCsr A_;
+    int n_ = 0;
+    double* vec_ = nullptr;  // all work vectors, contiguous
+    double *rhs_ = nullptr, *x_ = nullptr, *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *q_ = nullptr;
+    double *sh_ = nullptr, *s_ = nullptr, *t_ = nullptr, *ph_ = nullptr, *shat_ = nullptr;
+    void* ctl_ = nullptr;      // device control block
+    void* hctl_ = nullptr;     // pinned host mirror
+    hipGraphExec_t graph_[2] = {nullptr, nullptr};
+    double normb_ = 0, tol_ = 0;
+};
+
+}  // namespace tf

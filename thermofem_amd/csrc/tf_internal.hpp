@@ -1,0 +1,210 @@
+// Host-side internals shared by the library's translation units: the
+// per-device context, error translation and typed launch wrappers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/thermofem_hip.h"
+
+namespace tf {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(TF_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void set_error(const char* msg);
+
+// Run f, translating exceptions into status codes for the C ABI.
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return TF_OK;
+    }
+    catch (const Error& e) {
+        set_error(e.what());
+        return e.code;
+    }
+    catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return TF_E_CAPACITY;
+    }
+    catch (const std::exception& e) {
+        set_error(e.what());
+        return TF_E_INVALID;
+    }
+}
+
+inline void require(bool ok, const char* msg, int code = TF_E_INVALID) {
+    if (!ok) throw Error(code, msg);
+}
+
+constexpr int kMaxReduceBlocks = 2048;
+constexpr int kTickets = 64;
+
+// Scoped current-device switch that restores the caller's device
+// (OperationScope, src/LinearAlgebra/Backend.cu:116-126).
+struct DeviceScope {
+    int previous = -1;
+    explicit DeviceScope(int device) {
+        check(hipGetDevice(&previous), "hipGetDevice");
+        if (previous != device) check(hipSetDevice(device), "hipSetDevice");
+    }
+    ~DeviceScope() {
+        int now = -1;
+        if (hipGetDevice(&now) == hipSuccess && now != previous && previous >= 0) (void)hipSetDevice(previous);
+    }
+};
+
+}  // namespace tf
+
+struct tf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double* partials = nullptr;   // per-block partial sums (kMaxReduceBlocks x kTickets slots)
+    unsigned* tickets = nullptr;  // last-block tickets, zero between launches
+    double* dscalars = nullptr;   // device result scalars
+    double* hscalars = nullptr;   // pinned host mirror
+    int* dflags = nullptr;        // device integer flags/counters
+    int* hflags = nullptr;        // pinned host mirror
+    void* scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    void* ws_ptr[32] = {};     // grow-only setup workspace slots
+    size_t ws_bytes[32] = {};
+
+    void sync() { tf::check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+    // Grow-only scratch for scans.
+    int* scan_scratch(size_t ints) {
+        const size_t bytes = ints * sizeof(int);
+        if (bytes > scan_tmp_bytes) {
+            if (scan_tmp) {
+                sync();
+                tf::check(hipFree(scan_tmp), "hipFree");
+            }
+            scan_tmp = nullptr;
+            tf::check(hipMalloc(&scan_tmp, bytes), "hipMalloc scan scratch");
+            scan_tmp_bytes = bytes;
+        }
+        return static_cast<int*>(scan_tmp);
+    }
+    // Grow-only device workspace slot (contents undefined on growth).
+    void* ws(int slot, size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        if (bytes > ws_bytes[slot]) {
+            if (ws_ptr[slot]) {
+                sync();
+                tf::check(hipFree(ws_ptr[slot]), "hipFree workspace");
+            }
+            ws_ptr[slot] = nullptr;
+            ws_bytes[slot] = 0;
+            const size_t grown = bytes + bytes / 8;
+            tf::check(hipMalloc(&ws_ptr[slot], grown), "hipMalloc workspace");
+            ws_bytes[slot] = grown;
+        }
+        return ws_ptr[slot];
+    }
+    template <class T>
+    T* wst(int slot, size_t count) {
+        return static_cast<T*>(ws(slot, count * sizeof(T)));
+    }
+    // Synchronously reads one device int.
+    int read_int(const int* p) {
+        tf::check(hipMemcpyAsync(hflags, p, sizeof(int), hipMemcpyDeviceToHost, stream), "int download");
+        sync();
+        return hflags[0];
+    }
+    // Reads n ints from device flags after the queued work completes.
+    void read_flags(int n) {
+        tf::check(hipMemcpyAsync(hflags, dflags, size_t(n) * sizeof(int), hipMemcpyDeviceToHost, stream),
+                  "flag download");
+        sync();
+    }
+    void clear_flags(int n) {
+        tf::check(hipMemsetAsync(dflags, 0, size_t(n) * sizeof(int), stream), "flag clear");
+    }
+};
+
+namespace tf {
+
+// ---- typed kernel launchers (defined in the .hip translation units) ----
+template <class T>
+void spmv(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const T* x, T* y);
+template <class T>
+void residual(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const T* b, const T* x, T* r);
+template <class T>
+void jacobi_fused(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const T* inv, T w,
+                  const T* b, const T* xin, T* xout);
+template <class T>
+void jacobi_update(tf_ctx& c, int n, T w, const T* inv, const T* b, const T* product, T* x);
+template <class T>
+void spmm(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, int k, const T* X, T* Y);
+template <class T>
+void axpy(tf_ctx& c, int n, T alpha, const T* x, T* y);
+template <class T>
+void scal(tf_ctx& c, int n, T alpha, T* x);
+// Sum of x*y into the device scalar dst (slot selects partial/ticket storage).
+template <class T>
+void dot_device(tf_ctx& c, int n, const T* x, const T* y, double* dst, int slot);
+template <class A, class B>
+void convert(tf_ctx& c, size_t n, const A* in, B* out);
+template <class T>
+void restrict_agg(tf_ctx& c, int nc, const int* offsets, const int* members, const T* fine, T* coarse);
+template <class T>
+void prolong_agg(tf_ctx& c, int n, const int* map, const T* coarse, T* fine, bool set = false);
+template <class T>
+void inverse_diagonal(tf_ctx& c, int n, const int* rp, const int* ci, const T* v, T* inv, int* dbad);
+void fill_zero(tf_ctx& c, void* dst, size_t bytes);
+void scan_excl(tf_ctx& c, int n, const int* in, int* out);
+template <class T>
+void coarse_inv_apply(tf_ctx& c, int n, const double* inverse, const T* b, T* x);
+template <class T>
+void coarse_lu_solve(tf_ctx& c, int n, const double* lu, const int* piv, const T* b, T* x);
+bool coarse_lu_factor(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* lu, int* piv,
+                      double* inverse);
+
+// ---- aggregation setup (tf_agg.hip) ----
+// Device CSR owned by the caller (hipMalloc'd arrays, released by free_csr).
+struct DevCsr {
+    int n = 0;
+    int64_t nnz = 0;
+    int* rp = nullptr;
+    int* ci = nullptr;
+    double* v = nullptr;
+};
+void free_csr(DevCsr& a);
+// Aggregation transfer (AggregateTransfer, GpuAggregation.cuh:13-17).
+struct DevAgg {
+    int n = 0, nc = 0;
+    int* map = nullptr;
+    int* offsets = nullptr;
+    int* members = nullptr;
+    int64_t hist[4] = {0, 0, 0, 0};
+};
+void free_agg(DevAgg& t);
+template <class T>
+T* dev_alloc(size_t count) {
+    void* p = nullptr;
+    check(hipMalloc(&p, (count ? count : 1) * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+int csr_check(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v);
+bool csr_structure_ok(tf_ctx& c, int rows, int cols, int64_t nnz, const int* rp, const int* ci);
+DevCsr canonicalize(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v);
+DevCsr copy_csr(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v);
+bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax);
+DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta);
+void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal);
+int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map);
+DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc);
+DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb);
+DevCsr galerkin(tf_ctx& c, const DevCsr& a, const int* map, int nc, const int* offsets, const int* members);
+
+}  // namespace tf

@@ -1,0 +1,311 @@
+// Linear-algebra kernels for gfx950: CSR SpMV with sub-wave row groups and
+// fused epilogues (residual, weighted Jacobi), SpMM over row-major blocks of
+// right-hand sides, level-1 BLAS with deterministic reductions, aggregate
+// restriction/prolongation and element conversions.
+#include "tf_device.hpp"
+#include "tf_internal.hpp"
+
+namespace tf {
+namespace {
+
+enum Epilogue { kSpmv = 0, kResidual = 1, kJacobi = 2 };
+
+// One group of G lanes per row; the groups of a 256-thread block own 256/G
+// consecutive rows.  Column indices and values are streamed with
+// non-temporal loads (read once per product); x is gathered through the
+// cache, where the XCD-contiguous block mapping keeps stencil neighbours in L2.
+template <int G, class T, int E>
+__global__ __launch_bounds__(kBlock) void k_csr_rows(int n, int nblocks, const int* __restrict__ rp,
+                                                     const int* __restrict__ ci, const T* __restrict__ v,
+                                                     const T* __restrict__ x, T* __restrict__ y,
+                                                     const T* __restrict__ b, const T* __restrict__ inv, T w) {
+    const int blk = xcd_block(nblocks);
+    if (blk < 0) return;
+    constexpr int R = kBlock / G;
+    const int row = blk * R + int(threadIdx.x) / G;
+    const int lane = int(threadIdx.x) % G;
+    if (row >= n) return;  // whole groups leave together
+    const int s = rp[row], e = rp[row + 1];
+    T acc = 0;
+    for (int j = s + lane; j < e; j += G) acc += ld_stream(v + j) * x[ld_stream(ci + j)];
+    acc = group_sum<G>(acc);
+    if (lane == 0) {
+        if constexpr (E == kSpmv) {
+            y[row] = acc;
+        }
+        else if constexpr (E == kResidual) {
+            y[row] = b[row] - acc;
+        }
+        else {
+            y[row] = x[row] + w * inv[row] * (b[row] - acc);
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_jacobi_zero(int n, const T* __restrict__ inv, T w,
+                                                        const T* __restrict__ b, T* __restrict__ xout) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i < n) xout[i] = w * inv[i] * b[i];
+}
+
+int group_width(int n, int64_t nnz) {
+    const double avg = n > 0 ? double(nnz) / double(n) : 1.0;
+    int g = 2;
+    while (g < 64 && g < avg) g <<= 1;
+    return g;
+}
+
+template <class T, int E>
+void launch_rows(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const T* x, T* y,
+                 const T* b, const T* inv, T w) {
+    if (n <= 0) return;
+    const int g = group_width(n, nnz);
+    const int rows = kBlock / g;
+    const int nb = (n + rows - 1) / rows;
+    const dim3 grid(round_xcd(nb));
+#define TF_ROWS(G)                                                                                          \
+    case G:                                                                                                 \
+        hipLaunchKernelGGL((k_csr_rows<G, T, E>), grid, dim3(kBlock), 0, c.stream, n, nb, rp, ci, v, x, y, b, \
+                           inv, w);                                                                         \
+        break;
+    switch (g) {
+        TF_ROWS(2) TF_ROWS(4) TF_ROWS(8) TF_ROWS(16) TF_ROWS(32) TF_ROWS(64)
+    }
+#undef TF_ROWS
+    check(hipGetLastError(), "csr row kernel launch");
+}
+
+// Y = A X, X and Y row-major n x k.  G lanes per row cover the k columns
+// (coalesced row reads of X); matrix entries are broadcast to the group.
+template <int G, class T>
+__global__ __launch_bounds__(kBlock) void k_spmm(int n, int k, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                 const T* __restrict__ v, const T* __restrict__ X,
+                                                 T* __restrict__ Y) {
+    constexpr int R = kBlock / G;
+    const int row = int(blockIdx.x) * R + int(threadIdx.x) / G;
+    const int lane = int(threadIdx.x) % G;
+    if (row >= n) return;
+    const int s = rp[row], e = rp[row + 1];
+    for (int c0 = 0; c0 < k; c0 += G) {
+        const int col = c0 + lane;
+        T acc = 0;
+        if (col < k) {
+            for (int j = s; j < e; ++j) acc += v[j] * X[size_t(ci[j]) * size_t(k) + col];
+            Y[size_t(row) * size_t(k) + col] = acc;
+        }
+    }
+}
+
+template <class T>
+__global__ void k_jacobi_update(int n, T w, const T* __restrict__ inv, const T* __restrict__ b,
+                                const T* __restrict__ p, T* __restrict__ x) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i < n) x[i] += w * inv[i] * (b[i] - p[i]);
+}
+
+template <class T>
+__global__ void k_axpy(int n, T alpha, const T* __restrict__ x, T* __restrict__ y) {
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock)
+        y[i] = alpha * x[i] + y[i];
+}
+
+template <class T>
+__global__ void k_scal(int n, T alpha, T* __restrict__ x) {
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) x[i] = alpha * x[i];
+}
+
+// Fixed grid and grid-stride order make the partial sums, and thus the
+// result, independent of timing.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dot(int n, const T* __restrict__ x, const T* __restrict__ y,
+                                                double* partials, unsigned* ticket, double* dst) {
+    double s = 0;
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock)
+        s += double(x[i]) * double(y[i]);
+    s = block_sum(s);
+    grid_sum(s, partials, ticket, dst);
+}
+
+template <class A, class B>
+__global__ void k_convert(size_t n, const A* __restrict__ in, B* __restrict__ out) {
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock)
+        out[i] = static_cast<B>(in[i]);
+}
+
+// coarse[c] = sum of fine[members] in ascending member order, starting from 0.
+template <class T>
+__global__ void k_restrict(int nc, const int* __restrict__ off, const int* __restrict__ mem, const T* __restrict__ f,
+                           T* __restrict__ cvec) {
+    const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (c >= nc) return;
+    T s = 0;
+    for (int k = off[c]; k < off[c + 1]; ++k) s += f[mem[k]];
+    cvec[c] = s;
+}
+
+// fine += P coarse, or fine = P coarse when the fine iterate is known zero.
+template <class T>
+__global__ void k_prolong(int n, const int* __restrict__ map, const T* __restrict__ cvec, T* __restrict__ f, int set) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i < n) f[i] = set ? cvec[map[i]] : f[i] + cvec[map[i]];
+}
+
+// Inverse diagonal: duplicate diagonal entries summed in stored order.
+template <class T>
+__global__ void k_inverse_diagonal(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                   const T* __restrict__ v, T* __restrict__ inv, int* bad) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    T d = 0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j)
+        if (ci[j] == i) d += v[j];
+    const T r = T(1) / d;
+    if (!isfinite(d) || d == T(0) || !isfinite(r)) atomicOr(bad, 1);
+    inv[i] = r;
+}
+
+__global__ void k_fill_zero(size_t words, uint32_t* __restrict__ p) {
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < words; i += size_t(gridDim.x) * kBlock) p[i] = 0u;
+}
+__global__ void k_fill_zero_bytes(size_t bytes, uint8_t* __restrict__ p) {
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < bytes; i += size_t(gridDim.x) * kBlock) p[i] = 0;
+}
+
+int blocks_for(size_t n, int cap = 4096) {
+    size_t b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    return int(b > size_t(cap) ? size_t(cap) : b);
+}
+
+}  // namespace
+
+template <class T>
+void spmv(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const T* x, T* y) {
+    launch_rows<T, kSpmv>(c, n, nnz, rp, ci, v, x, y, nullptr, nullptr, T(0));
+}
+template <class T>
+void residual(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const T* b, const T* x, T* r) {
+    launch_rows<T, kResidual>(c, n, nnz, rp, ci, v, x, r, b, nullptr, T(0));
+}
+template <class T>
+void jacobi_fused(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const T* inv, T w,
+                  const T* b, const T* xin, T* xout) {
+    if (n <= 0) return;
+    if (!xin) {
+        hipLaunchKernelGGL(k_jacobi_zero<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, n, inv, w, b,
+                           xout);
+        check(hipGetLastError(), "jacobi zero launch");
+        return;
+    }
+    launch_rows<T, kJacobi>(c, n, nnz, rp, ci, v, xin, xout, b, inv, w);
+}
+template <class T>
+void jacobi_update(tf_ctx& c, int n, T w, const T* inv, const T* b, const T* product, T* x) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_jacobi_update<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, n, w, inv, b,
+                       product, x);
+    check(hipGetLastError(), "jacobi update launch");
+}
+template <class T>
+void spmm(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, int k, const T* X, T* Y) {
+    (void)nnz;
+    if (n <= 0 || k <= 0) return;
+    int g = 1;
+    while (g < 64 && g < k) g <<= 1;
+    const int rows = kBlock / g;
+    const dim3 grid((n + rows - 1) / rows);
+#define TF_MM(G)                                                                                                \
+    case G:                                                                                                     \
+        hipLaunchKernelGGL((k_spmm<G, T>), grid, dim3(kBlock), 0, c.stream, n, k, rp, ci, v, X, Y);              \
+        break;
+    switch (g) {
+        TF_MM(1) TF_MM(2) TF_MM(4) TF_MM(8) TF_MM(16) TF_MM(32) TF_MM(64)
+    }
+#undef TF_MM
+    check(hipGetLastError(), "spmm launch");
+}
+template <class T>
+void axpy(tf_ctx& c, int n, T alpha, const T* x, T* y) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_axpy<T>, dim3(blocks_for(size_t(n))), dim3(kBlock), 0, c.stream, n, alpha, x, y);
+    check(hipGetLastError(), "axpy launch");
+}
+template <class T>
+void scal(tf_ctx& c, int n, T alpha, T* x) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scal<T>, dim3(blocks_for(size_t(n))), dim3(kBlock), 0, c.stream, n, alpha, x);
+    check(hipGetLastError(), "scal launch");
+}
+template <class T>
+void dot_device(tf_ctx& c, int n, const T* x, const T* y, double* dst, int slot) {
+    require(slot >= 0 && slot < kTickets, "reduction slot out of range");
+    const int blocks = blocks_for(size_t(n > 0 ? n : 1), 1024);
+    hipLaunchKernelGGL(k_dot<T>, dim3(blocks), dim3(kBlock), 0, c.stream, n, x, y,
+                       c.partials + size_t(slot) * kMaxReduceBlocks, c.tickets + slot, dst);
+    check(hipGetLastError(), "dot launch");
+}
+template <class A, class B>
+void convert(tf_ctx& c, size_t n, const A* in, B* out) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_convert<A, B>), dim3(blocks_for(n)), dim3(kBlock), 0, c.stream, n, in, out);
+    check(hipGetLastError(), "convert launch");
+}
+template <class T>
+void restrict_agg(tf_ctx& c, int nc, const int* offsets, const int* members, const T* fine, T* coarse) {
+    if (nc <= 0) return;
+    hipLaunchKernelGGL(k_restrict<T>, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, nc, offsets,
+                       members, fine, coarse);
+    check(hipGetLastError(), "restrict launch");
+}
+template <class T>
+void prolong_agg(tf_ctx& c, int n, const int* map, const T* coarse, T* fine, bool set) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_prolong<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, n, map, coarse, fine,
+                       int(set));
+    check(hipGetLastError(), "prolong launch");
+}
+template <class T>
+void inverse_diagonal(tf_ctx& c, int n, const int* rp, const int* ci, const T* v, T* inv, int* dbad) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_inverse_diagonal<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, n, rp, ci,
+                       v, inv, dbad);
+    check(hipGetLastError(), "inverse diagonal launch");
+}
+void fill_zero(tf_ctx& c, void* dst, size_t bytes) {
+    if (!bytes) return;
+    if ((reinterpret_cast<uintptr_t>(dst) % 4) == 0 && bytes % 4 == 0) {
+        hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(bytes / 4)), dim3(kBlock), 0, c.stream, bytes / 4,
+                           static_cast<uint32_t*>(dst));
+    }
+    else {
+        hipLaunchKernelGGL(k_fill_zero_bytes, dim3(blocks_for(bytes)), dim3(kBlock), 0, c.stream, bytes,
+                           static_cast<uint8_t*>(dst));
+    }
+    check(hipGetLastError(), "fill launch");
+}
+
+#define TF_LA(T)                                                                                                   \
+    template void spmv<T>(tf_ctx&, int, int64_t, const int*, const int*, const T*, const T*, T*);                \
+    template void residual<T>(tf_ctx&, int, int64_t, const int*, const int*, const T*, const T*, const T*, T*);  \
+    template void jacobi_fused<T>(tf_ctx&, int, int64_t, const int*, const int*, const T*, const T*, T, const T*, \
+                                  const T*, T*);                                                                 \
+    template void jacobi_update<T>(tf_ctx&, int, T, const T*, const T*, const T*, T*);                           \
+    template void spmm<T>(tf_ctx&, int, int64_t, const int*, const int*, const T*, int, const T*, T*);           \
+    template void axpy<T>(tf_ctx&, int, T, const T*, T*);                                                        \
+    template void scal<T>(tf_ctx&, int, T, T*);                                                                  \
+    template void dot_device<T>(tf_ctx&, int, const T*, const T*, double*, int);                                 \
+    template void restrict_agg<T>(tf_ctx&, int, const int*, const int*, const T*, T*);                           \
+    template void prolong_agg<T>(tf_ctx&, int, const int*, const T*, T*, bool);                                        \
+    template void inverse_diagonal<T>(tf_ctx&, int, const int*, const int*, const T*, T*, int*);
+TF_LA(float)
+TF_LA(double)
+#undef TF_LA
+template void convert<double, float>(tf_ctx&, size_t, const double*, float*);
+template void convert<float, double>(tf_ctx&, size_t, const float*, double*);
+template void convert<int64_t, int>(tf_ctx&, size_t, const int64_t*, int*);
+template void convert<int, int64_t>(tf_ctx&, size_t, const int*, int64_t*);
+template void convert<double, double>(tf_ctx&, size_t, const double*, double*);
+template void convert<float, float>(tf_ctx&, size_t, const float*, float*);
+
+}  // namespace tf
