@@ -80,8 +80,9 @@ def restrict(t, fine, dtype=np.float64):
     nc = len(off) - 1
     f = up(fine, dtype)
     c = DeviceArray(nc, dtype)
+    doff, dmem = up(off, np.int32), up(mem, np.int32)  # keep alive across the launch
     fn = lib().tf_restrict_f64 if dtype == np.float64 else lib().tf_restrict_f32
-    check(fn(ctx().h, nc, up(off, np.int32).ptr, up(mem, np.int32).ptr, f.ptr, c.ptr))
+    check(fn(ctx().h, nc, doff.ptr, dmem.ptr, f.ptr, c.ptr))
     ctx().sync()
     return c.get()
 
@@ -90,8 +91,9 @@ def prolong(t, coarse, fine, dtype=np.float64):
     mp = t[0]
     f = up(fine, dtype)
     c = up(coarse, dtype)
+    dmp = up(mp, np.int32)  # keep alive across the launch
     fn = lib().tf_prolong_f64 if dtype == np.float64 else lib().tf_prolong_f32
-    check(fn(ctx().h, len(mp), up(mp, np.int32).ptr, c.ptr, f.ptr))
+    check(fn(ctx().h, len(mp), dmp.ptr, c.ptr, f.ptr))
     ctx().sync()
     return f.get()
 

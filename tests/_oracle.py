@@ -169,7 +169,8 @@ def lu_solve(lu, piv, b):
 def spmv(a, x):
     L = lib(); rp, ci, v = a; n = len(rp) - 1
     y = np.zeros(n)
-    L.tfo_spmv(n, rp, ci, v, np.ascontiguousarray(x, np.float64), y)
+    L.tfo_spmv(n, np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32),
+               np.ascontiguousarray(v, np.float64), np.ascontiguousarray(x, np.float64), y)
     return y
 
 

@@ -96,7 +96,8 @@ def test_coarse_lu_scale_invariance(scale):
     G.ctx().sync()
     assert np.array_equal(b.get(), x.get())
     y = G.DeviceArray(2, np.float64)
-    check(lib().tf_coarse_inv_apply_f64(G.ctx().h, 2, inv.ptr, G.up(scale * np.array([5.0, 10.0])).ptr, y.ptr))
+    b2 = G.up(scale * np.array([5.0, 10.0]))
+    check(lib().tf_coarse_inv_apply_f64(G.ctx().h, 2, inv.ptr, b2.ptr, y.ptr))
     G.ctx().sync()
     assert np.allclose(y.get(), [1, 2], atol=1e-12)
 
@@ -119,7 +120,8 @@ def test_coarse_lu_256_matches_oracle():
     import ctypes as C
     from thermofem_amd._lib import check, lib
     x = G.DeviceArray(256, np.float64)
-    check(lib().tf_coarse_inv_apply_f64(G.ctx().h, 256, inv.ptr, G.up(b).ptr, x.ptr))
+    db = G.up(b)
+    check(lib().tf_coarse_inv_apply_f64(G.ctx().h, 256, inv.ptr, db.ptr, x.ptr))
     G.ctx().sync()
     xh = O.lu_solve(lu_h, piv_h, b)
     assert np.allclose(x.get(), xh, rtol=1e-11, atol=1e-12)
