@@ -13,6 +13,8 @@
 // order, which is the order of the stable sort + sequential merge the
 // reference's host oracle defines (test/support/NativeAggregationOracle.cu:43-75),
 // so structures and values are bit-identical to it.
+#include <algorithm>
+
 #include "tf_device.hpp"
 #include "tf_internal.hpp"
 
@@ -42,6 +44,10 @@ enum Slot {
     kMembers = 17,
     kHist = 18,
     kUnique = 19,
+    kChanged = 20,
+    kStaging = 21,
+    kRawLen = 22,
+    kMaxPart = 23,
 };
 
 inline dim3 grid1(int64_t n) { return dim3(unsigned((n + kBlock - 1) / kBlock > 0 ? (n + kBlock - 1) / kBlock : 1)); }
@@ -155,6 +161,83 @@ __global__ void k_strength_fill(int n, const int* __restrict__ rp, const int* __
     }
 }
 
+// Position of column `col` in the sorted row [b, e), or -1.
+__device__ __forceinline__ int find_col(const int* __restrict__ ci, int b, int e, int col) {
+    while (b < e) {
+        const int m = (b + e) >> 1;
+        const int c = ci[m];
+        if (c == col) return m;
+        if (c < col)
+            b = m + 1;
+        else
+            e = m;
+    }
+    return -1;
+}
+
+// Flags a canonical matrix whose pattern is not symmetric.
+__global__ void k_struct_sym(int n, const int* __restrict__ rp, const int* __restrict__ ci, int* flag) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const int c = ci[j];
+        if (c != i && find_col(ci, rp[c], rp[c + 1], i) < 0) {
+            atomicOr(flag, 1);
+            return;
+        }
+    }
+}
+
+// Strength graph of a structurally symmetric canonical matrix without any
+// merge: row i of the graph keeps the pattern of row i, with weight
+// max(q_ij, q_ji) where q_ij = |a_ij|/rowmax_i when that qualifies
+// (rowmax_i > 0, q > 0, q >= theta) and 0 otherwise; entries with weight 0
+// are dropped.  This is exactly the max-merge of both emitted directions.
+// One pass writes the weights in the layout of A (column -1 = dropped) and
+// the kept count per row; k_compact_rows packs them.
+__global__ void k_strength_sym(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                               const double* __restrict__ a, const double* __restrict__ rowmax, double theta,
+                               int* __restrict__ scol, double* __restrict__ sw, int* __restrict__ cnt) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i == n) cnt[n] = 0;
+    if (i >= n) return;
+    const double mi = rowmax[i];
+    int kept = 0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const int c = ci[j];
+        double w = 0.0;
+        if (c != i) {
+            const double wi = mi > 0 ? fabs(a[j]) / mi : 0.0;
+            const double qi = (wi >= theta && wi > 0) ? wi : 0.0;
+            const double mc = rowmax[c];
+            const int k = find_col(ci, rp[c], rp[c + 1], i);
+            const double wc = (mc > 0 && k >= 0) ? fabs(a[k]) / mc : 0.0;
+            const double qc = (wc >= theta && wc > 0) ? wc : 0.0;
+            w = qi > qc ? qi : qc;
+        }
+        scol[j] = w > 0 ? c : -1;
+        sw[j] = w;
+        kept += w > 0 ? 1 : 0;
+    }
+    cnt[i] = kept;
+}
+
+__global__ void k_compact_rows(int n, const int* __restrict__ rp, const int* __restrict__ scol,
+                               const double* __restrict__ sw, const int* __restrict__ orp, int* __restrict__ oci,
+                               double* __restrict__ ow) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    int o = orp[i];
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const int c = scol[j];
+        if (c >= 0) {
+            oci[o] = c;
+            ow[o] = sw[j];
+            ++o;
+        }
+    }
+}
+
 // Output row I gathers the rows of its members (all rows of `offsets`
 // order; offsets == nullptr means the identity aggregate {I}); columns go
 // through map (nullptr: identity).  With `graph` set, intra-aggregate and
@@ -172,25 +255,155 @@ __device__ __forceinline__ void member_range(int I, const int* offsets, int& b, 
 
 __global__ void k_gather_count(int nc, const int* __restrict__ offsets, const int* __restrict__ members,
                                const int* __restrict__ map, const int* __restrict__ rp, const int* __restrict__ ci,
-                               const double* __restrict__ v, int graph, int* cnt) {
+                               const double* __restrict__ v, int graph, int* cnt, int* rawLen) {
     const int I = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (I == nc) cnt[nc] = 0;
     if (I >= nc) return;
     int mb, me;
     member_range(I, offsets, mb, me);
-    int total = 0;
+    int total = 0, raw = 0;
     for (int k = mb; k < me; ++k) {
         const int m = offsets ? members[k] : k;
-        if (!graph) {
-            total += rp[m + 1] - rp[m];
-            continue;
-        }
+        raw += rp[m + 1] - rp[m];
+        if (!graph) continue;
         for (int j = rp[m]; j < rp[m + 1]; ++j) {
             const int J = map ? map[ci[j]] : ci[j];
             if (J != I && v[j] > 0) ++total;
         }
     }
-    cnt[I] = total;
+    cnt[I] = graph ? total : raw;
+    if (rawLen) rawLen[I] = raw;  // LDS staging holds unfiltered entries
+}
+
+// Maximum of a[0..n): block maxima to partials, then one block combines them.
+__global__ __launch_bounds__(kBlock) void k_max_partials(int n, const int* __restrict__ a, int* __restrict__ part) {
+    __shared__ int wm[kBlock / kWave];
+    int m = 0;
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) m = max(m, a[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) wm[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int r = 0;
+        for (int i = 0; i < kBlock / kWave; ++i) r = max(r, wm[i]);
+        part[blockIdx.x] = r;
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_max_final(int nparts, const int* __restrict__ part, int* out) {
+    __shared__ int wm[kBlock / kWave];
+    int m = 0;
+    for (int i = int(threadIdx.x); i < nparts; i += kBlock) m = max(m, part[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) wm[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int r = 0;
+        for (int i = 0; i < kBlock / kWave; ++i) r = max(r, wm[i]);
+        *out = r;
+    }
+}
+
+// Group-per-output-row merge (G lanes, at most G*8 staged entries per row).
+// The group gathers its row's member entries into LDS in encounter order;
+// each lane then takes entries, keeps those that are the first occurrence
+// of their column, merges every later duplicate into it sequentially in
+// encounter order (sum or max: the order of the stable merge) and places
+// the result at its rank among the distinct columns, i.e. sorted by column.
+// Results go to a staging area at gstart[I]; ucnt[I] = distinct columns.
+// Groups never use block barriers: the wave-level fence orders the LDS
+// writes of a group before its reads.
+constexpr int kDedupCap = 512;
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int G, bool Max>
+__global__ __launch_bounds__(kBlock) void k_dedup_rows(int nc, const int* __restrict__ offsets,
+                                                       const int* __restrict__ members, const int* __restrict__ map,
+                                                       const int* __restrict__ rp, const int* __restrict__ ci,
+                                                       const double* __restrict__ v, int graph,
+                                                       const int* __restrict__ gstart, int* __restrict__ scol,
+                                                       double* __restrict__ sval, int* __restrict__ ucnt, int* bad) {
+    constexpr int CAP = G * 8;
+    constexpr int NG = kBlock / G;
+    __shared__ int lcol[NG][CAP];
+    __shared__ double lval[NG][CAP];
+    __shared__ unsigned char lfirst[NG][CAP];
+    const int g = int(threadIdx.x) / G, lane = int(threadIdx.x) % G;
+    const int I = int(blockIdx.x) * NG + g;
+    const bool active = I < nc;
+    int* C = lcol[g];
+    double* V = lval[g];
+    unsigned char* F = lfirst[g];
+    int m = 0;
+    if (active) {
+        int mb, me;
+        member_range(I, offsets, mb, me);
+        for (int k = mb; k < me; ++k) {
+            const int row = offsets ? members[k] : k;
+            const int s = rp[row], len = rp[row + 1] - s;
+            for (int t = lane; t < len; t += G) {
+                const int J = map ? map[ci[s + t]] : ci[s + t];
+                const double x = v[s + t];
+                const bool keep = !graph || (J != I && x > 0);
+                C[m + t] = keep ? J : -1;
+                V[m + t] = x;
+            }
+            m += len;
+        }
+    }
+    wave_lds_sync();
+    for (int e = lane; e < m; e += G) {
+        const int J = C[e];
+        bool first = J >= 0;
+        for (int q = 0; q < e && first; ++q)
+            if (C[q] == J) first = false;
+        F[e] = first ? 1 : 0;
+    }
+    wave_lds_sync();
+    int mine = 0;
+    const int base = active ? gstart[I] : 0;
+    for (int e = lane; e < m; e += G) {
+        if (!F[e]) continue;
+        const int J = C[e];
+        double acc = V[e];
+        for (int q = e + 1; q < m; ++q) {
+            if (C[q] != J) continue;
+            const double x = V[q];
+            acc = Max ? (acc > x ? acc : x) : acc + x;
+            if (!isfinite(acc)) atomicOr(bad, 1);
+        }
+        int rank = 0;
+        for (int q = 0; q < m; ++q)
+            if (F[q] && C[q] < J) ++rank;
+        scol[base + rank] = J;
+        sval[base + rank] = acc;
+        ++mine;
+    }
+    mine = group_sum<G>(mine);
+    if (active && lane == 0) {
+        ucnt[I] = mine;
+        if (I == nc - 1) ucnt[nc] = 0;
+    }
+}
+
+// Staging -> CSR, 16 lanes per row (coalesced row copies).
+__global__ __launch_bounds__(kBlock) void k_dedup_copy(int nc, const int* __restrict__ gstart,
+                                                       const int* __restrict__ scol, const double* __restrict__ sval,
+                                                       const int* __restrict__ orp, int* __restrict__ oci,
+                                                       double* __restrict__ ov) {
+    const int I = int(blockIdx.x) * (kBlock / 16) + int(threadIdx.x) / 16;
+    const int lane = int(threadIdx.x) % 16;
+    if (I >= nc) return;
+    const int b = gstart[I], o = orp[I], u = orp[I + 1] - o;
+    for (int k = lane; k < u; k += 16) {
+        oci[o + k] = scol[b + k];
+        ov[o + k] = sval[b + k];
+    }
 }
 
 __global__ void k_gather_fill(int nc, const int* __restrict__ offsets, const int* __restrict__ members,
@@ -363,8 +576,11 @@ __global__ void k_merge_write(int rows, const int* __restrict__ start, const u64
 }
 
 // ---- matching (GpuAggregation.cu:402-505) ----
+// A round after one that accepted nothing would reproduce the same
+// proposals and accept nothing again, so it is skipped (prevChanged == 0).
 __global__ void k_propose(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ w,
-                          const int* __restrict__ mate, int* __restrict__ proposal) {
+                          const int* __restrict__ mate, int* __restrict__ proposal, const int* prevChanged) {
+    if (prevChanged && *prevChanged == 0) return;
     const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (i >= n) return;
     int best = -1;
@@ -387,11 +603,16 @@ __global__ void k_propose(int n, const int* __restrict__ rp, const int* __restri
     proposal[i] = best;
 }
 
-__global__ void k_accept(int n, const int* __restrict__ proposal, int* __restrict__ mate) {
+__global__ void k_accept(int n, const int* __restrict__ proposal, int* __restrict__ mate, const int* prevChanged,
+                         int* changed) {
+    if (prevChanged && *prevChanged == 0) return;
     const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (i >= n) return;
     const int p = proposal[i];
-    if (p >= 0 && proposal[p] == i) mate[i] = p;
+    if (p >= 0 && proposal[p] == i) {
+        mate[i] = p;
+        if (changed) *changed = 1;
+    }
 }
 
 __global__ void k_roots(int n, const int* __restrict__ mate, int* __restrict__ flags) {
@@ -499,16 +720,32 @@ __global__ void k_occupied(int nc, const int* __restrict__ sizes, int* __restric
     if (c <= nc) flags[c] = (c < nc && sizes[c] > 0) ? 1 : 0;
 }
 
-__global__ void k_pack_members(int nc, const int* __restrict__ sizes, const int* __restrict__ slots,
-                               const int* __restrict__ offsets, int* __restrict__ out, int* histogram) {
+__global__ __launch_bounds__(kBlock) void k_pack_members(int nc, const int* __restrict__ sizes,
+                                                         const int* __restrict__ slots,
+                                                         const int* __restrict__ offsets, int* __restrict__ out,
+                                                         int* histogram) {
+    __shared__ int h[4];
+    if (threadIdx.x < 4) h[threadIdx.x] = 0;
+    __syncthreads();
     const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (c >= nc) return;
-    const int sz = sizes[c];
-    for (int k = 0; k < sz; ++k) out[offsets[c] + k] = slots[size_t(c) * 4 + k];
-    if (histogram && sz) atomicAdd(histogram + sz - 1, 1);
+    if (c < nc) {
+        const int sz = sizes[c];
+        for (int k = 0; k < sz; ++k) out[offsets[c] + k] = slots[size_t(c) * 4 + k];
+        if (histogram && sz) atomicAdd(&h[sz - 1], 1);
+    }
+    __syncthreads();
+    if (histogram && threadIdx.x < 4 && h[threadIdx.x]) atomicAdd(histogram + threadIdx.x, h[threadIdx.x]);
 }
 
 // ---------------- host helpers ----------------
+
+void match_round_flags(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal,
+                       const int* prevChanged, int* changed) {
+    hipLaunchKernelGGL(k_propose, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, w, mate, proposal, prevChanged);
+    check(hipGetLastError(), "propose launch");
+    hipLaunchKernelGGL(k_accept, grid1(n), dim3(kBlock), 0, c.stream, n, proposal, mate, prevChanged, changed);
+    check(hipGetLastError(), "accept launch");
+}
 
 void fill_i32(tf_ctx& c, int n, int value, int* p) {
     if (n <= 0) return;
@@ -537,11 +774,11 @@ DevCsr merge_rows(tf_ctx& c, int rows, const int* start, const u64* keys, const 
     int* ucnt = c.wst<int>(kUnique, size_t(rows) + 1);
     hipLaunchKernelGGL(k_unique_count, grid1(int64_t(rows) + 1), dim3(kBlock), 0, c.stream, rows, start, keys, ucnt);
     check(hipGetLastError(), "unique count launch");
-    out.rp = dev_alloc<int>(size_t(rows) + 1);
+    out.rp = dev_alloc<int>(c, size_t(rows) + 1);
     scan_excl(c, rows + 1, ucnt, out.rp);
     out.nnz = c.read_int(out.rp + rows);
-    out.ci = dev_alloc<int>(size_t(out.nnz));
-    out.v = dev_alloc<double>(size_t(out.nnz));
+    out.ci = dev_alloc<int>(c, size_t(out.nnz));
+    out.v = dev_alloc<double>(c, size_t(out.nnz));
     c.clear_flags(1);
     if (maximum)
         hipLaunchKernelGGL(k_merge_write<true>, grid1(rows), dim3(kBlock), 0, c.stream, rows, start, keys, vals,
@@ -552,22 +789,76 @@ DevCsr merge_rows(tf_ctx& c, int rows, const int* start, const u64* keys, const 
     check(hipGetLastError(), "merge launch");
     c.read_flags(1);
     if (c.hflags[0]) {
-        free_csr(out);
+        free_csr(c, out);
         throw Error(TF_E_NUMERIC, "Nonfinite aggregate reduction");
     }
     return out;
 }
 
-// Gather member rows into per-output-row segments, sort, merge.
+// Gather member rows into per-output-row segments and merge duplicate
+// columns: wave-per-row LDS merge when every row fits kDedupCap entries,
+// otherwise staged keys + per-segment sort + sequential merge.
 DevCsr gather_merge(tf_ctx& c, int nc, const int* offsets, const int* members, const int* map, const int* rp,
                     const int* ci, const double* v, bool graph, bool maximum) {
     int* cnt = c.wst<int>(kCount, size_t(nc) + 1);
+    int* rawLen = c.wst<int>(kRawLen, size_t(nc) + 1);
     hipLaunchKernelGGL(k_gather_count, grid1(int64_t(nc) + 1), dim3(kBlock), 0, c.stream, nc, offsets, members, map,
-                       rp, ci, v, int(graph), cnt);
+                       rp, ci, v, int(graph), cnt, rawLen);
     check(hipGetLastError(), "gather count launch");
+    {
+        const int parts = int(std::min<int64_t>((int64_t(nc) + kBlock - 1) / kBlock, 1024));
+        int* part = c.wst<int>(kMaxPart, size_t(parts) + 1);
+        hipLaunchKernelGGL(k_max_partials, dim3(parts), dim3(kBlock), 0, c.stream, nc, rawLen, part);
+        hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kBlock), 0, c.stream, parts, part, c.dflags);
+        check(hipGetLastError(), "max launch");
+    }
     int* start = c.wst<int>(kStart, size_t(nc) + 1);
     scan_excl(c, nc + 1, cnt, start);
-    const int total = c.read_int(start + nc);
+    check(hipMemcpyAsync(c.dflags + 1, start + nc, sizeof(int), hipMemcpyDeviceToDevice, c.stream), "total copy");
+    c.read_flags(2);
+    const int maxRow = c.hflags[0], total = c.hflags[1];
+    if (maxRow <= kDedupCap) {
+        int* scol = c.wst<int>(kKeys, size_t(total) + 1);
+        double* sval = c.wst<double>(kVals, size_t(total) + 1);
+        int* ucnt = c.wst<int>(kUnique, size_t(nc) + 1);
+        c.clear_flags(1);
+#define TF_DEDUP(GG, MX)                                                                                       \
+    hipLaunchKernelGGL((k_dedup_rows<GG, MX>), dim3((nc + kBlock / GG - 1) / (kBlock / GG)), dim3(kBlock), 0,    \
+                       c.stream, nc, offsets, members, map, rp, ci, v, int(graph), start, scol, sval, ucnt,      \
+                       c.dflags)
+        if (maxRow <= 16 * 8) {
+            if (maximum)
+                TF_DEDUP(16, true);
+            else
+                TF_DEDUP(16, false);
+        }
+        else {
+            if (maximum)
+                TF_DEDUP(64, true);
+            else
+                TF_DEDUP(64, false);
+        }
+#undef TF_DEDUP
+        check(hipGetLastError(), "dedup launch");
+        DevCsr out;
+        out.n = nc;
+        out.rp = dev_alloc<int>(c, size_t(nc) + 1);
+        scan_excl(c, nc + 1, ucnt, out.rp);
+        check(hipMemcpyAsync(c.dflags + 1, out.rp + nc, sizeof(int), hipMemcpyDeviceToDevice, c.stream),
+              "nnz copy");
+        c.read_flags(2);
+        if (c.hflags[0]) {
+            dev_free(c, out.rp);
+            throw Error(TF_E_NUMERIC, "Nonfinite aggregate reduction");
+        }
+        out.nnz = c.hflags[1];
+        out.ci = dev_alloc<int>(c, size_t(out.nnz));
+        out.v = dev_alloc<double>(c, size_t(out.nnz));
+        hipLaunchKernelGGL(k_dedup_copy, dim3((nc + kBlock / 16 - 1) / (kBlock / 16)), dim3(kBlock), 0, c.stream, nc,
+                           start, scol, sval, out.rp, out.ci, out.v);
+        check(hipGetLastError(), "dedup copy launch");
+        return out;
+    }
     u64* keys = c.wst<u64>(kKeys, size_t(total));
     double* vals = c.wst<double>(kVals, size_t(total));
     hipLaunchKernelGGL(k_gather_fill, grid1(nc), dim3(kBlock), 0, c.stream, nc, offsets, members, map, rp, ci, v,
@@ -593,18 +884,18 @@ void build_members(tf_ctx& c, int n, int nc, const int* map, int** sizesOut, int
 
 }  // namespace
 
-void free_csr(DevCsr& a) {
-    if (a.rp) (void)hipFree(a.rp);
-    if (a.ci) (void)hipFree(a.ci);
-    if (a.v) (void)hipFree(a.v);
+void free_csr(tf_ctx& c, DevCsr& a) {
+    dev_free(c, a.rp);
+    dev_free(c, a.ci);
+    dev_free(c, a.v);
     a.rp = a.ci = nullptr;
     a.v = nullptr;
 }
 
-void free_agg(DevAgg& t) {
-    if (t.map) (void)hipFree(t.map);
-    if (t.offsets) (void)hipFree(t.offsets);
-    if (t.members) (void)hipFree(t.members);
+void free_agg(tf_ctx& c, DevAgg& t) {
+    dev_free(c, t.map);
+    dev_free(c, t.offsets);
+    dev_free(c, t.members);
     t.map = t.offsets = t.members = nullptr;
 }
 
@@ -629,9 +920,9 @@ DevCsr copy_csr(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, con
     DevCsr out;
     out.n = n;
     out.nnz = nnz;
-    out.rp = dev_alloc<int>(size_t(n) + 1);
-    out.ci = dev_alloc<int>(size_t(nnz));
-    out.v = dev_alloc<double>(size_t(nnz));
+    out.rp = dev_alloc<int>(c, size_t(n) + 1);
+    out.ci = dev_alloc<int>(c, size_t(nnz));
+    out.v = dev_alloc<double>(c, size_t(nnz));
     check(hipMemcpyAsync(out.rp, rp, (size_t(n) + 1) * sizeof(int), hipMemcpyDefault, c.stream), "copy rp");
     if (nnz) {
         check(hipMemcpyAsync(out.ci, ci, size_t(nnz) * sizeof(int), hipMemcpyDefault, c.stream), "copy ci");
@@ -657,12 +948,41 @@ bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const doubl
     return c.hflags[0] == 0;
 }
 
-DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta) {
+double* rowmax_workspace(tf_ctx& c, int n) { return c.wst<double>(kRowmax, size_t(n)); }
+
+DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta,
+                      int symmetric, bool prevalidated, int* symmetricOut) {
     require(std::isfinite(theta) && theta >= 0 && theta <= 1, "Invalid strength threshold");
     if (nnz > int64_t(0x3fffffff)) throw Error(TF_E_CAPACITY, "Native AMG: strength graph exceeds 32-bit capacity");
-    double* rowmax = c.wst<double>(kRowmax, size_t(n));
-    if (!validate_rowmax(c, n, rp, ci, v, rowmax))
+    double* rowmax = rowmax_workspace(c, n);
+    if (!prevalidated && !validate_rowmax(c, n, rp, ci, v, rowmax))
         throw Error(TF_E_NUMERIC, "Native AMG: missing/zero diagonal, non-finite value or reciprocal in hierarchy");
+    if (symmetric < 0) {
+        c.clear_flags(1);
+        hipLaunchKernelGGL(k_struct_sym, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, c.dflags);
+        check(hipGetLastError(), "symmetry check launch");
+        c.read_flags(1);
+        symmetric = c.hflags[0] == 0 ? 1 : 0;
+    }
+    if (symmetricOut) *symmetricOut = symmetric;
+    if (symmetric) {
+        int* cnt = c.wst<int>(kCount, size_t(n) + 1);
+        int* scol = c.wst<int>(kKeys, size_t(nnz) + 1);
+        double* sw = c.wst<double>(kVals, size_t(nnz) + 1);
+        hipLaunchKernelGGL(k_strength_sym, grid1(int64_t(n) + 1), dim3(kBlock), 0, c.stream, n, rp, ci, v, rowmax,
+                           theta, scol, sw, cnt);
+        check(hipGetLastError(), "strength launch");
+        DevCsr g;
+        g.n = n;
+        g.rp = dev_alloc<int>(c, size_t(n) + 1);
+        scan_excl(c, n + 1, cnt, g.rp);
+        g.nnz = c.read_int(g.rp + n);
+        g.ci = dev_alloc<int>(c, size_t(g.nnz));
+        g.v = dev_alloc<double>(c, size_t(g.nnz));
+        hipLaunchKernelGGL(k_compact_rows, grid1(n), dim3(kBlock), 0, c.stream, n, rp, scol, sw, g.rp, g.ci, g.v);
+        check(hipGetLastError(), "strength compaction launch");
+        return g;
+    }
     int* cnt = c.wst<int>(kCount, size_t(n) + 1);
     fill_i32(c, n + 1, 0, cnt);
     hipLaunchKernelGGL(k_strength_count, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, v, rowmax, theta, cnt);
@@ -674,10 +994,10 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
         DevCsr g;
         g.n = n;
         g.nnz = 0;
-        g.rp = dev_alloc<int>(size_t(n) + 1);
+        g.rp = dev_alloc<int>(c, size_t(n) + 1);
         fill_i32(c, n + 1, 0, g.rp);
-        g.ci = dev_alloc<int>(0);
-        g.v = dev_alloc<double>(0);
+        g.ci = dev_alloc<int>(c, 0);
+        g.v = dev_alloc<double>(c, 0);
         return g;
     }
     int* fill = c.wst<int>(kFill, size_t(n));
@@ -692,17 +1012,17 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
 }
 
 void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal) {
-    hipLaunchKernelGGL(k_propose, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, w, mate, proposal);
-    check(hipGetLastError(), "propose launch");
-    hipLaunchKernelGGL(k_accept, grid1(n), dim3(kBlock), 0, c.stream, n, proposal, mate);
-    check(hipGetLastError(), "accept launch");
+    match_round_flags(c, n, rp, ci, w, mate, proposal, nullptr, nullptr);
 }
 
 int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map) {
     int* mate = c.wst<int>(kMate, size_t(n));
     int* proposal = c.wst<int>(kProposal, size_t(n));
     fill_i32(c, n, -1, mate);
-    for (int r = 0; r < rounds; ++r) match_round(c, n, rp, ci, w, mate, proposal);
+    int* changed = c.wst<int>(kChanged, 8);
+    fill_i32(c, 8, 0, changed);
+    for (int r = 0; r < rounds; ++r)
+        match_round_flags(c, n, rp, ci, w, mate, proposal, r ? changed + r - 1 : nullptr, changed + r);
     int* flags = c.wst<int>(kFlags, size_t(n) + 1);
     hipLaunchKernelGGL(k_roots, grid1(int64_t(n) + 1), dim3(kBlock), 0, c.stream, n, mate, flags);
     check(hipGetLastError(), "roots launch");
@@ -730,7 +1050,7 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb) {
     const int n = g.n;
     DevAgg t;
     t.n = n;
-    t.map = dev_alloc<int>(size_t(n));
+    t.map = dev_alloc<int>(c, size_t(n));
     try {
         const int first = pair(c, n, g.rp, g.ci, g.v, rounds, t.map);
         DevCsr contracted = contract_graph(c, n, g.rp, g.ci, g.v, t.map, first);
@@ -740,10 +1060,10 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb) {
             coarse = pair(c, first, contracted.rp, contracted.ci, contracted.v, rounds, second);
         }
         catch (...) {
-            free_csr(contracted);
+            free_csr(c, contracted);
             throw;
         }
-        free_csr(contracted);
+        free_csr(c, contracted);
         hipLaunchKernelGGL(k_compose, grid1(n), dim3(kBlock), 0, c.stream, n, t.map, second);
         check(hipGetLastError(), "compose launch");
         int *sizes, *slots;
@@ -782,8 +1102,8 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb) {
         c.read_flags(1);
         if (c.hflags[0]) throw Error(TF_E_INVALID, "Aggregate capacity exceeded");
         t.nc = coarse;
-        t.offsets = dev_alloc<int>(size_t(coarse) + 1);
-        t.members = dev_alloc<int>(size_t(n));
+        t.offsets = dev_alloc<int>(c, size_t(coarse) + 1);
+        t.members = dev_alloc<int>(c, size_t(n));
         scan_excl(c, coarse + 1, sizes, t.offsets);
         int* hist = c.wst<int>(kHist, 4);
         fill_i32(c, 4, 0, hist);
@@ -795,7 +1115,7 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb) {
         for (int k = 0; k < 4; ++k) t.hist[k] = c.hflags[k];
     }
     catch (...) {
-        free_agg(t);
+        free_agg(c, t);
         throw;
     }
     return t;

@@ -17,20 +17,24 @@ thread_local std::string g_error;
 
 // Device copies of host CSR arrays for the host_memory entry points.
 struct Staged {
+    tf_ctx* c = nullptr;
     int* rp = nullptr;
     int* ci = nullptr;
     double* v = nullptr;
     ~Staged() {
-        if (rp) (void)hipFree(rp);
-        if (ci) (void)hipFree(ci);
-        if (v) (void)hipFree(v);
+        if (!c) return;
+        (void)hipStreamSynchronize(c->stream);
+        dev_free(*c, rp);
+        dev_free(*c, ci);
+        dev_free(*c, v);
     }
 };
 
 void stage(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, Staged& s) {
-    s.rp = dev_alloc<int>(size_t(n) + 1);
-    s.ci = dev_alloc<int>(size_t(nnz));
-    s.v = dev_alloc<double>(size_t(nnz));
+    s.c = &c;
+    s.rp = dev_alloc<int>(c, size_t(n) + 1);
+    s.ci = dev_alloc<int>(c, size_t(nnz));
+    s.v = dev_alloc<double>(c, size_t(nnz));
     check(hipMemcpyAsync(s.rp, rp, (size_t(n) + 1) * sizeof(int), hipMemcpyHostToDevice, c.stream), "upload rp");
     if (nnz) {
         check(hipMemcpyAsync(s.ci, ci, size_t(nnz) * sizeof(int), hipMemcpyHostToDevice, c.stream), "upload ci");
@@ -61,6 +65,7 @@ void destroy_ctx(tf_ctx* c) {
     if (!c) return;
     DeviceScope scope(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->pool.trim();
     for (int i = 0; i < 32; ++i)
         if (c->ws_ptr[i]) (void)hipFree(c->ws_ptr[i]);
     if (c->scan_tmp) (void)hipFree(c->scan_tmp);
@@ -305,7 +310,7 @@ int tf_csr_canonicalize(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const in
         check(hipMemcpyAsync(out_rp, a.rp, (size_t(n) + 1) * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream),
               "canonical offsets");
         ctx->sync();
-        (void)hipFree(a.rp);
+        dev_free(*ctx, a.rp);
         *out_ci = a.ci;
         *out_v = a.v;
         *out_nnz = a.nnz;
@@ -440,8 +445,15 @@ int tf_amg_apply(tf_amg* amg, const double* rhs, double* x, int host_memory) {
         DeviceScope scope(amg->ctx->device);
         const int n = amg->eng->n();
         if (host_memory) {
-            double* d = dev_alloc<double>(size_t(n) * 2);
-            std::unique_ptr<double, void (*)(double*)> hold(d, [](double* p) { (void)hipFree(p); });
+            double* d = dev_alloc<double>(*amg->ctx, size_t(n) * 2);
+            struct Hold {
+                tf_ctx* c;
+                double* p;
+                ~Hold() {
+                    (void)hipStreamSynchronize(c->stream);
+                    dev_free(*c, p);
+                }
+            } hold{amg->ctx, d};
             check(hipMemcpyAsync(d, rhs, size_t(n) * sizeof(double), hipMemcpyHostToDevice, amg->ctx->stream),
                   "upload rhs");
             amg->eng->apply(d, d + n);
@@ -600,12 +612,17 @@ int tf_solver_apply(tf_solver* s, const double* b, double* x, int host_memory, t
         require(n > 0, "solver requires setup");
         Events ev;
         double* hx = x;
-        std::unique_ptr<double, void (*)(double*)> hold(nullptr, [](double* p) {
-            if (p) (void)hipFree(p);
-        });
+        struct Hold {
+            tf_ctx* c;
+            double* p;
+            ~Hold() {
+                (void)hipStreamSynchronize(c->stream);
+                dev_free(*c, p);
+            }
+        } hold{s->ctx, nullptr};
         if (host_memory) {
-            double* d = dev_alloc<double>(size_t(n) * 2);
-            hold.reset(d);
+            double* d = dev_alloc<double>(*s->ctx, size_t(n) * 2);
+            hold.p = d;
             check(hipMemcpyAsync(d, b, size_t(n) * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream), "upload b");
             check(hipMemcpyAsync(d + n, x, size_t(n) * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream),
                   "upload x");

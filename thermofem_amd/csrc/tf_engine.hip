@@ -33,36 +33,40 @@ enum { kRunning = -1, kConverged = 0, kMaxIt = 1, kBreakdown = 2, kNonFinite = 3
 namespace {
 
 constexpr int kSlotA = 40, kSlotB = 41, kSlotC = 42, kSlotD = 43, kSlotE = 44;
+// Levels up to this many rows compute residual and restriction in one
+// aggregate-parallel kernel (one thread per aggregate); measured slower than
+// the streaming residual kernel + restriction at every level of the 128^3
+// hierarchy (too little parallelism per level), so it is off by default.
+constexpr int kFusedResidualRows = 0;
+// Levels up to this many rows fuse prolongation into the first post-sweep
+// (saves a launch); on larger levels the extra map gather per nonzero costs
+// more than the separate prolongation pass.
+constexpr int kFusedProlongRows = 65536;
 
-int kgroup(int n, int64_t nnz) {
-    const double avg = n > 0 ? double(nnz) / double(n) : 1.0;
-    int g = 2;
-    while (g < 64 && g < avg) g <<= 1;
-    return g;
-}
 
-template <class T>
-__device__ __forceinline__ T group_row_dot(int s, int e, int lane, int G, const int* __restrict__ ci,
-                                           const T* __restrict__ v, const T* __restrict__ x) {
-    T acc = 0;
-    for (int j = s + lane; j < e; j += G) acc += ld_stream(v + j) * x[ld_stream(ci + j)];
-    return acc;
-}
-
-// Persistent row loop for the fused reduction kernels: the grid is capped at
-// kMaxReduceBlocks (one partial per block) and each XCD walks one contiguous
-// slab of logical row blocks.
-template <int G, class F>
-__device__ __forceinline__ void for_rows_xcd(int n, int nblocks, F&& f) {
+// Loop over LDS-staged row blocks for the fused reduction kernels (XCD
+// contiguous; with a grid below the block count each XCD walks its slab).
+// f(row, sum) runs in the thread owning the row.
+template <class F>
+__device__ __forceinline__ void for_stream_blocks(int n, int nblocks, int rpb, const int* __restrict__ rp,
+                                                  const int* __restrict__ ci, const double* __restrict__ v,
+                                                  const double* __restrict__ x, F&& f) {
+    __shared__ double prod[StreamCap<double>::value];
+    __shared__ int srp[kMaxStreamRows + 1];
     const int per = int(gridDim.x) / kXcds;
     const int xcd = int(blockIdx.x % kXcds), idx = int(blockIdx.x / kXcds);
     const int chunk = (nblocks + kXcds - 1) / kXcds;
     const int end = min(nblocks, (xcd + 1) * chunk);
-    constexpr int R = kBlock / G;
-    const int lane = int(threadIdx.x) % G;
     for (int lb = xcd * chunk + idx; lb < end; lb += per) {
-        const int row = lb * R + int(threadIdx.x) / G;
-        if (row < n) f(row, lane);
+        const int row0 = lb * rpb;
+        const int rows = min(rpb, n - row0);
+        double acc[kMaxStreamRows / kBlock];
+        csr_stream_rows<double>(row0, rows, rp, ci, v, [&](int col) { return x[col]; }, prod, srp, acc);
+#pragma unroll
+        for (int k = 0; k < kMaxStreamRows / kBlock; ++k) {
+            const int r = int(threadIdx.x) + k * kBlock;
+            if (r < rows) f(row0 + r, acc[k]);
+        }
     }
 }
 
@@ -161,11 +165,13 @@ __device__ void finalize(int mode, double total, KCtl* c) {
 
 __device__ __forceinline__ bool stopped(const KCtl* c) { return c->status != kRunning; }
 
-// Reduce `contrib` over the grid and run finalizer `mode`.
+// Block partial of a reduction whose finaliser runs in k_kfinal.
 __device__ __forceinline__ void reduce_fin(double contrib, double* partials, unsigned* ticket, KCtl* ctl, int mode) {
+    (void)ticket;
+    (void)ctl;
+    (void)mode;
     contrib = block_sum(contrib);
-    double total = 0;
-    if (grid_sum(contrib, partials, ticket, &total) && threadIdx.x == 0) finalize(mode, total, ctl);
+    if (threadIdx.x == 0) partials[blockIdx.x] = contrib;
 }
 
 // dot(x, y) -> finalizer.
@@ -177,80 +183,45 @@ __global__ __launch_bounds__(kBlock) void k_kdot(int n, const double* __restrict
     reduce_fin(s, partials, ticket, ctl, mode);
 }
 
-// y = A x fused with sum(w .* y) -> finalizer; skip when `early` is set if skipEarly.
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_kspmv_dot(int n, int nblocks, const int* __restrict__ rp,
+// y = A x fused with sum(w .* y) -> finalizer.
+__global__ __launch_bounds__(kBlock) void k_kspmv_dot(int n, int nblocks, int rpb, const int* __restrict__ rp,
                                                       const int* __restrict__ ci, const double* __restrict__ v,
                                                       const double* __restrict__ x, double* __restrict__ y,
                                                       const double* __restrict__ w, double* partials,
                                                       unsigned* ticket, KCtl* ctl, int mode) {
     if (stopped(ctl)) return;
     double contrib = 0;
-    for_rows_xcd<G>(n, nblocks, [&](int row, int lane) {
-        double acc = group_row_dot(rp[row], rp[row + 1], lane, G, ci, v, x);
-        acc = group_sum<G>(acc);
-        if (lane == 0) {
-            y[row] = acc;
-            contrib += w[row] * acc;
-        }
+    for_stream_blocks(n, nblocks, rpb, rp, ci, v, x, [&](int row, double acc) {
+        y[row] = acc;
+        contrib += w[row] * acc;
     });
     reduce_fin(contrib, partials, ticket, ctl, mode);
 }
 
 // t = A x fused with (t.t, t.s): BiCgStab.cuh:214-226 (omega).
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_kspmv_omega(int n, int nblocks, const int* __restrict__ rp,
+__global__ __launch_bounds__(kBlock) void k_kspmv_omega(int n, int nblocks, int rpb, const int* __restrict__ rp,
                                                         const int* __restrict__ ci, const double* __restrict__ v,
                                                         const double* __restrict__ x, double* __restrict__ t,
                                                         const double* __restrict__ s, double* partials,
                                                         unsigned* tickets, KCtl* ctl) {
     if (stopped(ctl) || ctl->early) return;
     double tt = 0, ts = 0;
-    for_rows_xcd<G>(n, nblocks, [&](int row, int lane) {
-        double acc = group_row_dot(rp[row], rp[row + 1], lane, G, ci, v, x);
-        acc = group_sum<G>(acc);
-        if (lane == 0) {
-            t[row] = acc;
-            tt += acc * acc;
-            ts += acc * s[row];
-        }
+    for_stream_blocks(n, nblocks, rpb, rp, ci, v, x, [&](int row, double acc) {
+        t[row] = acc;
+        tt += acc * acc;
+        ts += acc * s[row];
     });
     tt = block_sum(tt);
     ts = block_sum(ts);
-    // Two reductions share one ticket: partials for ts live after those for tt.
-    __shared__ int isLast;
     if (threadIdx.x == 0) {
         partials[blockIdx.x] = tt;
         partials[kMaxReduceBlocks + blockIdx.x] = ts;
-        const unsigned k = __hip_atomic_fetch_add(tickets, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        isLast = (k == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!isLast) return;
-    double a = 0, b = 0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
-        a += __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        b += __hip_atomic_load(partials + kMaxReduceBlocks + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    a = block_sum(a);
-    b = block_sum(b);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(tickets, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (!isfinite(a) || !isfinite(b)) ctl->status = kNonFinite;
-        else if (a == 0) ctl->status = kBreakdown;
-        else {
-            const double omega = b / a;
-            ctl->omega = omega;
-            if (!isfinite(omega)) ctl->status = kNonFinite;
-            else if (omega == 0) ctl->status = kBreakdown;
-        }
     }
 }
 
 // r = b - A x and sum(r.^2) when the control block asks for it; with `shadow`
 // the new residual is also copied into the shadow vector (BiCGStab restart).
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_ktrue(int n, int nblocks, const int* __restrict__ rp,
+__global__ __launch_bounds__(kBlock) void k_ktrue(int n, int nblocks, int rpb, const int* __restrict__ rp,
                                                   const int* __restrict__ ci, const double* __restrict__ v,
                                                   const double* __restrict__ b, const double* __restrict__ x,
                                                   double* __restrict__ r, double* __restrict__ shadow,
@@ -258,17 +229,41 @@ __global__ __launch_bounds__(kBlock) void k_ktrue(int n, int nblocks, const int*
     if (stopped(ctl)) return;
     if (onEarly ? !ctl->early : (!ctl->check || ctl->early)) return;
     double contrib = 0;
-    for_rows_xcd<G>(n, nblocks, [&](int row, int lane) {
-        double acc = group_row_dot(rp[row], rp[row + 1], lane, G, ci, v, x);
-        acc = group_sum<G>(acc);
-        if (lane == 0) {
-            const double res = b[row] - acc;
-            r[row] = res;
-            if (shadow) shadow[row] = res;
-            contrib += res * res;
-        }
+    for_stream_blocks(n, nblocks, rpb, rp, ci, v, x, [&](int row, double acc) {
+        const double res = b[row] - acc;
+        r[row] = res;
+        if (shadow) shadow[row] = res;
+        contrib += res * res;
     });
     reduce_fin(contrib, partials, ticket, ctl, kFinTrue);
+}
+
+// Finalisation of one reduction: adds the producer's block partials in
+// block order and applies the scalar recurrence.  It skips exactly when its
+// producer skipped (same control-block state), so partials are never stale.
+// mode kFinOmega reads two partial arrays (t.t, t.s).
+constexpr int kFinOmega = 100;
+__global__ __launch_bounds__(1024) void k_kfinal(const double* __restrict__ partials, int nparts, KCtl* ctl, int mode,
+                                                 int onEarly) {
+    if (stopped(ctl)) return;
+    if (mode == kFinTrue && (onEarly ? !ctl->early : (!ctl->check || ctl->early))) return;
+    if ((mode == kFinOmega || mode == kFinUpdBi) && ctl->early) return;
+    const double a = sum_partials(partials, nparts);
+    if (mode == kFinOmega) {
+        const double b = sum_partials(partials + kMaxReduceBlocks, nparts);
+        if (threadIdx.x == 0) {
+            if (!isfinite(a) || !isfinite(b)) ctl->status = kNonFinite;
+            else if (a == 0) ctl->status = kBreakdown;
+            else {
+                const double omega = b / a;
+                ctl->omega = omega;
+                if (!isfinite(omega)) ctl->status = kNonFinite;
+                else if (omega == 0) ctl->status = kBreakdown;
+            }
+        }
+        return;
+    }
+    if (threadIdx.x == 0) finalize(mode, a, ctl);
 }
 
 // CG direction: p = z (fresh) or p = z + beta p.
@@ -343,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(int n, const double* __res
 
 int vec_blocks(int n) {
     int b = (n + kBlock - 1) / kBlock;
-    if (b > 1024) b = 1024;
+    if (b > 8192) b = 8192;
     return b < 1 ? 1 : b;
 }
 
@@ -379,32 +374,30 @@ AmgEngine::AmgEngine(tf_ctx* ctx, const AmgOptionsC& o) : opt_(o), c_(ctx) {}
 AmgEngine::~AmgEngine() { release(); }
 
 namespace {
-void free_level(AmgLevel& l, bool f32) {
-    if (f32 && l.vals) (void)hipFree(l.vals);
+void free_level(tf_ctx& c, AmgLevel& l, bool f32) {
+    if (f32) dev_free(c, l.vals);
     l.vals = nullptr;
-    for (void** p : {&l.inv, &l.b, &l.x, &l.x2, &l.r})
-        if (*p) {
-            (void)hipFree(*p);
-            *p = nullptr;
-        }
-    free_csr(l.native);
-    free_agg(l.transfer);
+    for (void** p : {&l.inv, &l.b, &l.x, &l.x2, &l.r}) {
+        dev_free(c, *p);
+        *p = nullptr;
+    }
+    free_csr(c, l.native);
+    free_agg(c, l.transfer);
 }
 }  // namespace
 
 void AmgEngine::release() {
     if (c_) (void)hipStreamSynchronize(c_->stream);
-    for (auto& l : levels_) free_level(l, opt_.precision == 32);
+    for (auto& l : levels_) free_level(*c_, l, opt_.precision == 32);
     levels_.clear();
-    for (void* p : {static_cast<void*>(lu_), static_cast<void*>(piv_), static_cast<void*>(inverse_)})
-        if (p) (void)hipFree(p);
+    for (void* p : {static_cast<void*>(lu_), static_cast<void*>(piv_), static_cast<void*>(inverse_)}) dev_free(*c_, p);
     lu_ = nullptr;
     piv_ = nullptr;
     inverse_ = nullptr;
     factored_ = singular_ = false;
 }
 
-void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v) {
+void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v, bool canonicalInput) {
     const bool f32 = opt_.precision == 32;
     const size_t es = f32 ? sizeof(float) : sizeof(double);
     std::vector<AmgLevel> next;
@@ -414,33 +407,34 @@ void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const do
     bool factored = false, singular = false;
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(c_->stream);
-        for (auto& l : next) free_level(l, f32);
-        if (lu) (void)hipFree(lu);
-        if (piv) (void)hipFree(piv);
-        if (inv) (void)hipFree(inv);
+        for (auto& l : next) free_level(*c_, l, f32);
+        dev_free(*c_, lu);
+        dev_free(*c_, piv);
+        dev_free(*c_, inv);
     };
     try {
-        DevCsr native = canonicalize(*c_, n, nnz, rp, ci, v);
+        DevCsr native = canonicalInput ? copy_csr(*c_, n, nnz, rp, ci, v) : canonicalize(*c_, n, nnz, rp, ci, v);
+        int symmetric = -1;
         for (int depth = 0;; ++depth) {
             next.emplace_back();
             AmgLevel& L = next.back();
             L.native = native;
             const int ln = native.n;
-            if (!validate_rowmax(*c_, ln, native.rp, native.ci, native.v, nullptr))
+            if (!validate_rowmax(*c_, ln, native.rp, native.ci, native.v, rowmax_workspace(*c_, ln)))
                 throw Error(TF_E_NUMERIC,
                             "Native AMG: missing/zero diagonal, non-finite value or reciprocal in hierarchy");
             if (f32) {
-                L.vals = dev_alloc<float>(size_t(native.nnz));
+                L.vals = dev_alloc<float>(*c_, size_t(native.nnz));
                 convert<double, float>(*c_, size_t(native.nnz), native.v, static_cast<float*>(L.vals));
             }
             else {
                 L.vals = native.v;
             }
-            L.inv = dev_alloc<char>(size_t(ln) * es);
-            L.b = dev_alloc<char>(size_t(ln) * es);
-            L.x = dev_alloc<char>(size_t(ln) * es);
-            L.x2 = dev_alloc<char>(size_t(ln) * es);
-            L.r = dev_alloc<char>(size_t(ln) * es);
+            L.inv = dev_alloc<char>(*c_, size_t(ln) * es);
+            L.b = dev_alloc<char>(*c_, size_t(ln) * es);
+            L.x = dev_alloc<char>(*c_, size_t(ln) * es);
+            L.x2 = dev_alloc<char>(*c_, size_t(ln) * es);
+            L.r = dev_alloc<char>(*c_, size_t(ln) * es);
             c_->clear_flags(1);
             if (f32)
                 inverse_diagonal<float>(*c_, ln, native.rp, native.ci, static_cast<const float*>(L.vals),
@@ -458,9 +452,12 @@ void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const do
                 stop = "level limit";
                 break;
             }
-            DevCsr g = strength_graph(*c_, ln, native.nnz, native.rp, native.ci, native.v, opt_.theta);
+            // Galerkin products of a pattern-symmetric matrix are pattern-symmetric, so
+            // the check runs on the finest level only.
+            DevCsr g = strength_graph(*c_, ln, native.nnz, native.rp, native.ci, native.v, opt_.theta, symmetric,
+                                      true, &symmetric);
             if (g.nnz == 0) {
-                free_csr(g);
+                free_csr(*c_, g);
                 stop = "empty strength graph";
                 break;
             }
@@ -469,12 +466,12 @@ void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const do
                 t = aggregate(*c_, g, opt_.rounds, opt_.absorb);
             }
             catch (...) {
-                free_csr(g);
+                free_csr(*c_, g);
                 throw;
             }
-            free_csr(g);
+            free_csr(*c_, g);
             if (t.nc >= ln) {
-                free_agg(t);
+                free_agg(*c_, t);
                 stop = "no contraction";
                 break;
             }
@@ -484,9 +481,9 @@ void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const do
         }
         const DevCsr& last = next.back().native;
         if (last.n <= std::max(64, opt_.coarseRows)) {
-            lu = dev_alloc<double>(size_t(last.n) * size_t(last.n));
-            inv = dev_alloc<double>(size_t(last.n) * size_t(last.n));
-            piv = dev_alloc<int>(size_t(last.n));
+            lu = dev_alloc<double>(*c_, size_t(last.n) * size_t(last.n));
+            inv = dev_alloc<double>(*c_, size_t(last.n) * size_t(last.n));
+            piv = dev_alloc<int>(*c_, size_t(last.n));
             factored = coarse_lu_factor(*c_, last.n, last.rp, last.ci, last.v, lu, piv, inv);
             singular = !factored;
         }
@@ -509,10 +506,10 @@ void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const do
 }
 
 template <class L>
-void AmgEngine::smooth(size_t d, int sweeps) {
+void AmgEngine::smooth(size_t d, int sweeps, int first) {
     AmgLevel& l = levels_[d];
     const DevCsr& A = l.native;
-    for (int s = 0; s < sweeps; ++s) {
+    for (int s = first; s < sweeps; ++s) {
         if (l.zero) {
             jacobi_fused<L>(*c_, A.n, A.nnz, A.rp, A.ci, static_cast<const L*>(l.vals), static_cast<const L*>(l.inv),
                             L(opt_.omega), static_cast<const L*>(l.b), nullptr, static_cast<L*>(l.x));
@@ -527,6 +524,35 @@ void AmgEngine::smooth(size_t d, int sweeps) {
     }
 }
 
+// Restrict the rhs (ofResidual == false) or the residual of level d into the
+// rhs of level d+1.  With fuseFirstSweep the coarse level's first Jacobi
+// sweep from zero is computed in the same kernel.
+template <class L>
+void AmgEngine::restrict_to(size_t d, bool ofResidual, bool fuseFirstSweep) {
+    AmgLevel& l = levels_[d];
+    AmgLevel& nx = levels_[d + 1];
+    const DevAgg& t = l.transfer;
+    const L* inv = fuseFirstSweep ? static_cast<const L*>(nx.inv) : nullptr;
+    L* xc = fuseFirstSweep ? static_cast<L*>(nx.x) : nullptr;
+    const L w = L(opt_.omega);
+    if (!ofResidual || l.zero) {
+        restrict_agg<L>(*c_, t.nc, t.offsets, t.members, static_cast<const L*>(l.b), static_cast<L*>(nx.b), inv, w,
+                        xc);
+    }
+    else if (l.native.n <= kFusedResidualRows) {
+        residual_restrict<L>(*c_, t.nc, t.offsets, t.members, l.native.rp, l.native.ci,
+                             static_cast<const L*>(l.vals), static_cast<const L*>(l.b), static_cast<const L*>(l.x),
+                             static_cast<L*>(nx.b), inv, w, xc);
+    }
+    else {
+        residual<L>(*c_, l.native.n, l.native.nnz, l.native.rp, l.native.ci, static_cast<const L*>(l.vals),
+                    static_cast<const L*>(l.b), static_cast<const L*>(l.x), static_cast<L*>(l.r));
+        restrict_agg<L>(*c_, t.nc, t.offsets, t.members, static_cast<const L*>(l.r), static_cast<L*>(nx.b), inv, w,
+                        xc);
+    }
+    nx.preapplied = fuseFirstSweep;
+}
+
 template <class L>
 void AmgEngine::coarse(size_t d) {
     AmgLevel& l = levels_[d];
@@ -539,53 +565,72 @@ void AmgEngine::coarse(size_t d) {
     }
 }
 
+// executeCycle (Cycle.cuh:35-63) with the reference's operations mapped to
+// fused kernels; the clear of a zero initial guess is tracked lazily.
 template <class L>
 void AmgEngine::cycle(int kind, size_t d, bool zero) {
     ++visits_[d];
     AmgLevel& l = levels_[d];
-    if (zero || kind == 2) l.zero = true;
+    int done = 0;
+    if (l.preapplied) {
+        l.zero = false;
+        l.preapplied = false;
+        done = 1;
+    }
+    else if (zero || kind == 2) {
+        l.zero = true;
+    }
     if (d + 1 == levels_.size()) {
         coarse<L>(d);
         return;
     }
     AmgLevel& nx = levels_[d + 1];
     const DevAgg& t = l.transfer;
+    const bool nxSmooths = d + 2 < levels_.size() && opt_.pre >= 1;
     if (kind == 2) {
-        restrict_agg<L>(*c_, t.nc, t.offsets, t.members, static_cast<const L*>(l.b), static_cast<L*>(nx.b));
+        restrict_to<L>(d, false, false);
         cycle<L>(2, d + 1, true);
         prolong_agg<L>(*c_, l.native.n, t.map, static_cast<const L*>(nx.x), static_cast<L*>(l.x), l.zero);
         l.zero = false;
         cycle<L>(0, d, false);
         return;
     }
-    smooth<L>(d, opt_.pre);
-    if (l.zero) {
-        restrict_agg<L>(*c_, t.nc, t.offsets, t.members, static_cast<const L*>(l.b), static_cast<L*>(nx.b));
-    }
-    else {
-        residual<L>(*c_, l.native.n, l.native.nnz, l.native.rp, l.native.ci, static_cast<const L*>(l.vals),
-                    static_cast<const L*>(l.b), static_cast<const L*>(l.x), static_cast<L*>(l.r));
-        restrict_agg<L>(*c_, t.nc, t.offsets, t.members, static_cast<const L*>(l.r), static_cast<L*>(nx.b));
-    }
+    smooth<L>(d, opt_.pre, done);
+    restrict_to<L>(d, true, nxSmooths);
     cycle<L>(kind, d + 1, true);
     if (kind == 1) cycle<L>(kind, d + 1, false);
-    prolong_agg<L>(*c_, l.native.n, t.map, static_cast<const L*>(nx.x), static_cast<L*>(l.x), l.zero);
-    l.zero = false;
-    smooth<L>(d, opt_.post);
+    if (opt_.post >= 1 && l.native.n <= kFusedProlongRows) {
+        const DevCsr& A = l.native;
+        prolong_jacobi<L>(*c_, A.n, A.nnz, A.rp, A.ci, static_cast<const L*>(l.vals), t.map,
+                          static_cast<const L*>(nx.x), static_cast<const L*>(l.x), l.zero, static_cast<L*>(l.x2),
+                          static_cast<const L*>(l.b), static_cast<const L*>(l.inv), L(opt_.omega));
+        std::swap(l.x, l.x2);
+        l.zero = false;
+        smooth<L>(d, opt_.post, 1);
+    }
+    else {
+        prolong_agg<L>(*c_, l.native.n, t.map, static_cast<const L*>(nx.x), static_cast<L*>(l.x), l.zero);
+        l.zero = false;
+        smooth<L>(d, opt_.post);
+    }
 }
 
 template <class L>
 void AmgEngine::apply_typed(const double* rhs, double* x) {
     AmgLevel& l0 = levels_[0];
-    const size_t n = size_t(l0.native.n);
-    convert<double, L>(*c_, n, rhs, static_cast<L*>(l0.b));
+    const int n = l0.native.n;
+    const bool first = opt_.cycle != 2 && levels_.size() > 1 && opt_.pre >= 1;
+    enter_level<L>(*c_, n, rhs, static_cast<L*>(l0.b), static_cast<const L*>(l0.inv), L(opt_.omega),
+                   first ? static_cast<L*>(l0.x) : nullptr);
+    for (auto& l : levels_) l.preapplied = false;
+    l0.preapplied = first;
     std::fill(visits_.begin(), visits_.end(), 0);
     cycle<L>(opt_.cycle, 0, true);
     if (l0.zero) {
-        fill_zero(*c_, x, n * sizeof(double));
+        fill_zero(*c_, x, size_t(n) * sizeof(double));
     }
     else {
-        convert<L, double>(*c_, n, static_cast<const L*>(l0.x), x);
+        convert<L, double>(*c_, size_t(n), static_cast<const L*>(l0.x), x);
     }
 }
 
@@ -620,8 +665,8 @@ void SolverEngine::release_graphs() {
 SolverEngine::~SolverEngine() {
     if (c_) (void)hipStreamSynchronize(c_->stream);
     release_graphs();
-    free_csr(A_);
-    if (vec_) (void)hipFree(vec_);
+    free_csr(*c_, A_);
+    dev_free(*c_, vec_);
     if (ctl_) (void)hipFree(ctl_);
     if (hctl_) (void)hipHostFree(hctl_);
 }
@@ -631,15 +676,14 @@ void SolverEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const
     release_graphs();
     DevCsr A = canonicalize(*c_, n, nnz, rp, ci, v);
     if (vec_) {
-        c_->sync();
-        (void)hipFree(vec_);
+        dev_free(*c_, vec_);
         vec_ = nullptr;
     }
-    free_csr(A_);
+    free_csr(*c_, A_);
     A_ = A;
     n_ = n;
     const int nv = kind_ == 0 ? 6 : 11;
-    vec_ = dev_alloc<double>(size_t(n) * nv);
+    vec_ = dev_alloc<double>(*c_, size_t(n) * nv);
     double* p = vec_;
     auto take = [&]() {
         double* q = p;
@@ -659,7 +703,7 @@ void SolverEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const
         ph_ = take();
         shat_ = take();
     }
-    if (amg_) amg_->setup(A_.n, A_.nnz, A_.rp, A_.ci, A_.v);
+    if (amg_) amg_->setup(A_.n, A_.nnz, A_.rp, A_.ci, A_.v, true);
     c_->sync();
 }
 
@@ -682,48 +726,54 @@ double SolverEngine::norm_host(const double* x) {
 void SolverEngine::launch_part(int part) {
     (void)part;
     KCtl* ctl = static_cast<KCtl*>(ctl_);
-    const int g = kgroup(A_.n, A_.nnz);
-    const int rows = kBlock / g;
-    const int nb = (A_.n + rows - 1) / rows;
+    const int rpb = stream_rows_per_block<double>(A_.n, A_.nnz);
+    const int nb = (A_.n + rpb - 1) / rpb;
     const dim3 grid(std::min(round_xcd(nb), kMaxReduceBlocks)), vb(vec_blocks(n_));
     double* P = c_->partials;
     unsigned* T = c_->tickets;
     auto pslot = [&](int s) { return P + size_t(s) * kMaxReduceBlocks; };
-#define TF_G(NAME, ...)                                                                                 \
-    switch (g) {                                                                                       \
-    case 2: hipLaunchKernelGGL(NAME<2>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;        \
-    case 4: hipLaunchKernelGGL(NAME<4>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;        \
-    case 8: hipLaunchKernelGGL(NAME<8>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;        \
-    case 16: hipLaunchKernelGGL(NAME<16>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;      \
-    case 32: hipLaunchKernelGGL(NAME<32>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;      \
-    default: hipLaunchKernelGGL(NAME<64>, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__); break;      \
-    }
+#define TF_G(NAME, ...) hipLaunchKernelGGL(NAME, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__)
+    const int ng = int(grid.x), nv = int(vb.x);
+    auto fin = [&](int slot, int parts, int mode, int onEarly) {
+        hipLaunchKernelGGL(k_kfinal, dim3(1), dim3(1024), 0, c_->stream, pslot(slot), parts, ctl, mode, onEarly);
+    };
     if (kind_ == 0) {
         precondition(r_, z_);
         hipLaunchKernelGGL(k_kdot, vb, dim3(kBlock), 0, c_->stream, n_, r_, z_, pslot(kSlotA), T + kSlotA, ctl,
                            int(kFinRhoCg));
+        fin(kSlotA, nv, kFinRhoCg, 0);
         hipLaunchKernelGGL(k_cg_p, vb, dim3(kBlock), 0, c_->stream, n_, z_, p_, ctl);
-        TF_G(k_kspmv_dot, A_.n, nb, A_.rp, A_.ci, A_.v, p_, q_, p_, pslot(kSlotB), T + kSlotB, ctl, int(kFinCurvCg));
+        TF_G(k_kspmv_dot, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, p_, q_, p_, pslot(kSlotB), T + kSlotB, ctl, int(kFinCurvCg));
+        fin(kSlotB, ng, kFinCurvCg, 0);
         hipLaunchKernelGGL(k_cg_update, vb, dim3(kBlock), 0, c_->stream, n_, p_, q_, x_, r_, pslot(kSlotC),
                            T + kSlotC, ctl);
-        TF_G(k_ktrue, A_.n, nb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, static_cast<double*>(nullptr), pslot(kSlotD),
+        fin(kSlotC, nv, kFinUpdCg, 0);
+        TF_G(k_ktrue, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, static_cast<double*>(nullptr), pslot(kSlotD),
              T + kSlotD, ctl, 0);
+        fin(kSlotD, ng, kFinTrue, 0);
     }
     else {
         double* v = q_;
         hipLaunchKernelGGL(k_kdot, vb, dim3(kBlock), 0, c_->stream, n_, sh_, r_, pslot(kSlotA), T + kSlotA, ctl,
                            int(kFinRhoBi));
+        fin(kSlotA, nv, kFinRhoBi, 0);
         hipLaunchKernelGGL(k_bi_p, vb, dim3(kBlock), 0, c_->stream, n_, r_, v, p_, ctl);
         precondition(p_, ph_);
-        TF_G(k_kspmv_dot, A_.n, nb, A_.rp, A_.ci, A_.v, ph_, v, sh_, pslot(kSlotB), T + kSlotB, ctl, int(kFinDenBi));
+        TF_G(k_kspmv_dot, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, ph_, v, sh_, pslot(kSlotB), T + kSlotB, ctl, int(kFinDenBi));
+        fin(kSlotB, ng, kFinDenBi, 0);
         hipLaunchKernelGGL(k_bi_s, vb, dim3(kBlock), 0, c_->stream, n_, r_, v, ph_, s_, x_, pslot(kSlotC), T + kSlotC,
                            ctl);
-        TF_G(k_ktrue, A_.n, nb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 1);
+        fin(kSlotC, nv, kFinSBi, 0);
+        TF_G(k_ktrue, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 1);
+        fin(kSlotD, ng, kFinTrue, 1);
         precondition(s_, shat_);
-        TF_G(k_kspmv_omega, A_.n, nb, A_.rp, A_.ci, A_.v, shat_, t_, s_, pslot(kSlotE), T + kSlotE, ctl);
+        TF_G(k_kspmv_omega, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, shat_, t_, s_, pslot(kSlotE), T + kSlotE, ctl);
+        fin(kSlotE, ng, kFinOmega, 0);
         hipLaunchKernelGGL(k_bi_update, vb, dim3(kBlock), 0, c_->stream, n_, shat_, s_, t_, x_, r_, pslot(kSlotA),
                            T + kSlotA, ctl);
-        TF_G(k_ktrue, A_.n, nb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 0);
+        fin(kSlotA, nv, kFinUpdBi, 0);
+        TF_G(k_ktrue, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 0);
+        fin(kSlotD, ng, kFinTrue, 0);
     }
 #undef TF_G
     check(hipGetLastError(), "Krylov iteration launch");

@@ -28,6 +28,7 @@ struct AmgLevel {
     void* r = nullptr;      // residual
     DevAgg transfer;        // to the next coarser level (empty on the coarsest)
     bool zero = false;      // iterate is known to be zero (cleared lazily)
+    bool preapplied = false; // x already holds the first sweep from zero (fused upstream)
     int64_t hist[4] = {0, 0, 0, 0};
 };
 
@@ -35,7 +36,9 @@ class AmgEngine {
 public:
     AmgEngine(tf_ctx* ctx, const AmgOptionsC& o);
     ~AmgEngine();
-    void setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v);  // device arrays
+    // Device arrays; canonicalInput skips the canonical-form check/copy (the
+    // caller guarantees sorted unique columns and finite values).
+    void setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v, bool canonicalInput = false);
     // z = M^-1 r (FP64 device vectors; may alias).  Enqueued on the context stream.
     void apply(const double* rhs, double* x);
     bool ready() const { return !levels_.empty(); }
@@ -50,7 +53,9 @@ private:
     template <class L>
     void cycle(int kind, size_t d, bool zero);
     template <class L>
-    void smooth(size_t d, int sweeps);
+    void smooth(size_t d, int sweeps, int first = 0);
+    template <class L>
+    void restrict_to(size_t d, bool ofResidual, bool fuseFirstSweep);
     template <class L>
     void coarse(size_t d);
     template <class L>

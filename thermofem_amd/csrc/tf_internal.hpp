@@ -4,8 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 
 #include "../../include/thermofem_hip.h"
 
@@ -47,7 +49,7 @@ inline void require(bool ok, const char* msg, int code = TF_E_INVALID) {
     if (!ok) throw Error(code, msg);
 }
 
-constexpr int kMaxReduceBlocks = 2048;
+constexpr int kMaxReduceBlocks = 16384;
 constexpr int kTickets = 64;
 
 // Scoped current-device switch that restores the caller's device
@@ -66,8 +68,53 @@ struct DeviceScope {
 
 }  // namespace tf
 
+// Per-context cache of device allocations.  Blocks released by the
+// library's own structures (level matrices, transfers, vectors) are kept and
+// handed out again for a request of similar size, so repeated setups do not
+// pay hipMalloc/hipFree per level.  Blocks are reused only by the context
+// that released them; its single stream orders old and new users.
+struct DevicePool {
+    std::multimap<size_t, void*> free_;
+    std::unordered_map<void*, size_t> live_;
+    void* get(size_t bytes) {
+        bytes = (bytes + 255) / 256 * 256;
+        if (bytes == 0) bytes = 256;
+        auto it = free_.lower_bound(bytes);
+        if (it != free_.end() && it->first <= bytes + bytes / 4 + 65536) {
+            void* p = it->second;
+            live_[p] = it->first;
+            free_.erase(it);
+            return p;
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            trim();
+            tf::check(hipMalloc(&p, bytes), "hipMalloc");
+        }
+        live_[p] = bytes;
+        return p;
+    }
+    void put(void* p) {
+        if (!p) return;
+        auto it = live_.find(p);
+        if (it == live_.end()) {
+            (void)hipFree(p);
+            return;
+        }
+        free_.insert({it->second, p});
+        live_.erase(it);
+    }
+    void trim() {
+        (void)hipDeviceSynchronize();
+        for (auto& kv : free_) (void)hipFree(kv.second);
+        free_.clear();
+    }
+};
+
 struct tf_ctx {
     int device = 0;
+    DevicePool pool;
     hipStream_t stream = nullptr;
     double* partials = nullptr;   // per-block partial sums (kMaxReduceBlocks x kTickets slots)
     unsigned* tickets = nullptr;  // last-block tickets, zero between launches
@@ -77,6 +124,7 @@ struct tf_ctx {
     int* hflags = nullptr;        // pinned host mirror
     void* scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
+    int spmv_layout = 0;       // 0: LDS-staged row blocks, 1: sub-wave row groups
     void* ws_ptr[32] = {};     // grow-only setup workspace slots
     size_t ws_bytes[32] = {};
 
@@ -156,7 +204,16 @@ void dot_device(tf_ctx& c, int n, const T* x, const T* y, double* dst, int slot)
 template <class A, class B>
 void convert(tf_ctx& c, size_t n, const A* in, B* out);
 template <class T>
-void restrict_agg(tf_ctx& c, int nc, const int* offsets, const int* members, const T* fine, T* coarse);
+void restrict_agg(tf_ctx& c, int nc, const int* offsets, const int* members, const T* fine, T* coarse,
+                  const T* inv = nullptr, T w = T(0), T* xc = nullptr);
+template <class T>
+void residual_restrict(tf_ctx& c, int nc, const int* offsets, const int* members, const int* rp, const int* ci,
+                       const T* v, const T* b, const T* x, T* coarse, const T* inv, T w, T* xc);
+template <class T>
+void prolong_jacobi(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const int* map,
+                    const T* xc, const T* x, bool zero, T* xout, const T* b, const T* inv, T w);
+template <class T>
+void enter_level(tf_ctx& c, int n, const double* rhs, T* b, const T* inv, T w, T* x);
 template <class T>
 void prolong_agg(tf_ctx& c, int n, const int* map, const T* coarse, T* fine, bool set = false);
 template <class T>
@@ -179,7 +236,7 @@ struct DevCsr {
     int* ci = nullptr;
     double* v = nullptr;
 };
-void free_csr(DevCsr& a);
+void free_csr(tf_ctx& c, DevCsr& a);
 // Aggregation transfer (AggregateTransfer, GpuAggregation.cuh:13-17).
 struct DevAgg {
     int n = 0, nc = 0;
@@ -188,19 +245,22 @@ struct DevAgg {
     int* members = nullptr;
     int64_t hist[4] = {0, 0, 0, 0};
 };
-void free_agg(DevAgg& t);
+void free_agg(tf_ctx& c, DevAgg& t);
 template <class T>
-T* dev_alloc(size_t count) {
-    void* p = nullptr;
-    check(hipMalloc(&p, (count ? count : 1) * sizeof(T)), "hipMalloc");
-    return static_cast<T*>(p);
+T* dev_alloc(tf_ctx& c, size_t count) {
+    return static_cast<T*>(c.pool.get((count ? count : 1) * sizeof(T)));
 }
+inline void dev_free(tf_ctx& c, void* p) { c.pool.put(p); }
 int csr_check(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v);
 bool csr_structure_ok(tf_ctx& c, int rows, int cols, int64_t nnz, const int* rp, const int* ci);
 DevCsr canonicalize(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v);
 DevCsr copy_csr(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v);
 bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax);
-DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta);
+// symmetric: -1 unknown (checked here), 0/1 known pattern symmetry.
+// prevalidated: the caller ran validate_rowmax with rowmax_workspace(c, n).
+DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta,
+                      int symmetric = -1, bool prevalidated = false, int* symmetricOut = nullptr);
+double* rowmax_workspace(tf_ctx& c, int n);
 void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal);
 int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map);
 DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc);

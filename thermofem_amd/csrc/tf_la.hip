@@ -1,5 +1,5 @@
-// Linear-algebra kernels for gfx950: CSR SpMV with sub-wave row groups and
-// fused epilogues (residual, weighted Jacobi), SpMM over row-major blocks of
+// Linear-algebra kernels for gfx950: CSR SpMV over LDS-staged row blocks
+// (or sub-wave row groups) with fused epilogues (residual, weighted Jacobi), SpMM over row-major blocks of
 // right-hand sides, level-1 BLAS with deterministic reductions, aggregate
 // restriction/prolongation and element conversions.
 #include "tf_device.hpp"
@@ -10,10 +10,51 @@ namespace {
 
 enum Epilogue { kSpmv = 0, kResidual = 1, kJacobi = 2 };
 
-// One group of G lanes per row; the groups of a 256-thread block own 256/G
-// consecutive rows.  Column indices and values are streamed with
-// non-temporal loads (read once per product); x is gathered through the
-// cache, where the XCD-contiguous block mapping keeps stencil neighbours in L2.
+// CSR row products with fused epilogues.  Two layouts:
+//  * stream (default): LDS-staged row blocks (csr_stream_rows), coalesced
+//    unrolled streaming of columns/values, sequential row sums;
+//  * group: G lanes per row with a shuffle reduction (kept for very long
+//    rows and as an autotuner candidate).
+// Logical blocks are laid out XCD-contiguously.  Epilogues are evaluated
+// without contraction: y = sum, r = b - sum, x + (w*inv)*(b - sum).
+template <class T, int E>
+__device__ __forceinline__ void epilogue(int row, T acc, const T* __restrict__ x, T* __restrict__ y,
+                                         const T* __restrict__ b, const T* __restrict__ inv, T w) {
+#pragma clang fp contract(off)
+    if constexpr (E == kSpmv) {
+        y[row] = acc;
+    }
+    else if constexpr (E == kResidual) {
+        y[row] = b[row] - acc;
+    }
+    else {
+        const T wd = w * inv[row];
+        const T d = b[row] - acc;
+        const T upd = wd * d;
+        y[row] = x[row] + upd;
+    }
+}
+
+template <class T, int E>
+__global__ __launch_bounds__(kBlock) void k_csr_stream(int n, int nblocks, int rpb, const int* __restrict__ rp,
+                                                       const int* __restrict__ ci, const T* __restrict__ v,
+                                                       const T* __restrict__ x, T* __restrict__ y,
+                                                       const T* __restrict__ b, const T* __restrict__ inv, T w) {
+    __shared__ T prod[StreamCap<T>::value];
+    __shared__ int srp[kMaxStreamRows + 1];
+    const int blk = xcd_block(nblocks);
+    if (blk < 0) return;
+    const int row0 = blk * rpb;
+    const int rows = min(rpb, n - row0);
+    T acc[kMaxStreamRows / kBlock];
+    csr_stream_rows<T>(row0, rows, rp, ci, v, [&](int col) { return x[col]; }, prod, srp, acc);
+#pragma unroll
+    for (int k = 0; k < kMaxStreamRows / kBlock; ++k) {
+        const int r = int(threadIdx.x) + k * kBlock;
+        if (r < rows) epilogue<T, E>(row0 + r, acc[k], x, y, b, inv, w);
+    }
+}
+
 template <int G, class T, int E>
 __global__ __launch_bounds__(kBlock) void k_csr_rows(int n, int nblocks, const int* __restrict__ rp,
                                                      const int* __restrict__ ci, const T* __restrict__ v,
@@ -29,24 +70,18 @@ __global__ __launch_bounds__(kBlock) void k_csr_rows(int n, int nblocks, const i
     T acc = 0;
     for (int j = s + lane; j < e; j += G) acc += ld_stream(v + j) * x[ld_stream(ci + j)];
     acc = group_sum<G>(acc);
-    if (lane == 0) {
-        if constexpr (E == kSpmv) {
-            y[row] = acc;
-        }
-        else if constexpr (E == kResidual) {
-            y[row] = b[row] - acc;
-        }
-        else {
-            y[row] = x[row] + w * inv[row] * (b[row] - acc);
-        }
-    }
+    if (lane == 0) epilogue<T, E>(row, acc, x, y, b, inv, w);
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_jacobi_zero(int n, const T* __restrict__ inv, T w,
                                                         const T* __restrict__ b, T* __restrict__ xout) {
+#pragma clang fp contract(off)
     const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i < n) xout[i] = w * inv[i] * b[i];
+    if (i < n) {
+        const T wd = w * inv[i];
+        xout[i] = wd * b[i];
+    }
 }
 
 int group_width(int n, int64_t nnz) {
@@ -60,6 +95,15 @@ template <class T, int E>
 void launch_rows(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const T* x, T* y,
                  const T* b, const T* inv, T w) {
     if (n <= 0) return;
+    const double avg = double(nnz) / double(n);
+    if (c.spmv_layout == 0 && avg <= 256) {
+        const int rpb = stream_rows_per_block<T>(n, nnz);
+        const int nb = (n + rpb - 1) / rpb;
+        hipLaunchKernelGGL((k_csr_stream<T, E>), dim3(round_xcd(nb)), dim3(kBlock), 0, c.stream, n, nb, rpb, rp, ci,
+                           v, x, y, b, inv, w);
+        check(hipGetLastError(), "csr stream kernel launch");
+        return;
+    }
     const int g = group_width(n, nnz);
     const int rows = kBlock / g;
     const int nb = (n + rows - 1) / rows;
@@ -100,8 +144,14 @@ __global__ __launch_bounds__(kBlock) void k_spmm(int n, int k, const int* __rest
 template <class T>
 __global__ void k_jacobi_update(int n, T w, const T* __restrict__ inv, const T* __restrict__ b,
                                 const T* __restrict__ p, T* __restrict__ x) {
+#pragma clang fp contract(off)
     const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i < n) x[i] += w * inv[i] * (b[i] - p[i]);
+    if (i < n) {
+        const T wd = w * inv[i];
+        const T d = b[i] - p[i];
+        const T upd = wd * d;
+        x[i] = x[i] + upd;
+    }
 }
 
 template <class T>
@@ -119,12 +169,17 @@ __global__ void k_scal(int n, T alpha, T* __restrict__ x) {
 // result, independent of timing.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_dot(int n, const T* __restrict__ x, const T* __restrict__ y,
-                                                double* partials, unsigned* ticket, double* dst) {
+                                                double* partials) {
     double s = 0;
     for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock)
         s += double(x[i]) * double(y[i]);
     s = block_sum(s);
-    grid_sum(s, partials, ticket, dst);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(1024) void k_dot_final(const double* __restrict__ partials, int nparts, double* dst) {
+    const double s = sum_partials(partials, nparts);
+    if (threadIdx.x == 0) *dst = s;
 }
 
 template <class A, class B>
@@ -134,14 +189,100 @@ __global__ void k_convert(size_t n, const A* __restrict__ in, B* __restrict__ ou
 }
 
 // coarse[c] = sum of fine[members] in ascending member order, starting from 0.
+// With xc set, also the coarse level's first Jacobi sweep from a zero
+// iterate: xc[c] = (w*inv[c]) * coarse[c].
 template <class T>
 __global__ void k_restrict(int nc, const int* __restrict__ off, const int* __restrict__ mem, const T* __restrict__ f,
-                           T* __restrict__ cvec) {
+                           T* __restrict__ cvec, const T* __restrict__ inv, T w, T* __restrict__ xc) {
+#pragma clang fp contract(off)
     const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (c >= nc) return;
     T s = 0;
     for (int k = off[c]; k < off[c + 1]; ++k) s += f[mem[k]];
     cvec[c] = s;
+    if (xc) {
+        const T wd = w * inv[c];
+        xc[c] = wd * s;
+    }
+}
+
+// Residual and restriction in one pass for one aggregate per thread:
+// coarse[c] = sum over members m (ascending) of (b[m] - sum_j a_mj x[j]),
+// each row sum sequential in stored order -- the same roundings as the
+// separate residual and restriction kernels.  Optional first coarse sweep
+// from zero as in k_restrict.
+template <class T>
+__global__ void k_residual_restrict(int nc, const int* __restrict__ off, const int* __restrict__ mem,
+                                    const int* __restrict__ rp, const int* __restrict__ ci, const T* __restrict__ v,
+                                    const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ cvec,
+                                    const T* __restrict__ inv, T w, T* __restrict__ xc) {
+#pragma clang fp contract(off)
+    const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (c >= nc) return;
+    T s = 0;
+    for (int k = off[c]; k < off[c + 1]; ++k) {
+        const int m = mem[k];
+        T acc = 0;
+        for (int j = rp[m]; j < rp[m + 1]; ++j) {
+            const T prod = v[j] * x[ci[j]];
+            acc += prod;
+        }
+        const T r = b[m] - acc;
+        s += r;
+    }
+    cvec[c] = s;
+    if (xc) {
+        const T wd = w * inv[c];
+        xc[c] = wd * s;
+    }
+}
+
+// Prolongation fused into the first post-smoothing sweep: with
+// x'_j = x_j + xc[map_j] (or xc[map_j] when x is zero), computes
+// xout = x' + (w*inv)*(b - A x'), never storing x'.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prolong_jacobi(int n, int nblocks, int rpb, const int* __restrict__ rp,
+                                                           const int* __restrict__ ci, const T* __restrict__ v,
+                                                           const int* __restrict__ map, const T* __restrict__ xcv,
+                                                           const T* __restrict__ x, int zero, T* __restrict__ y,
+                                                           const T* __restrict__ b, const T* __restrict__ inv, T w) {
+#pragma clang fp contract(off)
+    __shared__ T prod[StreamCap<T>::value];
+    __shared__ int srp[kMaxStreamRows + 1];
+    const int blk = xcd_block(nblocks);
+    if (blk < 0) return;
+    const int row0 = blk * rpb;
+    const int rows = min(rpb, n - row0);
+    auto xp = [&](int col) -> T { return zero ? xcv[map[col]] : x[col] + xcv[map[col]]; };
+    T acc[kMaxStreamRows / kBlock];
+    csr_stream_rows<T>(row0, rows, rp, ci, v, xp, prod, srp, acc);
+#pragma unroll
+    for (int k = 0; k < kMaxStreamRows / kBlock; ++k) {
+        const int r = int(threadIdx.x) + k * kBlock;
+        if (r < rows) {
+            const int row = row0 + r;
+            const T xr = xp(row);
+            const T wd = w * inv[row];
+            const T d = b[row] - acc[k];
+            const T upd = wd * d;
+            y[row] = xr + upd;
+        }
+    }
+}
+
+// Level entry: b = (T) rhs and, optionally, the first sweep from zero x = (w*inv)*b.
+template <class T>
+__global__ void k_enter(int n, const double* __restrict__ rhs, T* __restrict__ b, const T* __restrict__ inv, T w,
+                        T* __restrict__ x) {
+#pragma clang fp contract(off)
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
+        const T bi = T(rhs[i]);
+        b[i] = bi;
+        if (x) {
+            const T wd = w * inv[i];
+            x[i] = wd * bi;
+        }
+    }
 }
 
 // fine += P coarse, or fine = P coarse when the fine iterate is known zero.
@@ -240,10 +381,12 @@ void scal(tf_ctx& c, int n, T alpha, T* x) {
 template <class T>
 void dot_device(tf_ctx& c, int n, const T* x, const T* y, double* dst, int slot) {
     require(slot >= 0 && slot < kTickets, "reduction slot out of range");
-    const int blocks = blocks_for(size_t(n > 0 ? n : 1), 1024);
-    hipLaunchKernelGGL(k_dot<T>, dim3(blocks), dim3(kBlock), 0, c.stream, n, x, y,
-                       c.partials + size_t(slot) * kMaxReduceBlocks, c.tickets + slot, dst);
+    const int blocks = blocks_for(size_t(n > 0 ? n : 1), 8192);
+    double* part = c.partials + size_t(slot) * kMaxReduceBlocks;
+    hipLaunchKernelGGL(k_dot<T>, dim3(blocks), dim3(kBlock), 0, c.stream, n, x, y, part);
     check(hipGetLastError(), "dot launch");
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(1024), 0, c.stream, part, blocks, dst);
+    check(hipGetLastError(), "dot final launch");
 }
 template <class A, class B>
 void convert(tf_ctx& c, size_t n, const A* in, B* out) {
@@ -252,11 +395,37 @@ void convert(tf_ctx& c, size_t n, const A* in, B* out) {
     check(hipGetLastError(), "convert launch");
 }
 template <class T>
-void restrict_agg(tf_ctx& c, int nc, const int* offsets, const int* members, const T* fine, T* coarse) {
+void restrict_agg(tf_ctx& c, int nc, const int* offsets, const int* members, const T* fine, T* coarse, const T* inv,
+                  T w, T* xc) {
     if (nc <= 0) return;
     hipLaunchKernelGGL(k_restrict<T>, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, nc, offsets,
-                       members, fine, coarse);
+                       members, fine, coarse, inv, w, xc);
     check(hipGetLastError(), "restrict launch");
+}
+template <class T>
+void residual_restrict(tf_ctx& c, int nc, const int* offsets, const int* members, const int* rp, const int* ci,
+                       const T* v, const T* b, const T* x, T* coarse, const T* inv, T w, T* xc) {
+    if (nc <= 0) return;
+    hipLaunchKernelGGL(k_residual_restrict<T>, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, nc,
+                       offsets, members, rp, ci, v, b, x, coarse, inv, w, xc);
+    check(hipGetLastError(), "residual restrict launch");
+}
+template <class T>
+void prolong_jacobi(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, const int* map,
+                    const T* xc, const T* x, bool zero, T* xout, const T* b, const T* inv, T w) {
+    if (n <= 0) return;
+    const int rpb = stream_rows_per_block<T>(n, nnz);
+    const int nb = (n + rpb - 1) / rpb;
+    hipLaunchKernelGGL(k_prolong_jacobi<T>, dim3(round_xcd(nb)), dim3(kBlock), 0, c.stream, n, nb, rpb, rp, ci, v, map,
+                       xc, x, int(zero), xout, b, inv, w);
+    check(hipGetLastError(), "prolong jacobi launch");
+}
+template <class T>
+void enter_level(tf_ctx& c, int n, const double* rhs, T* b, const T* inv, T w, T* x) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_enter<T>, dim3(blocks_for(size_t(n), 8192)), dim3(kBlock), 0, c.stream, n, rhs, b, inv, w,
+                       x);
+    check(hipGetLastError(), "level entry launch");
 }
 template <class T>
 void prolong_agg(tf_ctx& c, int n, const int* map, const T* coarse, T* fine, bool set) {
@@ -295,7 +464,12 @@ void fill_zero(tf_ctx& c, void* dst, size_t bytes) {
     template void axpy<T>(tf_ctx&, int, T, const T*, T*);                                                        \
     template void scal<T>(tf_ctx&, int, T, T*);                                                                  \
     template void dot_device<T>(tf_ctx&, int, const T*, const T*, double*, int);                                 \
-    template void restrict_agg<T>(tf_ctx&, int, const int*, const int*, const T*, T*);                           \
+    template void restrict_agg<T>(tf_ctx&, int, const int*, const int*, const T*, T*, const T*, T, T*);         \
+    template void residual_restrict<T>(tf_ctx&, int, const int*, const int*, const int*, const int*, const T*,   \
+                                       const T*, const T*, T*, const T*, T, T*);                                 \
+    template void prolong_jacobi<T>(tf_ctx&, int, int64_t, const int*, const int*, const T*, const int*, const T*, \
+                                    const T*, bool, T*, const T*, const T*, T);                                  \
+    template void enter_level<T>(tf_ctx&, int, const double*, T*, const T*, T, T*);                             \
     template void prolong_agg<T>(tf_ctx&, int, const int*, const T*, T*, bool);                                        \
     template void inverse_diagonal<T>(tf_ctx&, int, const int*, const int*, const T*, T*, int*);
 TF_LA(float)

@@ -189,28 +189,44 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_factor(int n, double* m, int*
     if (nonfinite) atomicOr(bad, 1);
 }
 
-// Column j of A^-1: permute e_j, forward then backward substitution, in
-// the column of `inv` owned by thread j (row-major, coalesced across j).
-__global__ void k_lu_inverse(int n, const double* __restrict__ lu, const int* __restrict__ piv, double* inv) {
-    const int j = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (j >= n) return;
-    for (int r = 0; r < n; ++r) inv[size_t(r) * n + j] = (r == j) ? 1.0 : 0.0;
+// A^-1 from the pivoted LU by substitution on all columns at once (one
+// workgroup): V = P I (the row swaps of the factorisation in order), then
+// for each k the unit-lower and upper updates run in parallel over rows and
+// columns.  Per column this is the operation order of the triangular solve.
+__global__ __launch_bounds__(kLuThreads) void k_lu_inverse(int n, const double* __restrict__ lu,
+                                                           const int* __restrict__ piv, double* inv) {
+    const int tid = int(threadIdx.x);
+    const int nn = n * n;
+    for (int idx = tid; idx < nn; idx += kLuThreads) inv[idx] = (idx / n == idx % n) ? 1.0 : 0.0;
+    __syncthreads();
     for (int k = 0; k < n; ++k) {
         const int p = piv[k];
         if (p != k) {
-            const double t = inv[size_t(k) * n + j];
-            inv[size_t(k) * n + j] = inv[size_t(p) * n + j];
-            inv[size_t(p) * n + j] = t;
+            for (int j = tid; j < n; j += kLuThreads) {
+                const double t = inv[size_t(k) * n + j];
+                inv[size_t(k) * n + j] = inv[size_t(p) * n + j];
+                inv[size_t(p) * n + j] = t;
+            }
         }
+        __syncthreads();
     }
     for (int k = 0; k < n; ++k) {
-        const double vk = inv[size_t(k) * n + j];
-        for (int r = k + 1; r < n; ++r) inv[size_t(r) * n + j] -= lu[size_t(r) * n + k] * vk;
+        const int rows = n - k - 1;
+        for (int idx = tid; idx < rows * n; idx += kLuThreads) {
+            const int r = k + 1 + idx / n, j = idx % n;
+            inv[size_t(r) * n + j] -= lu[size_t(r) * n + k] * inv[size_t(k) * n + j];
+        }
+        __syncthreads();
     }
     for (int k = n - 1; k >= 0; --k) {
-        const double vk = inv[size_t(k) * n + j] / lu[size_t(k) * n + k];
-        inv[size_t(k) * n + j] = vk;
-        for (int r = 0; r < k; ++r) inv[size_t(r) * n + j] -= lu[size_t(r) * n + k] * vk;
+        const double d = lu[size_t(k) * n + k];
+        for (int j = tid; j < n; j += kLuThreads) inv[size_t(k) * n + j] /= d;
+        __syncthreads();
+        for (int idx = tid; idx < k * n; idx += kLuThreads) {
+            const int r = idx / n, j = idx % n;
+            inv[size_t(r) * n + j] -= lu[size_t(r) * n + k] * inv[size_t(k) * n + j];
+        }
+        __syncthreads();
     }
 }
 
@@ -283,8 +299,7 @@ bool coarse_lu_factor(tf_ctx& c, int n, const int* rp, const int* ci, const doub
     c.read_flags(1);
     const bool ok = c.hflags[0] == 0;
     if (ok && inverse) {
-        hipLaunchKernelGGL(k_lu_inverse, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, n, lu, piv,
-                           inverse);
+        hipLaunchKernelGGL(k_lu_inverse, dim3(1), dim3(kLuThreads), 0, c.stream, n, lu, piv, inverse);
         check(hipGetLastError(), "lu inverse launch");
     }
     return ok;
