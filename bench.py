@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-graph", action="store_true", help="launch each iteration's kernels without hipGraph")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the level-0 SpMV launches and a copy kernel of known size (for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver runs on a single "
@@ -76,6 +78,10 @@ def main():
         dx.upload(zeros)
         solver.setup_device(n, nnz, *A.args())
         return solver.apply_device(db, dx)
+
+    if args.spmv_only:
+        spmv_only(args, n, nnz, A, v, np, P, check, lib, Context, DeviceArray)
+        return
 
     results = {}
     for name, prec in (("chosen", 32), ("plain", 64)):
@@ -188,11 +194,40 @@ def main():
         "spmv": {"f64_us": round(t64 * 1e6, 2), "f64_GBs": round(gbs64, 1), "f32_us": round(t32 * 1e6, 2),
                  "f32_GBs": round(gbs32, 1), "bytes_f64": bytes64, "bytes_f32": bytes32},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_csr_rows<8,double,0> level-0 FP64 SpMV"},
+                     "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": traffic_bytes(nx),
+                     "kernel": "k_csr_stream<double,0> level-0 FP64 SpMV",
+                     "algorithmic_bytes": bytes64},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def traffic_bytes(nx):
+    """HBM bytes per level-0 SpMV launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (copy-kernel calibrated), for the 128^3 workload only."""
+    path = os.path.join(ROOT, "profiles", "spmv_f64_level0_heat3d128_pmc.json")
+    if nx != 128 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return int(round(json.load(f)["traffic_bytes"]))
+
+
+def spmv_only(args, n, nnz, A, v, np, P, check, lib, Context, DeviceArray):
+    """Level-0 FP64 SpMV (k_csr_stream<double,0>) x reps and a plain copy kernel
+    (k_convert<double,double>, 2*8*n bytes per launch) x reps, for counter runs."""
+    ctx = Context(0)
+    xv = DeviceArray.from_host(P.uniform(11, n))
+    y = DeviceArray(n, np.float64)
+    for _ in range(args.spmv_reps):
+        check(lib().tf_spmv_csr_f64(ctx.h, n, nnz, A.rp.ptr, A.ci.ptr, A.v.ptr, xv.ptr, y.ptr))
+    big = DeviceArray(8 * n, np.float64)
+    big2 = DeviceArray(8 * n, np.float64)
+    for _ in range(args.spmv_reps):
+        check(lib().tf_copy_f64(ctx.h, 8 * n, big.ptr, big2.ptr))
+    ctx.sync()
+    print(json.dumps({"spmv_launches": args.spmv_reps, "spmv_bytes": P.spmv_bytes(n, nnz, 8),
+                      "copy_launches": args.spmv_reps, "copy_bytes_read": 8 * 8 * n, "copy_bytes_written": 8 * 8 * n}),
+          flush=True)
 
 
 if __name__ == "__main__":

@@ -114,6 +114,8 @@ int tf_convert_f64_f32(tf_ctx* ctx, size_t n, const double* in, float* out);
 int tf_convert_f32_f64(tf_ctx* ctx, size_t n, const float* in, double* out);
 int tf_convert_i64_i32(tf_ctx* ctx, size_t n, const int64_t* in, int* out);
 int tf_convert_i32_i64(tf_ctx* ctx, size_t n, const int* in, int64_t* out);
+/* dst = src as a plain streaming kernel (Vector::copyDataFrom device path, Memory.cu:179-196). */
+int tf_copy_f64(tf_ctx* ctx, size_t n, const double* src, double* dst);
 
 /* ---- scan: cub::DeviceScan::ExclusiveSum of GpuAggregation.cu:116-121 ---- */
 int tf_scan_excl_i32(tf_ctx* ctx, int n, const int* in, int* out); /* out has n entries */

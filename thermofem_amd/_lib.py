@@ -80,6 +80,7 @@ _SIGS = {
     "tf_convert_f32_f64": (i32, [vp, sz, vp, vp]),
     "tf_convert_i64_i32": (i32, [vp, sz, vp, vp]),
     "tf_convert_i32_i64": (i32, [vp, sz, vp, vp]),
+    "tf_copy_f64": (i32, [vp, sz, vp, vp]),
     "tf_scan_excl_i32": (i32, [vp, i32, vp, vp]),
     "tf_csr_check": (i32, [vp, i32, i64, vp, vp, vp, I32P]),
     "tf_csr_validate_structure": (i32, [vp, i32, i32, i64, vp, vp, I32P]),

@@ -292,6 +292,10 @@ int tf_convert_i32_i64(tf_ctx* ctx, size_t n, const int* in, int64_t* out) {
     TF_CTX_GUARD(ctx, (convert<int, int64_t>(*ctx, n, in, out)));
 }
 
+int tf_copy_f64(tf_ctx* ctx, size_t n, const double* src, double* dst) {
+    TF_CTX_GUARD(ctx, (convert<double, double>(*ctx, n, src, dst)));
+}
+
 int tf_scan_excl_i32(tf_ctx* ctx, int n, const int* in, int* out) { TF_CTX_GUARD(ctx, scan_excl(*ctx, n, in, out)); }
 
 int tf_csr_check(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v, int* host_flags) {
