@@ -42,6 +42,12 @@ int tf_device_count(int* count);
 /* Per-device execution context: one stream, reduction scratch, pinned
  * staging.  Replaces the per-call cuBLAS/cuSPARSE handles of
  * src/LinearAlgebra/Backend.cu:128-415 and IterationContext.cu:38-124. */
+/* Device name (NUL-terminated, truncated to name_len) and HIP runtime version
+ * (the cudaGetDeviceProperties / cudaRuntimeGetVersion report of
+ * example/solve-matrix.cu:70-85); either output may be NULL. */
+int tf_device_info(int device, char* name, size_t name_len, int* runtime_version);
+int tf_get_device(int* device); /* current HIP device of the calling thread */
+int tf_set_device(int device);
 int tf_ctx_create(int device, tf_ctx** ctx);
 int tf_ctx_destroy(tf_ctx* ctx);
 int tf_ctx_synchronize(tf_ctx* ctx);
@@ -224,9 +230,22 @@ int tf_solver_setup(tf_solver* solver, int n, int64_t nnz, const int* rp, const 
                     int host_memory);
 /* apply: x holds the initial guess and receives the solution. */
 int tf_solver_apply(tf_solver* solver, const double* b, double* x, int host_memory, tf_solve_result* result);
+/* A solver whose preconditioner is an existing AMG handle (not owned; it must
+ * outlive the solver).  The solver runs on the AMG's stream; setup re-runs the
+ * AMG setup on the solver's canonical matrix, as Cg::setup / BiCgStab::setup
+ * call preconditioner->setup (Cg.cuh:60-67, BiCgStab.cuh:70-77). */
+int tf_solver_create_with_amg(int kind, const tf_krylov_options* options, tf_amg* amg, tf_solver** solver);
 tf_amg* tf_solver_amg(tf_solver* solver);
 /* Seconds of the last setup / apply measured with HIP events on the solver stream. */
 int tf_solver_timings(tf_solver* solver, double* setup_seconds, double* apply_seconds);
+
+/* ---- IO: io::readMatrixMarket (include/IO/MatrixMarket.cuh, src/IO/MatrixMarket.cu:35-122) ----
+ * Host CSR of a Matrix Market file (rows in file order, duplicates kept,
+ * symmetric storage mirrored).  Call with NULL arrays to get rows, cols and
+ * nnz, then again with host arrays of rows+1, nnz and nnz elements.
+ * Malformed input returns TF_E_INVALID with the reader's message. */
+int tf_read_matrix_market(const char* path, int* rows, int* cols, int64_t* nnz, int* host_rp, int* host_ci,
+                          double* host_v);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,10 @@ class SolveResultC(C.Structure):
 _SIGS = {
     "tf_last_error": (C.c_char_p, []),
     "tf_device_count": (i32, [I32P]),
+    "tf_device_info": (i32, [i32, C.c_char_p, sz, I32P]),
+    "tf_read_matrix_market": (i32, [C.c_char_p, I32P, I32P, I64P, I32P, I32P, F64P]),
+    "tf_get_device": (i32, [I32P]),
+    "tf_set_device": (i32, [i32]),
     "tf_ctx_create": (i32, [i32, C.POINTER(vp)]),
     "tf_ctx_destroy": (i32, [vp]),
     "tf_ctx_synchronize": (i32, [vp]),
@@ -117,6 +121,7 @@ _SIGS = {
     "tf_solver_destroy": (i32, [vp]),
     "tf_solver_setup": (i32, [vp, i32, i64, vp, vp, vp, i32]),
     "tf_solver_apply": (i32, [vp, vp, vp, i32, C.POINTER(SolveResultC)]),
+    "tf_solver_create_with_amg": (i32, [i32, C.POINTER(KrylovOptions), vp, C.POINTER(vp)]),
     "tf_solver_amg": (vp, [vp]),
     "tf_solver_timings": (i32, [vp, F64P, F64P]),
 }

@@ -3,7 +3,9 @@
 Every .hip source under thermofem_amd/csrc is compiled with hipcc for gfx950
 (in parallel), then linked into one shared library whose C ABI is declared
 in include/thermofem_hip.h.  Packed-FP32 instruction selection is disabled
-(-packed-fp32-ops) for all device code.
+(-packed-fp32-ops) for all device code.  The host programs over the C++20
+headers (the solve-matrix CLI and the C++ API tests) are then compiled with
+g++ -std=c++20 and linked against the library.
 """
 import concurrent.futures as cf
 import os
@@ -33,6 +35,38 @@ def _compile(src):
     return obj
 
 
+ROOT = os.path.dirname(HERE)
+BIN = os.path.join(HERE, "bin")
+CXX = os.environ.get("CXX", "g++")
+# Host programs over the C++20 headers, linked against the library.
+PROGRAMS = {
+    os.path.join(BIN, "solve-matrix"): os.path.join(HERE, "cli", "solve_matrix.cpp"),
+    os.path.join(ROOT, "tests", "cpp", "build", "api_test"): os.path.join(ROOT, "tests", "cpp", "api_test.cpp"),
+}
+
+
+def _program(out, src):
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    rpath = os.path.relpath(HERE, os.path.dirname(out))
+    cmd = [CXX, "-std=c++20", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(HERE, "include"), "-I" + os.path.dirname(src), src, "-o", out + ".tmp",
+           "-L" + HERE, "-lthermofem_hip", f"-Wl,-rpath,$ORIGIN/{rpath}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{CXX} failed for {src}:\n{r.stdout}\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_programs(verbose=False):
+    with cf.ThreadPoolExecutor(len(PROGRAMS)) as ex:
+        outs = list(ex.map(lambda kv: _program(*kv), PROGRAMS.items()))
+    if verbose:
+        for o in outs:
+            print("built", o)
+    return outs
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
@@ -47,6 +81,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_programs(verbose)
     return LIB
 
 

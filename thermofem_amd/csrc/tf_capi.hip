@@ -4,10 +4,12 @@
 // tf_last_error().
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <string>
 #include <vector>
 
+#include "../include/thermofem/MatrixMarket.hpp"
 #include "tf_engine.hpp"
 #include "tf_internal.hpp"
 
@@ -103,6 +105,7 @@ struct tf_solver {
     tf::AmgEngine* amg = nullptr;
     tf::SolverEngine* eng = nullptr;
     tf_amg view;
+    bool owns = true;  // owns ctx and amg (false: bound to an external tf_amg)
 };
 
 using namespace tf;
@@ -123,6 +126,46 @@ int tf_device_count(int* count) {
         require(count != nullptr, "null count");
         check(hipGetDeviceCount(count), "hipGetDeviceCount");
     });
+}
+
+int tf_device_info(int device, char* name, size_t name_len, int* runtime_version) {
+    return guard([&] {
+        hipDeviceProp_t p{};
+        check(hipGetDeviceProperties(&p, device), "hipGetDeviceProperties");
+        if (name && name_len) {
+            // Some drivers leave the marketing name empty; the ISA name is always set.
+            std::strncpy(name, p.name[0] ? p.name : p.gcnArchName, name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (runtime_version) check(hipRuntimeGetVersion(runtime_version), "hipRuntimeGetVersion");
+    });
+}
+
+int tf_read_matrix_market(const char* path, int* rows, int* cols, int64_t* nnz, int* host_rp, int* host_ci,
+                          double* host_v) {
+    return guard([&] {
+        require(path && rows && cols && nnz, "null argument");
+        std::ifstream file(path);
+        require(bool(file), "Cannot open matrix file");
+        const auto a = thermofem::io::readMatrixMarket(file);
+        *rows = a.rows_;
+        *cols = a.columns_;
+        *nnz = int64_t(a.values_.size());
+        if (host_rp) std::memcpy(host_rp, a.offsets_.data(), a.offsets_.size() * sizeof(int));
+        if (host_ci && !a.indices_.empty()) std::memcpy(host_ci, a.indices_.data(), a.indices_.size() * sizeof(int));
+        if (host_v && !a.values_.empty()) std::memcpy(host_v, a.values_.data(), a.values_.size() * sizeof(double));
+    });
+}
+
+int tf_get_device(int* device) {
+    return guard([&] {
+        require(device != nullptr, "null device pointer");
+        check(hipGetDevice(device), "hipGetDevice");
+    });
+}
+
+int tf_set_device(int device) {
+    return guard([&] { check(hipSetDevice(device), "hipSetDevice"); });
 }
 
 int tf_ctx_create(int device, tf_ctx** ctx) {
@@ -208,9 +251,15 @@ int tf_device_view(int strategy, void* ptr, void** device_ptr) {
     });
 }
 
+// A plain hipMemcpy may return before a device-to-device copy has finished,
+// and the library's streams are non-blocking, so a kernel enqueued next on a
+// context stream could read the old contents.  The copy is therefore done on
+// the null stream and waited for: the data are in place when this returns.
 int tf_memcpy(void* dst, const void* src, size_t bytes) {
     return guard([&] {
-        if (bytes) check(hipMemcpy(dst, src, bytes, hipMemcpyDefault), "hipMemcpy");
+        if (!bytes) return;
+        check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, nullptr), "hipMemcpyAsync");
+        check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
     });
 }
 
@@ -439,6 +488,7 @@ int tf_amg_setup(tf_amg* amg, int n, int64_t nnz, const int* rp, const int* ci, 
         }
         else {
             amg->eng->setup(n, nnz, rp, ci, v);
+            amg->ctx->sync();
         }
     });
 }
@@ -554,15 +604,37 @@ int tf_solver_create(int device, int kind, const tf_krylov_options* options, con
     });
 }
 
+int tf_solver_create_with_amg(int kind, const tf_krylov_options* options, tf_amg* amg, tf_solver** solver) {
+    return guard([&] {
+        require(solver != nullptr && amg != nullptr, "null handle");
+        KrylovOptionsC o;
+        if (options) {
+            o.maxit = options->max_iterations;
+            o.rtol = options->relative_tolerance;
+            o.atol = options->absolute_tolerance;
+            o.refresh = options->residual_refresh_interval;
+            o.graph = options->use_graph != 0;
+        }
+        auto h = std::make_unique<tf_solver>();
+        h->ctx = amg->ctx;
+        h->amg = amg->eng;
+        h->owns = false;
+        DeviceScope scope(h->ctx->device);
+        h->eng = new SolverEngine(h->ctx, kind, o, h->amg);
+        h->view = *amg;
+        *solver = h.release();
+    });
+}
+
 int tf_solver_destroy(tf_solver* s) {
     return guard([&] {
         if (!s) return;
         {
             DeviceScope scope(s->ctx->device);
             delete s->eng;
-            delete s->amg;
+            if (s->owns) delete s->amg;
         }
-        destroy_ctx(s->ctx);
+        if (s->owns) destroy_ctx(s->ctx);
         delete s;
     });
 }
