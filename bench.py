@@ -12,9 +12,10 @@ matrix and right-hand side already resident in HBM.
   plain:  the same method with the cycle in FP64
 
 value = seconds per chosen setup+solve (lower is better); the plain time and
-the ratio are reported beside it.  roofline = the level-0 FP64 CSR SpMV
-(216,924,164 algorithmic bytes per launch at 128^3) timed with HIP events on
-the library stream.  cpu_baseline = the CPU oracle (FP64, OpenMP) on the
+the ratio are reported beside it.  roofline = the level-0 FP64 SpMV kernel
+the solve runs (sliced ELL with 16-bit column offsets; 216,924,164
+algorithmic CSR bytes per launch at 128^3) timed with HIP events on the
+library stream; the CSR kernel is timed beside it.  cpu_baseline = the CPU oracle (FP64, OpenMP) on the
 same problem (FP64 setup + PCG solve), timed on the host cores.
 """
 import argparse
@@ -114,35 +115,46 @@ def main():
     ch, pl = results["chosen"], results["plain"]
     err = float(np.linalg.norm(ch["x"] - pl["x"]) / np.linalg.norm(pl["x"]))
 
-    # Level-0 FP64 SpMV on the library stream, HIP events around the launches.
+    # Level-0 SpMV on the library stream, HIP events around the launches.
     ctx = Context(0)
-    y = DeviceArray(n, np.float64)
+    y64 = DeviceArray(n, np.float64)
+    y32 = DeviceArray(n, np.float32)
     xv = DeviceArray.from_host(P.uniform(11, n))
+    x32 = DeviceArray.from_host(P.uniform(11, n).astype(np.float32))
+    v32 = DeviceArray.from_host(v.astype(np.float32))
+    sell, bits, stored = make_sell(ctx, n, nnz, A, check, lib)
     ev0, ev1 = C.c_void_p(), C.c_void_p()
     check(lib().tf_event_create(C.byref(ev0)))
     check(lib().tf_event_create(C.byref(ev1)))
 
-    def time_spmv(fn, vals, xx, reps):
+    def time_launches(launch, reps):
         for _ in range(3):
-            check(fn(ctx.h, n, nnz, A.rp.ptr, A.ci.ptr, vals.ptr, xx.ptr, y.ptr))
+            check(launch())
         check(lib().tf_event_record(ctx.h, ev0))
         for _ in range(reps):
-            check(fn(ctx.h, n, nnz, A.rp.ptr, A.ci.ptr, vals.ptr, xx.ptr, y.ptr))
+            check(launch())
         check(lib().tf_event_record(ctx.h, ev1))
         ms = C.c_float()
         check(lib().tf_event_elapsed_ms(ev0, ev1, C.byref(ms)))
         return ms.value / reps * 1e-3
 
-    t64 = time_spmv(lib().tf_spmv_csr_f64, A.v, xv, args.spmv_reps)
-    v32 = DeviceArray.from_host(v.astype(np.float32))
-    x32 = DeviceArray.from_host(P.uniform(11, n).astype(np.float32))
-    y = DeviceArray(n, np.float32)
-    t32 = time_spmv(lib().tf_spmv_csr_f32, v32, x32, args.spmv_reps)
+    L = lib()
+    reps = args.spmv_reps
+    t64 = time_launches(lambda: L.tf_sell_spmv_f64(ctx.h, sell, xv.ptr, y64.ptr), reps)
+    t32 = time_launches(lambda: L.tf_sell_spmv_f32(ctx.h, sell, x32.ptr, y32.ptr), reps)
+    c64 = time_launches(lambda: L.tf_spmv_csr_f64(ctx.h, n, nnz, A.rp.ptr, A.ci.ptr, A.v.ptr, xv.ptr, y64.ptr), reps)
+    c32 = time_launches(lambda: L.tf_spmv_csr_f32(ctx.h, n, nnz, A.rp.ptr, A.ci.ptr, v32.ptr, x32.ptr, y32.ptr),
+                        reps)
+    check(L.tf_sell_destroy(sell))
     bytes64 = P.spmv_bytes(n, nnz, 8)
     bytes32 = P.spmv_bytes(n, nnz, 4)
     gbs64 = bytes64 / t64 / 1e9
     gbs32 = bytes32 / t32 / 1e9
-    log(f"spmv f64 {t64 * 1e6:.1f} us {gbs64:.0f} GB/s; f32 {t32 * 1e6:.1f} us {gbs32:.0f} GB/s")
+    # bytes the sliced-ELL kernel itself must move: values + 16-bit offsets
+    # (padding included), row lengths, slice offsets, x once and y
+    moved64 = stored * (8 + bits // 8) + n + 4 * ((n + 63) // 64 + 1) + 16 * n
+    log(f"spmv sliced-ELL f64 {t64 * 1e6:.1f} us {gbs64:.0f} GB/s; f32 {t32 * 1e6:.1f} us {gbs32:.0f} GB/s; "
+        f"CSR f64 {c64 * 1e6:.1f} us, f32 {c32 * 1e6:.1f} us")
 
     cpu = None
     oracle_err = None
@@ -192,20 +204,32 @@ def main():
         "rel_error_chosen_vs_plain_fp64": err,
         "rel_error_vs_fp64_oracle": oracle_err,
         "spmv": {"f64_us": round(t64 * 1e6, 2), "f64_GBs": round(gbs64, 1), "f32_us": round(t32 * 1e6, 2),
-                 "f32_GBs": round(gbs32, 1), "bytes_f64": bytes64, "bytes_f32": bytes32},
+                 "f32_GBs": round(gbs32, 1), "bytes_f64": bytes64, "bytes_f32": bytes32,
+                 "format": f"sliced ELL, slices of 64 rows, {bits}-bit columns, {stored} stored entries",
+                 "f64_bytes_moved": moved64, "f64_moved_GBs": round(moved64 / t64 / 1e9, 1),
+                 "csr_f64_us": round(c64 * 1e6, 2), "csr_f32_us": round(c32 * 1e6, 2)},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": traffic_bytes(nx),
-                     "kernel": "k_csr_stream<double,0> level-0 FP64 SpMV",
+                     "kernel": "k_sell<double,short,0> level-0 FP64 SpMV",
                      "algorithmic_bytes": bytes64},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
+def make_sell(ctx, n, nnz, A, check, lib):
+    """Sliced-ELL handle of the level-0 matrix (the format the solve uses)."""
+    h = C.c_void_p()
+    check(lib().tf_sell_create(ctx.h, n, nnz, A.rp.ptr, A.ci.ptr, A.v.ptr, 0.3, C.byref(h)))
+    bits, stored = C.c_int(), C.c_int64()
+    check(lib().tf_sell_info(h, C.byref(bits), C.byref(stored), None))
+    return h, bits.value, stored.value
+
+
 def traffic_bytes(nx):
     """HBM bytes per level-0 SpMV launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE runs (copy-kernel calibrated), for the 128^3 workload only."""
-    path = os.path.join(ROOT, "profiles", "spmv_f64_level0_heat3d128_pmc.json")
+    path = os.path.join(ROOT, "profiles", "spmv_sell_f64_level0_heat3d128_pmc.json")
     if nx != 128 or not os.path.exists(path):
         return None
     with open(path) as f:
@@ -213,13 +237,14 @@ def traffic_bytes(nx):
 
 
 def spmv_only(args, n, nnz, A, v, np, P, check, lib, Context, DeviceArray):
-    """Level-0 FP64 SpMV (k_csr_stream<double,0>) x reps and a plain copy kernel
+    """Level-0 FP64 SpMV (k_sell<double,short,0>) x reps and a plain copy kernel
     (k_convert<double,double>, 2*8*n bytes per launch) x reps, for counter runs."""
     ctx = Context(0)
     xv = DeviceArray.from_host(P.uniform(11, n))
     y = DeviceArray(n, np.float64)
+    sell, _, _ = make_sell(ctx, n, nnz, A, check, lib)
     for _ in range(args.spmv_reps):
-        check(lib().tf_spmv_csr_f64(ctx.h, n, nnz, A.rp.ptr, A.ci.ptr, A.v.ptr, xv.ptr, y.ptr))
+        check(lib().tf_sell_spmv_f64(ctx.h, sell, xv.ptr, y.ptr))
     big = DeviceArray(8 * n, np.float64)
     big2 = DeviceArray(8 * n, np.float64)
     for _ in range(args.spmv_reps):

@@ -189,6 +189,7 @@ typedef struct tf_amg_options {
     int absorb_singletons;
     int cycle;              /* 0 V, 1 W, 2 F (Cycle.cuh:10) */
     int level_precision;    /* 64: cycle in FP64 (reference); 32: FP32 cycle under an FP64 caller */
+    int coarse_tail;        /* coarse levels as one one-workgroup program: 0 timed at setup, 1 always, 2 never */
 } tf_amg_options;
 
 typedef struct tf_krylov_options {
@@ -238,6 +239,29 @@ int tf_solver_create_with_amg(int kind, const tf_krylov_options* options, tf_amg
 tf_amg* tf_solver_amg(tf_solver* solver);
 /* Seconds of the last setup / apply measured with HIP events on the solver stream. */
 int tf_solver_timings(tf_solver* solver, double* setup_seconds, double* apply_seconds);
+
+/* ---- sliced-ELL row products: IterationContext::multiply / residual and the
+ * Jacobi update (IterationContext.cu:26-32,75,155) over a storage made for
+ * 64-lane waves: slices of 64 rows, entries slice-major, 16-bit column
+ * offsets when every |column - row| <= 32767, one-byte row lengths.  Results
+ * equal the CSR entry points' bit for bit.  The handle keeps FP64 and FP32
+ * values and belongs to the context it was created on (which must outlive it). */
+typedef struct tf_sell tf_sell;
+/* TF_E_INVALID (no handle) if a row has more than 255 entries or the padding
+ * would exceed max_padding * nnz. */
+int tf_sell_create(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double max_padding,
+                   tf_sell** sell);
+int tf_sell_destroy(tf_sell* sell);
+/* index_bits 16 or 32; stored_entries includes padding; reordered: rows sorted by length in 256-row windows */
+int tf_sell_info(const tf_sell* sell, int* index_bits, int64_t* stored_entries, int* reordered);
+int tf_sell_spmv_f64(tf_ctx* ctx, const tf_sell* sell, const double* x, double* y);
+int tf_sell_spmv_f32(tf_ctx* ctx, const tf_sell* sell, const float* x, float* y);
+int tf_sell_residual_f64(tf_ctx* ctx, const tf_sell* sell, const double* b, const double* x, double* r);
+int tf_sell_residual_f32(tf_ctx* ctx, const tf_sell* sell, const float* b, const float* x, float* r);
+int tf_sell_jacobi_f64(tf_ctx* ctx, const tf_sell* sell, const double* inv, double w, const double* b,
+                       const double* x_in, double* x_out);
+int tf_sell_jacobi_f32(tf_ctx* ctx, const tf_sell* sell, const float* inv, float w, const float* b,
+                       const float* x_in, float* x_out);
 
 /* ---- IO: io::readMatrixMarket (include/IO/MatrixMarket.cuh, src/IO/MatrixMarket.cu:35-122) ----
  * Host CSR of a Matrix Market file (rows in file order, duplicates kept,

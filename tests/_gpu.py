@@ -118,3 +118,51 @@ def lu_factor(a):
     sing = C.c_int()
     check(lib().tf_coarse_lu_factor_f64(ctx().h, n, rp.ptr, ci.ptr, v.ptr, lu.ptr, piv.ptr, inv.ptr, C.byref(sing)))
     return sing.value == 0, lu, piv, inv
+
+
+class Sell:
+    """A sliced-ELL handle (tf_sell_*) built from a host CSR matrix."""
+
+    def __init__(self, a, max_padding=1.0):
+        self.n = len(a[0]) - 1
+        self.rp, self.ci, self.v = up(a[0], np.int32), up(a[1], np.int32), up(a[2], np.float64)
+        self.h = C.c_void_p()
+        check(lib().tf_sell_create(ctx().h, self.n, len(a[1]), self.rp.ptr, self.ci.ptr, self.v.ptr, max_padding,
+                                   C.byref(self.h)))
+
+    def info(self):
+        bits, stored, reordered = C.c_int(), C.c_int64(), C.c_int()
+        check(lib().tf_sell_info(self.h, C.byref(bits), C.byref(stored), C.byref(reordered)))
+        return bits.value, stored.value, bool(reordered.value)
+
+    def spmv(self, x, dtype=np.float64):
+        dx, y = up(x, dtype), DeviceArray(self.n, dtype)
+        fn = lib().tf_sell_spmv_f64 if dtype == np.float64 else lib().tf_sell_spmv_f32
+        check(fn(ctx().h, self.h, dx.ptr, y.ptr))
+        ctx().sync()
+        return y.get()
+
+    def residual(self, b, x, dtype=np.float64):
+        db, dx, r = up(b, dtype), up(x, dtype), DeviceArray(self.n, dtype)
+        fn = lib().tf_sell_residual_f64 if dtype == np.float64 else lib().tf_sell_residual_f32
+        check(fn(ctx().h, self.h, db.ptr, dx.ptr, r.ptr))
+        ctx().sync()
+        return r.get()
+
+    def jacobi(self, inv, w, b, x, dtype=np.float64):
+        di, db, dx, out = up(inv, dtype), up(b, dtype), up(x, dtype), DeviceArray(self.n, dtype)
+        fn = lib().tf_sell_jacobi_f64 if dtype == np.float64 else lib().tf_sell_jacobi_f32
+        check(fn(ctx().h, self.h, di.ptr, w, db.ptr, dx.ptr, out.ptr))
+        ctx().sync()
+        return out.get()
+
+    def close(self):
+        if self.h:
+            check(lib().tf_sell_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -212,10 +212,13 @@ def test_cycles_match_oracle(cycle, precision, tol):
         c = c.astype(np.float32).astype(np.float64)
     opts = dict(coarse_rows=4, cycle=int(cycle), level_precision=precision)
     host = O.Amg(**opts).setup(a)
-    dev = Amg(AmgOptions(coarse_rows=4, cycle=cycle, level_precision=precision)).setup(a)
+    dev = Amg(AmgOptions(coarse_rows=4, cycle=cycle, level_precision=precision, coarse_tail=1)).setup(a)
     assert dev.level_count() >= 3
     x = dev.apply(b)
     assert np.allclose(x, host.apply(b), atol=tol, rtol=0)
+    if precision == 32:
+        # FP32 levels: the same operations in the same order as the oracle's cycle
+        assert np.array_equal(x, host.apply(b))
     visits = dev.last_cycle_visits()
     L = len(visits)
     if cycle == Cycle.V:
@@ -329,3 +332,26 @@ def test_symmetric_pattern_strength_and_aggregation_bit_exact(theta):
     t = O.aggregate(g, 8, True)
     assert same(G.aggregate(g, 8, True), t)
     assert same(G.galerkin(a, t), O.galerkin(a, t[0]))
+
+
+@pytest.mark.parametrize("tail", [1, 2])
+@pytest.mark.parametrize("cycle", [Cycle.V, Cycle.W, Cycle.F])
+def test_cycles_c1_fp32_bit_exact(cycle, tail):
+    """2D Poisson 256^2 with coarse_rows 64: the finer levels run level by level;
+    with tail=1 the coarse levels run as one one-workgroup program, with tail=2
+    level by level.  With FP32 levels every operation rounds as the oracle's
+    cycle does, so both results are bit-identical to it, and so is a second
+    apply (no state carried over)."""
+    a, b = P.poisson2d_config(256, seed=0)
+    opts = dict(coarse_rows=64, cycle=int(cycle), level_precision=32, pre_sweeps=2, post_sweeps=2,
+                jacobi_weight=2.0 / 3.0)
+    host = O.Amg(**opts).setup(a)
+    dev = Amg(AmgOptions(coarse_rows=64, cycle=cycle, level_precision=32, pre_sweeps=2, post_sweeps=2,
+                         jacobi_weight=2.0 / 3.0, coarse_tail=tail)).setup(a)
+    log(f"c1 cycle {cycle} levels {dev.level_count()}")
+    assert dev.level_count() >= 5
+    b32 = b.astype(np.float32).astype(np.float64)
+    expected = host.apply(b32)
+    assert np.array_equal(dev.apply(b32), expected)
+    assert np.array_equal(dev.apply(b32), expected)
+    assert list(dev.last_cycle_visits()) == list(host.visits())

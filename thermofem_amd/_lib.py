@@ -26,7 +26,7 @@ class AmgOptions(C.Structure):
     _fields_ = [("max_levels", i32), ("coarse_rows", i32), ("matching_rounds", i32), ("pre_sweeps", i32),
                 ("post_sweeps", i32), ("coarse_sweeps", i32), ("jacobi_weight", f64),
                 ("strength_threshold", f64), ("absorb_singletons", i32), ("cycle", i32),
-                ("level_precision", i32)]
+                ("level_precision", i32), ("coarse_tail", i32)]
 
 
 class KrylovOptions(C.Structure):
@@ -42,6 +42,15 @@ _SIGS = {
     "tf_last_error": (C.c_char_p, []),
     "tf_device_count": (i32, [I32P]),
     "tf_device_info": (i32, [i32, C.c_char_p, sz, I32P]),
+    "tf_sell_create": (i32, [vp, i32, i64, vp, vp, vp, f64, C.POINTER(vp)]),
+    "tf_sell_destroy": (i32, [vp]),
+    "tf_sell_info": (i32, [vp, I32P, I64P, I32P]),
+    "tf_sell_spmv_f64": (i32, [vp, vp, vp, vp]),
+    "tf_sell_spmv_f32": (i32, [vp, vp, vp, vp]),
+    "tf_sell_residual_f64": (i32, [vp, vp, vp, vp, vp]),
+    "tf_sell_residual_f32": (i32, [vp, vp, vp, vp, vp]),
+    "tf_sell_jacobi_f64": (i32, [vp, vp, vp, f64, vp, vp, vp]),
+    "tf_sell_jacobi_f32": (i32, [vp, vp, vp, f32, vp, vp, vp]),
     "tf_read_matrix_market": (i32, [C.c_char_p, I32P, I32P, I64P, I32P, I32P, F64P]),
     "tf_get_device": (i32, [I32P]),
     "tf_set_device": (i32, [i32]),

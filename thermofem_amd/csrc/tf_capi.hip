@@ -100,6 +100,13 @@ struct tf_amg {
     tf::AmgEngine* eng = nullptr;
 };
 
+struct tf_sell {
+    tf_ctx* ctx = nullptr;
+    tf::DevSell s;
+    double* v64 = nullptr;
+    float* v32 = nullptr;
+};
+
 struct tf_solver {
     tf_ctx* ctx = nullptr;
     tf::AmgEngine* amg = nullptr;
@@ -140,6 +147,64 @@ int tf_device_info(int device, char* name, size_t name_len, int* runtime_version
         if (runtime_version) check(hipRuntimeGetVersion(runtime_version), "hipRuntimeGetVersion");
     });
 }
+
+int tf_sell_create(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double max_padding,
+                   tf_sell** sell) {
+    return guard([&] {
+        require(ctx != nullptr, "null context");
+        DeviceScope scope(ctx->device);
+        require(sell != nullptr && n >= 1 && nnz >= 0 && max_padding >= 0, "invalid sliced-ELL arguments");
+        *sell = nullptr;
+        auto h = std::make_unique<tf_sell>();
+        h->ctx = ctx;
+        require(sell_build(*ctx, n, nnz, rp, ci, h->s, max_padding),
+                "matrix not representable in sliced ELL (row over 255 entries or padding limit)");
+        h->v64 = dev_alloc<double>(*ctx, size_t(h->s.stored));
+        h->v32 = dev_alloc<float>(*ctx, size_t(h->s.stored));
+        sell_values<double, double>(*ctx, h->s, rp, v, h->v64);
+        sell_values<double, float>(*ctx, h->s, rp, v, h->v32);
+        ctx->sync();
+        *sell = h.release();
+    });
+}
+
+int tf_sell_destroy(tf_sell* h) {
+    return guard([&] {
+        if (!h) return;
+        DeviceScope scope(h->ctx->device);
+        h->ctx->sync();
+        free_sell(*h->ctx, h->s);
+        dev_free(*h->ctx, h->v64);
+        dev_free(*h->ctx, h->v32);
+        delete h;
+    });
+}
+
+int tf_sell_info(const tf_sell* h, int* index_bits, int64_t* stored_entries, int* reordered) {
+    return guard([&] {
+        require(h != nullptr, "null handle");
+        if (index_bits) *index_bits = h->s.narrow ? 16 : 32;
+        if (stored_entries) *stored_entries = h->s.stored;
+        if (reordered) *reordered = h->s.perm != nullptr;
+    });
+}
+
+#define TF_SELL_ABI(T, SUF, VAL)                                                                                  \
+    int tf_sell_spmv_##SUF(tf_ctx* ctx, const tf_sell* h, const T* x, T* y) {                                       \
+        TF_CTX_GUARD(ctx, require(h != nullptr && x != y, "SpMV shape or alias mismatch");                           \
+                     sell_spmv<T>(*ctx, h->s, h->VAL, x, y));                                                        \
+    }                                                                                                                \
+    int tf_sell_residual_##SUF(tf_ctx* ctx, const tf_sell* h, const T* b, const T* x, T* r) {                       \
+        TF_CTX_GUARD(ctx, require(h != nullptr && r != b && r != x, "Residual shape or alias mismatch");             \
+                     sell_residual<T>(*ctx, h->s, h->VAL, b, x, r));                                                 \
+    }                                                                                                                \
+    int tf_sell_jacobi_##SUF(tf_ctx* ctx, const tf_sell* h, const T* inv, T w, const T* b, const T* x_in, T* x_out) { \
+        TF_CTX_GUARD(ctx, require(h != nullptr && x_in != x_out, "fused Jacobi needs distinct input and output iterates"); \
+                     sell_jacobi<T>(*ctx, h->s, h->VAL, inv, w, b, x_in, x_out));                                    \
+    }
+TF_SELL_ABI(double, f64, v64)
+TF_SELL_ABI(float, f32, v32)
+#undef TF_SELL_ABI
 
 int tf_read_matrix_market(const char* path, int* rows, int* cols, int64_t* nnz, int* host_rp, int* host_ci,
                           double* host_v) {

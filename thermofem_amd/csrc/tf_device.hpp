@@ -139,4 +139,123 @@ inline int stream_rows_per_block(int n, int64_t nnz) {
     return r;
 }
 
+enum Epilogue { kSpmv = 0, kResidual = 1, kJacobi = 2 };
+
+// Fused epilogues of the row-product kernels, evaluated without
+// contraction: y = sum, r = b - sum, x + (w*inv)*(b - sum).
+template <class T, int E>
+__device__ __forceinline__ void epilogue(int row, T acc, const T* __restrict__ x, T* __restrict__ y,
+                                         const T* __restrict__ b, const T* __restrict__ inv, T w) {
+#pragma clang fp contract(off)
+    if constexpr (E == kSpmv) {
+        y[row] = acc;
+    }
+    else if constexpr (E == kResidual) {
+        y[row] = b[row] - acc;
+    }
+    else {
+        const T wd = w * inv[row];
+        const T d = b[row] - acc;
+        const T upd = wd * d;
+        y[row] = x[row] + upd;
+    }
+}
+
+// Read-only view of a sliced-ELL matrix (slices of 64 row positions, entries
+// stored slice-major; C = short: column offsets from the row, int: columns).
+// With perm, rows are ordered by decreasing length inside windows of 256
+// rows so that the rows sharing a slice have similar lengths.
+template <class C>
+struct SellView {
+    int n;
+    int slices;
+    const int* base;           // entry offset of each slice (slices + 1)
+    const unsigned char* len;  // row length per position
+    const C* col;
+    const int* perm;           // row at each position (nullptr: identity)
+};
+
+__device__ __forceinline__ int sell_col(short c, int row) { return row + int(c); }
+__device__ __forceinline__ int sell_col(int c, int) { return c; }
+
+// Sum of one row of a sliced-ELL matrix for the calling lane: vp and cp
+// point at the row's first entry (stride 64 between its entries), len >= 0.
+// Chunks of 8 entries issue all their loads and gathers before the adds;
+// slots past the row's end re-read its last entry (no branch, no padding
+// read) and are not added.  Products are rounded and added one by one in
+// stored order, as csr_stream_rows does.
+template <class T, class C, class XF>
+__device__ __forceinline__ T sell_row_sum(int row, int len, const T* __restrict__ vp, const C* __restrict__ cp,
+                                          XF&& xval) {
+#pragma clang fp contract(off)
+    T acc = 0;
+    for (int k0 = 0; k0 < len; k0 += 8) {
+        T pv[8], xv[8];
+        int pc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = min(k0 + u, len - 1);
+            pv[u] = ld_stream(vp + k * kWave);
+            pc[u] = sell_col(ld_stream(cp + k * kWave), row);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = xval(pc[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const T p = pv[u] * xv[u];
+            const T s = acc + p;
+            acc = k0 + u < len ? s : acc;
+        }
+    }
+    return acc;
+}
+
+// x = inv * b for the dense coarsest level (order n <= kCoarseMax), one
+// 1024-thread block: b is staged in LDS (bs) as FP64 first, so rhs and x may
+// alias; wave w forms rows w, w+16, ... from lane-strided partial sums (lane
+// l adds entries l, l+64, ... in that order) and a wave reduction.  Up to
+// order 256 the loads of four rows are issued together.  All threads of the
+// block must call it.
+constexpr int kCoarseMax = 1024;
+template <class T>
+__device__ __forceinline__ void coarse_inverse_apply(int n, const double* __restrict__ inv, const T* b, T* x,
+                                                     double* bs) {
+    for (int i = int(threadIdx.x); i < n; i += 1024) bs[i] = double(b[i]);
+    __syncthreads();
+    const int lane = int(threadIdx.x) & (kWave - 1), w = int(threadIdx.x) / kWave;
+    constexpr int RB = 4, UB = 4;
+    if (n <= UB * kWave) {
+        for (int r0 = w; r0 < n; r0 += 16 * RB) {
+            double a[RB][UB];
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const int r = min(r0 + 16 * q, n - 1);
+#pragma unroll
+                for (int u = 0; u < UB; ++u) a[q][u] = inv[size_t(r) * n + min(lane + kWave * u, n - 1)];
+            }
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                double acc = 0;
+#pragma unroll
+                for (int u = 0; u < UB; ++u) {
+                    const int j = lane + kWave * u;
+                    if (j < n) acc += a[q][u] * bs[j];
+                }
+                const int r = r0 + 16 * q;
+                if (r < n) {
+                    acc = wave_sum(acc);
+                    if (lane == 0) x[r] = T(acc);
+                }
+            }
+        }
+        return;
+    }
+    for (int r = w; r < n; r += 1024 / kWave) {
+        double acc = 0;
+        for (int j = lane; j < n; j += kWave) acc += inv[size_t(r) * n + j] * bs[j];
+        acc = wave_sum(acc);
+        if (lane == 0) x[r] = T(acc);
+    }
+}
+
 }  // namespace tf

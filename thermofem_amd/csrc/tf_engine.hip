@@ -10,6 +10,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "tf_device.hpp"
 #include "tf_engine.hpp"
@@ -42,6 +45,83 @@ constexpr int kFusedResidualRows = 0;
 // (saves a launch); on larger levels the extra map gather per nonzero costs
 // more than the separate prolongation pass.
 constexpr int kFusedProlongRows = 65536;
+// Operators are also stored in sliced ELL when that costs at most this much
+// padding (relative to nnz) and the level is not the LU-solved coarsest one:
+// one lane per row, 16-bit column offsets, no LDS staging (tf_sell.hip).
+constexpr double kSellMaxPadding = 0.3;
+constexpr int kSellMinRows = 256;
+// Levels with at least this many nonzeros are tuned: sliced ELL and the CSR
+// kernel are both timed on the level's own operator and the faster one is
+// kept (sliced ELL wins on stencil-like levels, not always on coarsened ones).
+constexpr int64_t kTuneMinNnz = 200000;
+
+// Process-wide cache of tuning decisions, keyed by the operator's shape and
+// the element type, so repeated setups of one problem time each level once.
+struct TuneKey {
+    int op;  // 0: Jacobi sweep of an AMG level, 1: the Krylov operator's product
+    int n;
+    int64_t nnz, stored;
+    int bytes;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(op, n, nnz, stored, bytes) < std::tie(o.op, o.n, o.nnz, o.stored, o.bytes);
+    }
+};
+std::map<TuneKey, bool> g_tuned;
+std::mutex g_tune_mutex;
+
+// Times the two candidates alternately (one untimed launch each first) and
+// compares their fastest launches, which is robust to the odd slow launch;
+// true if the first is faster.
+template <class A, class B>
+bool first_is_faster(tf_ctx& c, A&& first, B&& second, int samples = 5) {
+    hipEvent_t e[4];
+    for (auto& ev : e) check(hipEventCreate(&ev), "hipEventCreate");
+    first();
+    second();
+    float best[2] = {1e30f, 1e30f};
+    for (int i = 0; i < samples; ++i) {
+        check(hipEventRecord(e[0], c.stream), "event record");
+        first();
+        check(hipEventRecord(e[1], c.stream), "event record");
+        second();
+        check(hipEventRecord(e[2], c.stream), "event record");
+        check(hipEventSynchronize(e[2]), "event sync");
+        float t0 = 0, t1 = 0;
+        check(hipEventElapsedTime(&t0, e[0], e[1]), "event time");
+        check(hipEventElapsedTime(&t1, e[1], e[2]), "event time");
+        best[0] = std::min(best[0], t0);
+        best[1] = std::min(best[1], t1);
+    }
+    for (auto& ev : e) (void)hipEventDestroy(ev);
+    return best[0] <= best[1];
+}
+
+// Keeps the sliced-ELL copy of a level only where it beats the CSR sweep.
+template <class L>
+bool sell_pays(tf_ctx& c, const AmgLevel& l, double w) {
+    const DevCsr& A = l.native;
+    if (A.nnz < kTuneMinNnz) return true;
+    const TuneKey key{0, A.n, A.nnz, l.sell.stored, int(sizeof(L))};
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) return it->second;
+    }
+    fill_zero(c, l.b, size_t(A.n) * sizeof(L));
+    fill_zero(c, l.x, size_t(A.n) * sizeof(L));
+    const L* vals = static_cast<const L*>(l.vals);
+    const L* svals = static_cast<const L*>(l.svals);
+    const L* inv = static_cast<const L*>(l.inv);
+    const L* b = static_cast<const L*>(l.b);
+    const L* x = static_cast<const L*>(l.x);
+    L* x2 = static_cast<L*>(l.x2);
+    const bool sell = first_is_faster(
+        c, [&] { sell_jacobi<L>(c, l.sell, svals, inv, L(w), b, x, x2); },
+        [&] { jacobi_fused<L>(c, A.n, A.nnz, A.rp, A.ci, vals, inv, L(w), b, x, x2); });
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tuned[key] = sell;
+    return sell;
+}
 
 
 // Loop over LDS-staged row blocks for the fused reduction kernels (XCD
@@ -69,6 +149,50 @@ __device__ __forceinline__ void for_stream_blocks(int n, int nblocks, int rpb, c
         }
     }
 }
+
+// The same over a sliced-ELL operator: one lane per row, slices of 64 rows,
+// 4 slices per block, blocks XCD-contiguous.
+template <class C, class F>
+__device__ __forceinline__ void for_sell_rows(const SellView<C>& m, const double* __restrict__ v,
+                                              const double* __restrict__ x, F&& f) {
+    constexpr int spb = kBlock / kWave;
+    const int nblocks = (m.slices + spb - 1) / spb;
+    const int per = int(gridDim.x) / kXcds;
+    const int xcd = int(blockIdx.x % kXcds), idx = int(blockIdx.x / kXcds);
+    const int chunk = (nblocks + kXcds - 1) / kXcds;
+    const int end = min(nblocks, (xcd + 1) * chunk);
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    for (int lb = xcd * chunk + idx; lb < end; lb += per) {
+        const int slice = lb * spb + int(threadIdx.x) / kWave;
+        const int pos = slice * kWave + lane;
+        if (pos < m.n) {
+            const int off = m.base[slice] + lane;
+            const int row = m.perm ? m.perm[pos] : pos;
+            f(row, sell_row_sum<double>(row, int(m.len[pos]), v + off, m.col + off, [&](int c) { return x[c]; }));
+        }
+    }
+}
+
+// Row sources of the fused kernels below.
+struct CsrRows {
+    int n, nblocks, rpb;
+    const int* rp;
+    const int* ci;
+    const double* v;
+    template <class F>
+    __device__ __forceinline__ void each(const double* x, F&& f) const {
+        for_stream_blocks(n, nblocks, rpb, rp, ci, v, x, f);
+    }
+};
+template <class C>
+struct SellRows {
+    SellView<C> m;
+    const double* v;
+    template <class F>
+    __device__ __forceinline__ void each(const double* x, F&& f) const {
+        for_sell_rows(m, v, x, f);
+    }
+};
 
 // Finalisers run by thread 0 of the last block of a reduction.
 enum Fin { kFinRhoCg, kFinCurvCg, kFinUpdCg, kFinTrue, kFinRhoBi, kFinDenBi, kFinSBi, kFinUpdBi };
@@ -184,14 +308,13 @@ __global__ __launch_bounds__(kBlock) void k_kdot(int n, const double* __restrict
 }
 
 // y = A x fused with sum(w .* y) -> finalizer.
-__global__ __launch_bounds__(kBlock) void k_kspmv_dot(int n, int nblocks, int rpb, const int* __restrict__ rp,
-                                                      const int* __restrict__ ci, const double* __restrict__ v,
-                                                      const double* __restrict__ x, double* __restrict__ y,
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_kspmv_dot(R rows, const double* __restrict__ x, double* __restrict__ y,
                                                       const double* __restrict__ w, double* partials,
                                                       unsigned* ticket, KCtl* ctl, int mode) {
     if (stopped(ctl)) return;
     double contrib = 0;
-    for_stream_blocks(n, nblocks, rpb, rp, ci, v, x, [&](int row, double acc) {
+    rows.each(x, [&](int row, double acc) {
         y[row] = acc;
         contrib += w[row] * acc;
     });
@@ -199,14 +322,13 @@ __global__ __launch_bounds__(kBlock) void k_kspmv_dot(int n, int nblocks, int rp
 }
 
 // t = A x fused with (t.t, t.s): BiCgStab.cuh:214-226 (omega).
-__global__ __launch_bounds__(kBlock) void k_kspmv_omega(int n, int nblocks, int rpb, const int* __restrict__ rp,
-                                                        const int* __restrict__ ci, const double* __restrict__ v,
-                                                        const double* __restrict__ x, double* __restrict__ t,
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_kspmv_omega(R rows, const double* __restrict__ x, double* __restrict__ t,
                                                         const double* __restrict__ s, double* partials,
                                                         unsigned* tickets, KCtl* ctl) {
     if (stopped(ctl) || ctl->early) return;
     double tt = 0, ts = 0;
-    for_stream_blocks(n, nblocks, rpb, rp, ci, v, x, [&](int row, double acc) {
+    rows.each(x, [&](int row, double acc) {
         t[row] = acc;
         tt += acc * acc;
         ts += acc * s[row];
@@ -221,15 +343,14 @@ __global__ __launch_bounds__(kBlock) void k_kspmv_omega(int n, int nblocks, int 
 
 // r = b - A x and sum(r.^2) when the control block asks for it; with `shadow`
 // the new residual is also copied into the shadow vector (BiCGStab restart).
-__global__ __launch_bounds__(kBlock) void k_ktrue(int n, int nblocks, int rpb, const int* __restrict__ rp,
-                                                  const int* __restrict__ ci, const double* __restrict__ v,
-                                                  const double* __restrict__ b, const double* __restrict__ x,
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_ktrue(R rows, const double* __restrict__ b, const double* __restrict__ x,
                                                   double* __restrict__ r, double* __restrict__ shadow,
                                                   double* partials, unsigned* ticket, KCtl* ctl, int onEarly) {
     if (stopped(ctl)) return;
     if (onEarly ? !ctl->early : (!ctl->check || ctl->early)) return;
     double contrib = 0;
-    for_stream_blocks(n, nblocks, rpb, rp, ci, v, x, [&](int row, double acc) {
+    rows.each(x, [&](int row, double acc) {
         const double res = b[row] - acc;
         r[row] = res;
         if (shadow) shadow[row] = res;
@@ -360,12 +481,14 @@ AmgOptionsC amg_options_from(const tf_amg_options* o) {
         a.absorb = o->absorb_singletons != 0;
         a.cycle = o->cycle;
         a.precision = o->level_precision;
+        a.tail = o->coarse_tail;
     }
     // Amg.cuh:66-75 (coarse_rows widened to 1..1024).
     require(a.maxLevels >= 1 && a.maxLevels <= 20 && a.coarseRows >= 1 && a.coarseRows <= 1024 && a.rounds >= 1 &&
                 a.rounds <= 8 && a.pre >= 0 && a.post >= 0 && a.coarseSweeps >= 1 && std::isfinite(a.theta) &&
                 a.theta >= 0 && a.theta <= 1 && std::isfinite(a.omega) && a.omega > 0 && a.omega < 2 &&
-                a.cycle >= 0 && a.cycle <= 2 && (a.precision == 32 || a.precision == 64),
+                a.cycle >= 0 && a.cycle <= 2 && (a.precision == 32 || a.precision == 64) && a.tail >= 0 &&
+                a.tail <= 2,
             "Invalid AMG options");
     return a;
 }
@@ -377,6 +500,9 @@ namespace {
 void free_level(tf_ctx& c, AmgLevel& l, bool f32) {
     if (f32) dev_free(c, l.vals);
     l.vals = nullptr;
+    free_sell(c, l.sell);
+    dev_free(c, l.svals);
+    l.svals = nullptr;
     for (void** p : {&l.inv, &l.b, &l.x, &l.x2, &l.r}) {
         dev_free(c, *p);
         *p = nullptr;
@@ -390,10 +516,17 @@ void AmgEngine::release() {
     if (c_) (void)hipStreamSynchronize(c_->stream);
     for (auto& l : levels_) free_level(*c_, l, opt_.precision == 32);
     levels_.clear();
-    for (void* p : {static_cast<void*>(lu_), static_cast<void*>(piv_), static_cast<void*>(inverse_)}) dev_free(*c_, p);
+    for (void* p : {static_cast<void*>(lu_), static_cast<void*>(piv_), static_cast<void*>(inverse_),
+                    static_cast<void*>(tailLevels_), static_cast<void*>(tailOps_), tailImage_})
+        dev_free(*c_, p);
+    tailImage_ = nullptr;
+    tailImageBytes_ = 0;
     lu_ = nullptr;
     piv_ = nullptr;
     inverse_ = nullptr;
+    tailLevels_ = nullptr;
+    tailOps_ = nullptr;
+    tail_ = -1;
     factored_ = singular_ = false;
 }
 
@@ -487,6 +620,29 @@ void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const do
             factored = coarse_lu_factor(*c_, last.n, last.rp, last.ci, last.v, lu, piv, inv);
             singular = !factored;
         }
+        for (size_t d = 0; d < next.size(); ++d) {
+            AmgLevel& L = next[d];
+            if (L.native.n < kSellMinRows || (factored && d + 1 == next.size())) continue;
+            if (!sell_build(*c_, L.native.n, L.native.nnz, L.native.rp, L.native.ci, L.sell, kSellMaxPadding))
+                continue;
+            bool keep;
+            if (f32) {
+                L.svals = dev_alloc<float>(*c_, size_t(L.sell.stored));
+                sell_values<double, float>(*c_, L.sell, L.native.rp, L.native.v, static_cast<float*>(L.svals));
+                keep = sell_pays<float>(*c_, L, opt_.omega);
+            }
+            else {
+                L.svals = dev_alloc<double>(*c_, size_t(L.sell.stored));
+                sell_values<double, double>(*c_, L.sell, L.native.rp, L.native.v, static_cast<double*>(L.svals));
+                keep = sell_pays<double>(*c_, L, opt_.omega);
+            }
+            if (!keep) {
+                c_->sync();
+                free_sell(*c_, L.sell);
+                dev_free(*c_, L.svals);
+                L.svals = nullptr;
+            }
+        }
         c_->sync();
     }
     catch (...) {
@@ -503,6 +659,247 @@ void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const do
     stop_ = stop;
     coarseSolver_ = factored ? "pivoted GPU LU" : (singular ? "Jacobi (singular LU)" : "Jacobi (coarsening stopped)");
     visits_.assign(levels_.size(), 0);
+    build_tail();
+    tune_tail();
+}
+
+// The one-workgroup tail is kept only where it beats launching the same
+// operations level by level (timed once per hierarchy shape and cycle).
+void AmgEngine::tune_tail() {
+    if (tail_ < 0 || opt_.tail == 1) return;
+    if (opt_.tail == 2) {
+        tail_ = -1;
+        return;
+    }
+    int64_t sig = 0;
+    for (const auto& l : levels_) sig = sig * 1000003 + l.native.nnz;
+    const TuneKey key{2, levels_[0].native.n, sig, int64_t(opt_.cycle * 100 + opt_.pre * 10 + opt_.post),
+                      opt_.precision};
+    bool useTail;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tuned.find(key);
+        useTail = it != g_tuned.end() ? it->second : false;
+        if (it != g_tuned.end()) {
+            if (!useTail) tail_ = -1;
+            return;
+        }
+    }
+    const size_t n = size_t(levels_[0].native.n);
+    double* tmp = dev_alloc<double>(*c_, 2 * n);
+    fill_zero(*c_, tmp, 2 * n * sizeof(double));
+    const int t = tail_;
+    useTail = first_is_faster(
+        *c_,
+        [&] {
+            tail_ = t;
+            apply(tmp, tmp + n);
+        },
+        [&] {
+            tail_ = -1;
+            apply(tmp, tmp + n);
+        },
+        3);
+    c_->sync();
+    dev_free(*c_, tmp);
+    tail_ = useTail ? t : -1;
+    std::fill(visits_.begin(), visits_.end(), 0);
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tuned[key] = useTail;
+}
+
+namespace {
+// Host mirror of AmgEngine::cycle that records operations instead of
+// launching kernels (same flag logic: lazily cleared iterates, first sweeps
+// fused into the restriction), with explicit iterate buffers per level.
+struct TailGen {
+    const std::vector<AmgLevel>& levels;
+    const AmgOptionsC& opt;
+    bool factored;
+    std::vector<char> zero, pre;
+    std::vector<int> cur;
+    std::vector<int64_t> visits;
+    std::vector<TailOp> ops;
+
+    void emit(int kind, int level, int in, int out, int flag) { ops.push_back(TailOp{kind, level, in, out, flag}); }
+    void smooth(size_t d, int sweeps, int first) {
+        for (int s = first; s < sweeps; ++s) {
+            if (zero[d]) {
+                emit(kTSweepZero, int(d), 0, cur[d], 0);
+                zero[d] = 0;
+            }
+            else {
+                emit(kTSweep, int(d), cur[d], 1 - cur[d], 0);
+                cur[d] = 1 - cur[d];
+            }
+        }
+    }
+    void restrict_to(size_t d, bool ofResidual, bool fuse) {
+        if (!ofResidual || zero[d]) {
+            emit(kTRestrict, int(d), 0, cur[d + 1], fuse);
+        }
+        else {
+            emit(kTResidual, int(d), cur[d], 0, 0);
+            emit(kTRestrict, int(d), 1, cur[d + 1], fuse);
+        }
+        pre[d + 1] = fuse;
+    }
+    void prolong(size_t d) {
+        emit(kTProlong, int(d), cur[d + 1], cur[d], zero[d]);
+        zero[d] = 0;
+    }
+    void cycle(int kind, size_t d, bool z) {
+        ++visits[d];
+        int done = 0;
+        if (pre[d]) {
+            zero[d] = 0;
+            pre[d] = 0;
+            done = 1;
+        }
+        else if (z || kind == 2) {
+            zero[d] = 1;
+        }
+        if (d + 1 == levels.size()) {
+            if (factored) {
+                emit(kTCoarse, int(d), 0, cur[d], 0);
+                zero[d] = 0;
+            }
+            else {
+                smooth(d, opt.coarseSweeps, 0);
+            }
+            return;
+        }
+        const bool nxSmooths = d + 2 < levels.size() && opt.pre >= 1;
+        if (kind == 2) {
+            restrict_to(d, false, false);
+            cycle(2, d + 1, true);
+            prolong(d);
+            cycle(0, d, false);
+            return;
+        }
+        smooth(d, opt.pre, done);
+        restrict_to(d, true, nxSmooths);
+        cycle(kind, d + 1, true);
+        if (kind == 1) cycle(kind, d + 1, false);
+        prolong(d);
+        smooth(d, opt.post, 0);
+    }
+};
+}  // namespace
+
+// Chooses the tail's first level and records its programs for every entry
+// state (cycle kind, zero start, first sweep already applied by the parent's
+// restriction).
+void AmgEngine::build_tail() {
+    tail_ = -1;
+    if (levels_.size() < 2 || (factored_ && levels_.back().native.n > kCoarseMax)) return;
+    const size_t es = opt_.precision == 32 ? sizeof(float) : sizeof(double);
+    auto al = [](size_t b) { return (b + 15) / 16 * 16; };
+    // LDS bytes of levels d..coarsest: five vectors, row offsets and the transfer.
+    std::vector<size_t> need(levels_.size() + 1, 0);
+    for (size_t d = levels_.size(); d-- > 0;) {
+        const AmgLevel& l = levels_[d];
+        const size_t n = size_t(l.native.n);
+        size_t b = 5 * al(n * es) + al((n + 1) * 4);
+        if (l.transfer.nc > 0) b += 2 * al(n * 4) + al((size_t(l.transfer.nc) + 1) * 4);
+        need[d] = need[d + 1] + b;
+    }
+    int t = -1;
+    for (size_t d = 1; d < levels_.size(); ++d)
+        if (need[d] <= size_t(kTailLds) && levels_[d].native.n <= kTailMaxRows) {
+            t = int(d);
+            break;
+        }
+    if (t < 0) return;
+    std::vector<TailLevel> table(levels_.size());
+    size_t at = 0;
+    auto place = [&](size_t bytes) {
+        const int off = int(at);
+        at += al(bytes);
+        return off;
+    };
+    for (size_t d = size_t(t); d < levels_.size(); ++d) {
+        const AmgLevel& l = levels_[d];
+        TailLevel& e = table[d];
+        const size_t n = size_t(l.native.n);
+        e.n = l.native.n;
+        e.rp = l.native.rp;
+        e.ci = l.native.ci;
+        e.v = l.vals;
+        e.inv = l.inv;
+        e.b = l.b;
+        e.x = l.x;
+        e.nc = l.transfer.nc;
+        e.map = l.transfer.map;
+        e.off = l.transfer.offsets;
+        e.mem = l.transfer.members;
+        e.lb = place(n * es);
+        e.lx0 = place(n * es);
+        e.lx1 = place(n * es);
+        e.lr = place(n * es);
+        e.linv = place(n * es);
+        e.lrp = place((n + 1) * 4);
+        if (e.nc > 0) {
+            e.lmap = place(n * 4);
+            e.lmem = place(n * 4);
+            e.loff = place((size_t(e.nc) + 1) * 4);
+        }
+    }
+    std::vector<TailOp> all;
+    for (int key = 0; key < 12; ++key) {
+        const int kind = key / 4;
+        const bool zero = (key & 2) != 0, pre = (key & 1) != 0;
+        TailGen g{levels_, opt_, factored_, std::vector<char>(levels_.size(), 0),
+                  std::vector<char>(levels_.size(), 0), std::vector<int>(levels_.size(), 0),
+                  std::vector<int64_t>(levels_.size(), 0), {}};
+        g.pre[size_t(t)] = pre;
+        g.cycle(kind, size_t(t), zero);
+        if (g.cur[size_t(t)] != 0) g.emit(kTCopy, t, 1, 0, 0);
+        if (g.ops.size() > size_t(kMaxTailOps)) return;  // (W-cycles over very deep tails)
+        tailPrograms_[key].offset = int(all.size());
+        tailPrograms_[key].count = int(g.ops.size());
+        tailPrograms_[key].visits = g.visits;
+        all.insert(all.end(), g.ops.begin(), g.ops.end());
+    }
+    // LDS image: each level's inverse diagonal, row offsets and transfer at
+    // their LDS offsets (the vector slots in between are left undefined).
+    tailImageBytes_ = int(al(at));
+    tailImage_ = dev_alloc<char>(*c_, size_t(tailImageBytes_));
+    char* img = static_cast<char*>(tailImage_);
+    auto put = [&](int off, const void* src, size_t bytes) {
+        if (bytes) check(hipMemcpyAsync(img + off, src, bytes, hipMemcpyDeviceToDevice, c_->stream), "tail image");
+    };
+    for (size_t d = size_t(t); d < levels_.size(); ++d) {
+        const TailLevel& e = table[d];
+        const size_t n = size_t(e.n);
+        put(e.linv, e.inv, n * es);
+        put(e.lrp, e.rp, (n + 1) * 4);
+        if (e.nc > 0) {
+            put(e.lmap, e.map, n * 4);
+            put(e.lmem, e.mem, n * 4);
+            put(e.loff, e.off, (size_t(e.nc) + 1) * 4);
+        }
+    }
+    tailLevels_ = dev_alloc<TailLevel>(*c_, table.size());
+    tailOps_ = dev_alloc<TailOp>(*c_, all.size());
+    check(hipMemcpyAsync(tailLevels_, table.data(), table.size() * sizeof(TailLevel), hipMemcpyHostToDevice,
+                         c_->stream),
+          "tail table upload");
+    check(hipMemcpyAsync(tailOps_, all.data(), all.size() * sizeof(TailOp), hipMemcpyHostToDevice, c_->stream),
+          "tail program upload");
+    c_->sync();
+    tail_ = t;
+}
+
+template <class L>
+void AmgEngine::run_tail(int kind, bool zero) {
+    AmgLevel& l = levels_[size_t(tail_)];
+    const TailProgram& p = tailPrograms_[kind * 4 + (zero ? 2 : 0) + (l.preapplied ? 1 : 0)];
+    tail_run<L>(*c_, tailLevels_, tail_, int(levels_.size()) - tail_, tailImage_, tailImageBytes_,
+                tailOps_ + p.offset, p.count, inverse_, L(opt_.omega));
+    for (size_t d = 0; d < visits_.size(); ++d) visits_[d] += p.visits[d];
+    l.preapplied = false;
+    l.zero = false;
 }
 
 template <class L>
@@ -516,9 +913,14 @@ void AmgEngine::smooth(size_t d, int sweeps, int first) {
             l.zero = false;
         }
         else {
-            jacobi_fused<L>(*c_, A.n, A.nnz, A.rp, A.ci, static_cast<const L*>(l.vals), static_cast<const L*>(l.inv),
-                            L(opt_.omega), static_cast<const L*>(l.b), static_cast<const L*>(l.x),
-                            static_cast<L*>(l.x2));
+            if (l.sell)
+                sell_jacobi<L>(*c_, l.sell, static_cast<const L*>(l.svals), static_cast<const L*>(l.inv),
+                               L(opt_.omega), static_cast<const L*>(l.b), static_cast<const L*>(l.x),
+                               static_cast<L*>(l.x2));
+            else
+                jacobi_fused<L>(*c_, A.n, A.nnz, A.rp, A.ci, static_cast<const L*>(l.vals),
+                                static_cast<const L*>(l.inv), L(opt_.omega), static_cast<const L*>(l.b),
+                                static_cast<const L*>(l.x), static_cast<L*>(l.x2));
             std::swap(l.x, l.x2);
         }
     }
@@ -545,8 +947,12 @@ void AmgEngine::restrict_to(size_t d, bool ofResidual, bool fuseFirstSweep) {
                              static_cast<L*>(nx.b), inv, w, xc);
     }
     else {
-        residual<L>(*c_, l.native.n, l.native.nnz, l.native.rp, l.native.ci, static_cast<const L*>(l.vals),
-                    static_cast<const L*>(l.b), static_cast<const L*>(l.x), static_cast<L*>(l.r));
+        if (l.sell)
+            sell_residual<L>(*c_, l.sell, static_cast<const L*>(l.svals), static_cast<const L*>(l.b),
+                             static_cast<const L*>(l.x), static_cast<L*>(l.r));
+        else
+            residual<L>(*c_, l.native.n, l.native.nnz, l.native.rp, l.native.ci, static_cast<const L*>(l.vals),
+                        static_cast<const L*>(l.b), static_cast<const L*>(l.x), static_cast<L*>(l.r));
         restrict_agg<L>(*c_, t.nc, t.offsets, t.members, static_cast<const L*>(l.r), static_cast<L*>(nx.b), inv, w,
                         xc);
     }
@@ -569,6 +975,10 @@ void AmgEngine::coarse(size_t d) {
 // fused kernels; the clear of a zero initial guess is tracked lazily.
 template <class L>
 void AmgEngine::cycle(int kind, size_t d, bool zero) {
+    if (int(d) == tail_) {
+        run_tail<L>(kind, zero);
+        return;
+    }
     ++visits_[d];
     AmgLevel& l = levels_[d];
     int done = 0;
@@ -666,6 +1076,8 @@ SolverEngine::~SolverEngine() {
     if (c_) (void)hipStreamSynchronize(c_->stream);
     release_graphs();
     free_csr(*c_, A_);
+    free_sell(*c_, As_);
+    dev_free(*c_, Asv_);
     dev_free(*c_, vec_);
     if (ctl_) (void)hipFree(ctl_);
     if (hctl_) (void)hipHostFree(hctl_);
@@ -680,6 +1092,9 @@ void SolverEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const
         vec_ = nullptr;
     }
     free_csr(*c_, A_);
+    free_sell(*c_, As_);
+    dev_free(*c_, Asv_);
+    Asv_ = nullptr;
     A_ = A;
     n_ = n;
     const int nv = kind_ == 0 ? 6 : 11;
@@ -703,8 +1118,37 @@ void SolverEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const
         ph_ = take();
         shat_ = take();
     }
+    if (n >= kSellMinRows && sell_build(*c_, A_.n, A_.nnz, A_.rp, A_.ci, As_, kSellMaxPadding)) {
+        Asv_ = dev_alloc<double>(*c_, size_t(As_.stored));
+        sell_values<double, double>(*c_, As_, A_.rp, A_.v, Asv_);
+        if (!sell_spmv_pays()) {
+            c_->sync();
+            free_sell(*c_, As_);
+            dev_free(*c_, Asv_);
+            Asv_ = nullptr;
+        }
+    }
     if (amg_) amg_->setup(A_.n, A_.nnz, A_.rp, A_.ci, A_.v, true);
     c_->sync();
+}
+
+// The operator's products run in sliced ELL only where that beats the CSR
+// kernel (timed once per operator shape, as for the AMG levels).
+bool SolverEngine::sell_spmv_pays() {
+    if (A_.nnz < kTuneMinNnz) return true;
+    const TuneKey key{1, A_.n, A_.nnz, As_.stored, 8};
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) return it->second;
+    }
+    fill_zero(*c_, p_, size_t(n_) * sizeof(double));
+    const bool sell = first_is_faster(
+        *c_, [&] { sell_spmv<double>(*c_, As_, Asv_, p_, q_); },
+        [&] { spmv<double>(*c_, A_.n, A_.nnz, A_.rp, A_.ci, A_.v, p_, q_); });
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tuned[key] = sell;
+    return sell;
 }
 
 void SolverEngine::precondition(const double* in, double* out) {
@@ -725,9 +1169,23 @@ double SolverEngine::norm_host(const double* x) {
 // One iteration of the solver as a fixed launch sequence (part 0).
 void SolverEngine::launch_part(int part) {
     (void)part;
+    if (!As_) {
+        const int rpb = stream_rows_per_block<double>(A_.n, A_.nnz);
+        const int nb = (A_.n + rpb - 1) / rpb;
+        launch_iteration(CsrRows{A_.n, nb, rpb, A_.rp, A_.ci, A_.v}, nb);
+    }
+    else {
+        const int nb = (As_.slices + kBlock / kWave - 1) / (kBlock / kWave);
+        if (As_.narrow)
+            launch_iteration(SellRows<short>{As_.view<short>(), Asv_}, nb);
+        else
+            launch_iteration(SellRows<int>{As_.view<int>(), Asv_}, nb);
+    }
+}
+
+template <class R>
+void SolverEngine::launch_iteration(const R& rows, int nb) {
     KCtl* ctl = static_cast<KCtl*>(ctl_);
-    const int rpb = stream_rows_per_block<double>(A_.n, A_.nnz);
-    const int nb = (A_.n + rpb - 1) / rpb;
     const dim3 grid(std::min(round_xcd(nb), kMaxReduceBlocks)), vb(vec_blocks(n_));
     double* P = c_->partials;
     unsigned* T = c_->tickets;
@@ -743,13 +1201,12 @@ void SolverEngine::launch_part(int part) {
                            int(kFinRhoCg));
         fin(kSlotA, nv, kFinRhoCg, 0);
         hipLaunchKernelGGL(k_cg_p, vb, dim3(kBlock), 0, c_->stream, n_, z_, p_, ctl);
-        TF_G(k_kspmv_dot, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, p_, q_, p_, pslot(kSlotB), T + kSlotB, ctl, int(kFinCurvCg));
+        TF_G(k_kspmv_dot<R>, rows, p_, q_, p_, pslot(kSlotB), T + kSlotB, ctl, int(kFinCurvCg));
         fin(kSlotB, ng, kFinCurvCg, 0);
         hipLaunchKernelGGL(k_cg_update, vb, dim3(kBlock), 0, c_->stream, n_, p_, q_, x_, r_, pslot(kSlotC),
                            T + kSlotC, ctl);
         fin(kSlotC, nv, kFinUpdCg, 0);
-        TF_G(k_ktrue, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, static_cast<double*>(nullptr), pslot(kSlotD),
-             T + kSlotD, ctl, 0);
+        TF_G(k_ktrue<R>, rows, rhs_, x_, r_, static_cast<double*>(nullptr), pslot(kSlotD), T + kSlotD, ctl, 0);
         fin(kSlotD, ng, kFinTrue, 0);
     }
     else {
@@ -759,20 +1216,20 @@ void SolverEngine::launch_part(int part) {
         fin(kSlotA, nv, kFinRhoBi, 0);
         hipLaunchKernelGGL(k_bi_p, vb, dim3(kBlock), 0, c_->stream, n_, r_, v, p_, ctl);
         precondition(p_, ph_);
-        TF_G(k_kspmv_dot, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, ph_, v, sh_, pslot(kSlotB), T + kSlotB, ctl, int(kFinDenBi));
+        TF_G(k_kspmv_dot<R>, rows, ph_, v, sh_, pslot(kSlotB), T + kSlotB, ctl, int(kFinDenBi));
         fin(kSlotB, ng, kFinDenBi, 0);
         hipLaunchKernelGGL(k_bi_s, vb, dim3(kBlock), 0, c_->stream, n_, r_, v, ph_, s_, x_, pslot(kSlotC), T + kSlotC,
                            ctl);
         fin(kSlotC, nv, kFinSBi, 0);
-        TF_G(k_ktrue, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 1);
+        TF_G(k_ktrue<R>, rows, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 1);
         fin(kSlotD, ng, kFinTrue, 1);
         precondition(s_, shat_);
-        TF_G(k_kspmv_omega, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, shat_, t_, s_, pslot(kSlotE), T + kSlotE, ctl);
+        TF_G(k_kspmv_omega<R>, rows, shat_, t_, s_, pslot(kSlotE), T + kSlotE, ctl);
         fin(kSlotE, ng, kFinOmega, 0);
         hipLaunchKernelGGL(k_bi_update, vb, dim3(kBlock), 0, c_->stream, n_, shat_, s_, t_, x_, r_, pslot(kSlotA),
                            T + kSlotA, ctl);
         fin(kSlotA, nv, kFinUpdBi, 0);
-        TF_G(k_ktrue, A_.n, nb, rpb, A_.rp, A_.ci, A_.v, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 0);
+        TF_G(k_ktrue<R>, rows, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 0);
         fin(kSlotD, ng, kFinTrue, 0);
     }
 #undef TF_G
@@ -803,8 +1260,15 @@ void SolverEngine::capture_or_launch(int part) {
     check(hipGraphLaunch(graph_[part], c_->stream), "graph launch");
 }
 
+void SolverEngine::true_residual() {
+    if (As_)
+        sell_residual<double>(*c_, As_, Asv_, rhs_, x_, r_);
+    else
+        residual<double>(*c_, A_.n, A_.nnz, A_.rp, A_.ci, A_.v, rhs_, x_, r_);
+}
+
 tf_solve_result SolverEngine::finish(int status, int iterations) {
-    residual<double>(*c_, A_.n, A_.nnz, A_.rp, A_.ci, A_.v, rhs_, x_, r_);
+    true_residual();
     const double res = norm_host(r_);
     if (!std::isfinite(res) || !std::isfinite(normb_))
         status = kNonFinite;
@@ -827,7 +1291,7 @@ tf_solve_result SolverEngine::apply(const double* b, double* x) {
     tf_solve_result out{};
     normb_ = norm_host(rhs_);
     tol_ = std::max(o_.atol, o_.rtol * normb_);
-    residual<double>(*c_, A_.n, A_.nnz, A_.rp, A_.ci, A_.v, rhs_, x_, r_);
+    true_residual();
     const double res = norm_host(r_);
     if (!std::isfinite(normb_) || !std::isfinite(res)) {
         out = finish(kNonFinite, 0);

@@ -15,6 +15,7 @@ struct AmgOptionsC {
     bool absorb = true;
     int cycle = 0;
     int precision = 64;
+    int tail = 0;  // 0 timed, 1 always, 2 never
 };
 AmgOptionsC amg_options_from(const tf_amg_options* o);
 
@@ -27,10 +28,45 @@ struct AmgLevel {
     void* x2 = nullptr;     // Jacobi ping-pong partner
     void* r = nullptr;      // residual
     DevAgg transfer;        // to the next coarser level (empty on the coarsest)
+    DevSell sell;           // sliced-ELL copy for the sweeps and residuals (empty: CSR kernels)
+    void* svals = nullptr;  // its values, level precision
     bool zero = false;      // iterate is known to be zero (cleared lazily)
     bool preapplied = false; // x already holds the first sweep from zero (fused upstream)
     int64_t hist[4] = {0, 0, 0, 0};
 };
+
+// The cycle below a small level runs as one program of one 1024-thread
+// workgroup (tf_tail.hip): the same operations as the per-level kernels, in
+// the same order and arithmetic, without a launch per operation.
+enum TailKind { kTSweep = 0, kTSweepZero = 1, kTResidual = 2, kTRestrict = 3, kTProlong = 4, kTCoarse = 5, kTCopy = 6 };
+struct TailOp {
+    int kind, level;
+    int in, out;  // iterate buffers (0: x, 1: x2); kTRestrict: in 0 restricts b, 1 restricts r
+    int flag;     // kTRestrict: fuse the next level's first sweep; kTProlong: overwrite (zero iterate)
+};
+struct TailLevel {
+    int n;
+    const int* rp;
+    const int* ci;
+    const void* v;  // values, level precision
+    const void* inv;
+    void* b;        // global rhs and iterate: the entry level's are read at start,
+    void* x;        // its iterate written back at the end
+    int nc;         // rows of the next level (0 on the coarsest)
+    const int* map;
+    const int* off;
+    const int* mem;
+    // byte offsets of this level's arrays in the workgroup's LDS
+    int lb, lx0, lx1, lr, linv, lrp, lmap, loff, lmem;
+};
+// LDS bytes for the tail's vectors and index arrays.
+constexpr int kTailLds = 104 * 1024;
+constexpr int kTailMaxRows = 4096;  // rows of a tail level (4 per thread)
+constexpr int kMaxTailLevels = 20;
+constexpr int kMaxTailOps = 256;    // operations of one program
+template <class L>
+void tail_run(tf_ctx& c, const TailLevel* levels, int top, int count, const void* image, int imageBytes,
+              const TailOp* ops, int nops, const double* inverse, L w);
 
 class AmgEngine {
 public:
@@ -60,7 +96,24 @@ private:
     void coarse(size_t d);
     template <class L>
     void apply_typed(const double* rhs, double* x);
+    template <class L>
+    void run_tail(int kind, bool zero);
+    void build_tail();
+    void tune_tail();
     void release();
+
+    // One-workgroup tail: first level it covers (-1: none), level table,
+    // and the programs for every (cycle kind, zero start, pre-applied sweep).
+    struct TailProgram {
+        int offset = 0, count = 0;
+        std::vector<int64_t> visits;
+    };
+    int tail_ = -1;
+    TailLevel* tailLevels_ = nullptr;
+    TailOp* tailOps_ = nullptr;
+    void* tailImage_ = nullptr;  // the levels' constant arrays in their LDS layout
+    int tailImageBytes_ = 0;
+    TailProgram tailPrograms_[12];
 
     tf_ctx* c_;
     bool factored_ = false, singular_ = false;
@@ -90,7 +143,11 @@ private:
     void precondition(const double* in, double* out);
     void capture_or_launch(int part);
     void launch_part(int part);
+    template <class R>
+    void launch_iteration(const R& rows, int nb);
     tf_solve_result finish(int status, int iterations);
+    void true_residual();  // r = rhs - A x
+    bool sell_spmv_pays();
     double norm_host(const double* x);
     void release_graphs();
 
@@ -99,6 +156,8 @@ private:
     KrylovOptionsC o_;
     AmgEngine* amg_;
     DevCsr A_;
+    DevSell As_;             // sliced-ELL copy of A_ for the iteration's products (empty: CSR)
+    double* Asv_ = nullptr;
     int n_ = 0;
     double* vec_ = nullptr;  // all work vectors, contiguous
     double *rhs_ = nullptr, *x_ = nullptr, *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *q_ = nullptr;

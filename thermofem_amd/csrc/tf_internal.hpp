@@ -10,6 +10,7 @@
 #include <unordered_map>
 
 #include "../../include/thermofem_hip.h"
+#include "tf_device.hpp"
 
 namespace tf {
 
@@ -246,6 +247,35 @@ struct DevAgg {
     int64_t hist[4] = {0, 0, 0, 0};
 };
 void free_agg(tf_ctx& c, DevAgg& t);
+// Sliced-ELL copy of a CSR structure (tf_sell.hip); values live in separate
+// arrays of the element type the kernels run in.
+constexpr int kSellWsSlot = 28;
+struct DevSell {
+    int n = 0, slices = 0;
+    int64_t stored = 0;        // entries including padding
+    bool narrow = false;       // 16-bit column offsets from the row
+    int* base = nullptr;
+    unsigned char* len = nullptr;
+    void* col = nullptr;
+    int* perm = nullptr;       // row at each position, nullptr when rows keep their order
+    template <class C>
+    SellView<C> view() const {
+        return SellView<C>{n, slices, base, len, static_cast<const C*>(col), perm};
+    }
+    explicit operator bool() const { return base != nullptr; }
+};
+// Builds the structure; false (nothing kept) if a row has more than 255
+// entries or padding would exceed maxPadding * nnz.
+bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, DevSell& s, double maxPadding);
+template <class A, class T>
+void sell_values(tf_ctx& c, const DevSell& s, const int* rp, const A* v, T* out);
+void free_sell(tf_ctx& c, DevSell& s);
+template <class T>
+void sell_spmv(tf_ctx& c, const DevSell& s, const T* v, const T* x, T* y);
+template <class T>
+void sell_residual(tf_ctx& c, const DevSell& s, const T* v, const T* b, const T* x, T* r);
+template <class T>
+void sell_jacobi(tf_ctx& c, const DevSell& s, const T* v, const T* inv, T w, const T* b, const T* xin, T* xout);
 template <class T>
 T* dev_alloc(tf_ctx& c, size_t count) {
     return static_cast<T*>(c.pool.get((count ? count : 1) * sizeof(T)));

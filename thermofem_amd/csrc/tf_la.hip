@@ -8,33 +8,6 @@
 namespace tf {
 namespace {
 
-enum Epilogue { kSpmv = 0, kResidual = 1, kJacobi = 2 };
-
-// CSR row products with fused epilogues.  Two layouts:
-//  * stream (default): LDS-staged row blocks (csr_stream_rows), coalesced
-//    unrolled streaming of columns/values, sequential row sums;
-//  * group: G lanes per row with a shuffle reduction (kept for very long
-//    rows and as an autotuner candidate).
-// Logical blocks are laid out XCD-contiguously.  Epilogues are evaluated
-// without contraction: y = sum, r = b - sum, x + (w*inv)*(b - sum).
-template <class T, int E>
-__device__ __forceinline__ void epilogue(int row, T acc, const T* __restrict__ x, T* __restrict__ y,
-                                         const T* __restrict__ b, const T* __restrict__ inv, T w) {
-#pragma clang fp contract(off)
-    if constexpr (E == kSpmv) {
-        y[row] = acc;
-    }
-    else if constexpr (E == kResidual) {
-        y[row] = b[row] - acc;
-    }
-    else {
-        const T wd = w * inv[row];
-        const T d = b[row] - acc;
-        const T upd = wd * d;
-        y[row] = x[row] + upd;
-    }
-}
-
 template <class T, int E>
 __global__ __launch_bounds__(kBlock) void k_csr_stream(int n, int nblocks, int rpb, const int* __restrict__ rp,
                                                        const int* __restrict__ ci, const T* __restrict__ v,

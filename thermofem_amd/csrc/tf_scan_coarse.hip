@@ -233,16 +233,8 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_inverse(int n, const double* 
 // x = inv * b in FP64; b is staged in LDS first so rhs and x may alias.
 template <class T>
 __global__ __launch_bounds__(kLuThreads) void k_inv_apply(int n, const double* __restrict__ inv, const T* b, T* x) {
-    __shared__ double bs[1024];
-    for (int i = int(threadIdx.x); i < n; i += kLuThreads) bs[i] = double(b[i]);
-    __syncthreads();
-    const int lane = int(threadIdx.x) & (kWave - 1), w = int(threadIdx.x) / kWave;
-    for (int r = w; r < n; r += kLuThreads / kWave) {
-        double acc = 0;
-        for (int j = lane; j < n; j += kWave) acc += inv[size_t(r) * n + j] * bs[j];
-        acc = wave_sum(acc);
-        if (lane == 0) x[r] = T(acc);
-    }
+    __shared__ double bs[kCoarseMax];
+    coarse_inverse_apply<T>(n, inv, b, x, bs);
 }
 
 // Triangular solve of GpuAggregation.cu:904-939 (pivots, forward, backward)

@@ -1,0 +1,249 @@
+// Sliced-ELL storage of a CSR matrix for the row-product kernels on gfx950.
+//
+// Rows are grouped in slices of 64 consecutive rows (one wave).  Slice s
+// stores its entries slice-major: entry k of row 64s+l sits at
+// base[s] + 64k + l, for k < width(s) = the longest row of the slice, so the
+// wave's k-th loads are 64 consecutive elements (one or two cache lines per
+// array) and no LDS staging or block barrier is needed.  Columns are kept as
+// 16-bit offsets from the row whenever every offset of the matrix fits
+// (stencil and aggregation-coarsened matrices), which cuts the index bytes
+// per entry from 4 to 2; otherwise as 32-bit columns.  Row lengths are one
+// byte per row (rows longer than 255 entries are not representable: the
+// CSR kernels serve those matrices).  When rows of very different lengths
+// share slices (aggregation-coarsened levels), rows are reordered by
+// decreasing length inside windows of 256 rows (SELL-C-sigma with C = 64,
+// sigma = 256) and a position -> row table is kept; results are written to
+// the original rows, so the reordering is invisible.  Each lane adds its row's rounded
+// products sequentially in stored (ascending column) order, so the row sums
+// equal the CSR kernels' and the oracle's bit for bit.  Padding slots are
+// never read.
+#include <climits>
+#include <vector>
+
+#include "tf_device.hpp"
+#include "tf_internal.hpp"
+
+namespace tf {
+namespace {
+
+constexpr int kSliceRows = kWave;
+constexpr int kSlicesPerBlock = kBlock / kWave;
+constexpr int kSortWindow = 256;
+
+__global__ __launch_bounds__(kBlock) void k_sell_width(int n, int slices, const int* __restrict__ rp,
+                                                       const int* __restrict__ ci, int* __restrict__ width64,
+                                                       int* __restrict__ flags) {
+    const int slice = int(blockIdx.x) * kSlicesPerBlock + int(threadIdx.x) / kWave;
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    const int row = slice * kSliceRows + lane;
+    int len = 0;
+    if (slice < slices && row < n) {
+        const int s = rp[row], e = rp[row + 1];
+        len = e - s;
+        if (len > 255) flags[0] = 1;
+        bool wide = false;
+        for (int j = s; j < e; ++j) {
+            const int d = ci[j] - row;
+            wide |= d < -32768 || d > 32767;
+        }
+        if (wide) flags[1] = 1;
+    }
+    int m = len;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
+    if (slice < slices && lane == 0) width64[slice] = m * kSliceRows;
+    if (slice == slices - 1 && lane == 0) width64[slices] = 0;
+}
+
+// Rows of each 256-row window ranked by (length descending, row ascending);
+// writes the position -> row table and the width of every slice.
+__global__ __launch_bounds__(kSortWindow) void k_sell_sort(int n, int slices, const int* __restrict__ rp,
+                                                           int* __restrict__ perm, int* __restrict__ width64) {
+    __shared__ int lens[kSortWindow];
+    const int w0 = int(blockIdx.x) * kSortWindow;
+    const int i = int(threadIdx.x);
+    const int row = w0 + i;
+    const int len = row < n ? rp[row + 1] - rp[row] : -1;
+    lens[i] = len;
+    __syncthreads();
+    if (row >= n) return;
+    int rank = 0;
+    for (int j = 0; j < kSortWindow; ++j) {
+        const int lj = lens[j];
+        rank += (lj > len) || (lj == len && j < i);
+    }
+    const int pos = w0 + rank;
+    perm[pos] = row;
+    if ((pos & (kSliceRows - 1)) == 0) width64[pos / kSliceRows] = len * kSliceRows;
+    if (pos / kSliceRows == slices - 1 && (pos & (kSliceRows - 1)) == 0) width64[slices] = 0;
+}
+
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                      const int* __restrict__ perm, const int* __restrict__ base,
+                                                      unsigned char* __restrict__ len, C* __restrict__ col) {
+    const int pos = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (pos >= n) return;
+    const int row = perm ? perm[pos] : pos;
+    const int s = rp[row], e = rp[row + 1];
+    len[pos] = static_cast<unsigned char>(e - s);
+    const int off = base[pos / kSliceRows] + (pos & (kSliceRows - 1));
+    for (int j = s; j < e; ++j) {
+        if constexpr (sizeof(C) == 2)
+            col[off + (j - s) * kSliceRows] = static_cast<C>(ci[j] - row);
+        else
+            col[off + (j - s) * kSliceRows] = ci[j];
+    }
+}
+
+template <class A, class T>
+__global__ __launch_bounds__(kBlock) void k_sell_values(int n, const int* __restrict__ rp, const int* __restrict__ perm,
+                                                        const int* __restrict__ base, const A* __restrict__ v,
+                                                        T* __restrict__ out) {
+    const int pos = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (pos >= n) return;
+    const int row = perm ? perm[pos] : pos;
+    const int s = rp[row], e = rp[row + 1];
+    const int off = base[pos / kSliceRows] + (pos & (kSliceRows - 1));
+    for (int j = s; j < e; ++j) out[off + (j - s) * kSliceRows] = static_cast<T>(v[j]);
+}
+
+template <class T, class C, int E>
+__global__ __launch_bounds__(kBlock) void k_sell(SellView<C> m, const T* __restrict__ v, const T* __restrict__ x,
+                                                 T* __restrict__ y, const T* __restrict__ b,
+                                                 const T* __restrict__ inv, T w) {
+    const int blk = xcd_block((m.slices + kSlicesPerBlock - 1) / kSlicesPerBlock);
+    if (blk < 0) return;
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    const int slice = blk * kSlicesPerBlock + int(threadIdx.x) / kWave;
+    const int pos = slice * kSliceRows + lane;
+    if (pos >= m.n) return;
+    const int off = m.base[slice] + lane;
+    const int row = m.perm ? m.perm[pos] : pos;
+    const T acc = sell_row_sum<T>(row, int(m.len[pos]), v + off, m.col + off, [&](int c) { return x[c]; });
+    epilogue<T, E>(row, acc, x, y, b, inv, w);
+}
+
+}  // namespace
+
+namespace {
+// Slice offsets from the widths (exclusive scan); false if the total would
+// pass the int range or the padding limit.  Sets s.base and s.stored.
+bool sell_offsets(tf_ctx& c, int slices, int64_t nnz, const int* width64, double maxPadding, DevSell& s) {
+    if (int64_t(slices) * kSliceRows * 255 > int64_t(INT32_MAX)) {
+        // The offsets are scanned in int: check the exact total first when it could overflow.
+        std::vector<int> w(static_cast<size_t>(slices));
+        check(hipMemcpyAsync(w.data(), width64, size_t(slices) * sizeof(int), hipMemcpyDeviceToHost, c.stream),
+              "sell widths");
+        c.sync();
+        int64_t total = 0;
+        for (int x : w) total += x;
+        if (total > int64_t(INT32_MAX)) return false;
+    }
+    int* base = dev_alloc<int>(c, size_t(slices) + 1);
+    scan_excl(c, slices + 1, width64, base);
+    const int64_t stored = c.read_int(base + slices);
+    if (double(stored) > double(nnz) * (1.0 + maxPadding)) {
+        dev_free(c, base);
+        return false;
+    }
+    s.base = base;
+    s.stored = stored;
+    return true;
+}
+}  // namespace
+
+bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, DevSell& s, double maxPadding) {
+    s = DevSell{};
+    if (n <= 0) return false;
+    const int slices = (n + kSliceRows - 1) / kSliceRows;
+    int* width64 = c.wst<int>(kSellWsSlot, size_t(slices) + 1);
+    c.clear_flags(2);
+    hipLaunchKernelGGL(k_sell_width, dim3((slices + kSlicesPerBlock - 1) / kSlicesPerBlock), dim3(kBlock), 0,
+                       c.stream, n, slices, rp, ci, width64, c.dflags);
+    check(hipGetLastError(), "sell width launch");
+    c.read_flags(2);
+    if (c.hflags[0]) return false;  // a row over 255 entries
+    const bool wide = c.hflags[1] != 0;
+    if (!sell_offsets(c, slices, nnz, width64, maxPadding, s)) {
+        // Rows of one slice differ too much in length: order rows by length in 256-row windows.
+        int* perm = dev_alloc<int>(c, size_t(n));
+        hipLaunchKernelGGL(k_sell_sort, dim3((n + kSortWindow - 1) / kSortWindow), dim3(kSortWindow), 0, c.stream, n,
+                           slices, rp, perm, width64);
+        check(hipGetLastError(), "sell sort launch");
+        if (!sell_offsets(c, slices, nnz, width64, maxPadding, s)) {
+            dev_free(c, perm);
+            return false;
+        }
+        s.perm = perm;
+    }
+    s.n = n;
+    s.slices = slices;
+    s.narrow = !wide;
+    s.len = dev_alloc<unsigned char>(c, size_t(n));
+    s.col = s.narrow ? static_cast<void*>(dev_alloc<short>(c, size_t(s.stored)))
+                     : static_cast<void*>(dev_alloc<int>(c, size_t(s.stored)));
+    const dim3 grid((n + kBlock - 1) / kBlock);
+    if (s.narrow)
+        hipLaunchKernelGGL(k_sell_fill<short>, grid, dim3(kBlock), 0, c.stream, n, rp, ci, s.perm, s.base, s.len,
+                           static_cast<short*>(s.col));
+    else
+        hipLaunchKernelGGL(k_sell_fill<int>, grid, dim3(kBlock), 0, c.stream, n, rp, ci, s.perm, s.base, s.len,
+                           static_cast<int*>(s.col));
+    check(hipGetLastError(), "sell fill launch");
+    return true;
+}
+
+template <class A, class T>
+void sell_values(tf_ctx& c, const DevSell& s, const int* rp, const A* v, T* out) {
+    if (s.n <= 0) return;
+    hipLaunchKernelGGL((k_sell_values<A, T>), dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, s.n, rp,
+                       s.perm, s.base, v, out);
+    check(hipGetLastError(), "sell values launch");
+}
+
+void free_sell(tf_ctx& c, DevSell& s) {
+    dev_free(c, s.perm);
+    dev_free(c, s.base);
+    dev_free(c, s.len);
+    dev_free(c, s.col);
+    s = DevSell{};
+}
+
+template <class T, int E>
+void sell_rows(tf_ctx& c, const DevSell& s, const T* v, const T* x, T* y, const T* b, const T* inv, T w) {
+    if (s.n <= 0) return;
+    const dim3 grid(round_xcd((s.slices + kSlicesPerBlock - 1) / kSlicesPerBlock));
+    if (s.narrow)
+        hipLaunchKernelGGL((k_sell<T, short, E>), grid, dim3(kBlock), 0, c.stream, s.view<short>(), v, x, y, b, inv,
+                           w);
+    else
+        hipLaunchKernelGGL((k_sell<T, int, E>), grid, dim3(kBlock), 0, c.stream, s.view<int>(), v, x, y, b, inv, w);
+    check(hipGetLastError(), "sell kernel launch");
+}
+
+template <class T>
+void sell_spmv(tf_ctx& c, const DevSell& s, const T* v, const T* x, T* y) {
+    sell_rows<T, kSpmv>(c, s, v, x, y, nullptr, nullptr, T(0));
+}
+template <class T>
+void sell_residual(tf_ctx& c, const DevSell& s, const T* v, const T* b, const T* x, T* r) {
+    sell_rows<T, kResidual>(c, s, v, x, r, b, nullptr, T(0));
+}
+template <class T>
+void sell_jacobi(tf_ctx& c, const DevSell& s, const T* v, const T* inv, T w, const T* b, const T* xin, T* xout) {
+    sell_rows<T, kJacobi>(c, s, v, xin, xout, b, inv, w);
+}
+
+template void sell_values<double, double>(tf_ctx&, const DevSell&, const int*, const double*, double*);
+template void sell_values<double, float>(tf_ctx&, const DevSell&, const int*, const double*, float*);
+template void sell_values<float, float>(tf_ctx&, const DevSell&, const int*, const float*, float*);
+#define TF_SELL(T)                                                                                        \
+    template void sell_spmv<T>(tf_ctx&, const DevSell&, const T*, const T*, T*);                          \
+    template void sell_residual<T>(tf_ctx&, const DevSell&, const T*, const T*, const T*, T*);            \
+    template void sell_jacobi<T>(tf_ctx&, const DevSell&, const T*, const T*, T, const T*, const T*, T*);
+TF_SELL(float)
+TF_SELL(double)
+#undef TF_SELL
+
+}  // namespace tf

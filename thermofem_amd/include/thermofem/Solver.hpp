@@ -255,6 +255,7 @@ private:
         o.absorb_singletons = options_.absorbSingletons_ ? 1 : 0;
         o.cycle = int(options_.cycle_);
         o.level_precision = std::same_as<T, float> ? 32 : 64;
+        o.coarse_tail = 0;
         return o;
     }
     void ensureEngine(int device) {

@@ -51,11 +51,13 @@ class AmgOptions:
     absorb_singletons: bool = True
     cycle: Cycle = Cycle.V
     level_precision: int = 64  # 32: FP32 cycle under the FP64 caller
+    coarse_tail: int = 0  # coarse levels as one one-workgroup program: 0 timed at setup, 1 always, 2 never
 
     def c(self):
         return _lib.AmgOptions(self.max_levels, self.coarse_rows, self.matching_rounds, self.pre_sweeps,
                                self.post_sweeps, self.coarse_sweeps, self.jacobi_weight, self.strength_threshold,
-                               int(self.absorb_singletons), int(self.cycle), self.level_precision)
+                               int(self.absorb_singletons), int(self.cycle), self.level_precision,
+                               self.coarse_tail)
 
 
 @dataclass
