@@ -195,15 +195,20 @@ __global__ void k_struct_sym(int n, const int* __restrict__ rp, const int* __res
 // are dropped.  This is exactly the max-merge of both emitted directions.
 // One pass writes the weights in the layout of A (column -1 = dropped) and
 // the kept count per row; k_compact_rows packs them.
+template <int G>
 __global__ void k_strength_sym(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                const double* __restrict__ a, const double* __restrict__ rowmax, double theta,
                                int* __restrict__ scol, double* __restrict__ sw, int* __restrict__ cnt) {
-    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i == n) cnt[n] = 0;
-    if (i >= n) return;
+    // G lanes per row, one entry per lane: the loads of a row are coalesced and
+    // the binary searches in the neighbours' rows run side by side.
+    const int i = int((int64_t(blockIdx.x) * kBlock + threadIdx.x) / G);
+    const int lane = int(threadIdx.x) % G;
+    if (i == n && lane == 0) cnt[n] = 0;
+    if (i >= n) return;  // whole groups leave together
     const double mi = rowmax[i];
+    const int s = rp[i], e = rp[i + 1];
     int kept = 0;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+    for (int j = s + lane; j < e; j += G) {
         const int c = ci[j];
         double w = 0.0;
         if (c != i) {
@@ -219,23 +224,48 @@ __global__ void k_strength_sym(int n, const int* __restrict__ rp, const int* __r
         sw[j] = w;
         kept += w > 0 ? 1 : 0;
     }
-    cnt[i] = kept;
+    kept = group_sum<G>(kept);
+    if (lane == 0) cnt[i] = kept;
 }
 
+// Packs the kept entries of each row in order: G lanes per row, positions
+// from a ballot within the group.
+template <int G>
 __global__ void k_compact_rows(int n, const int* __restrict__ rp, const int* __restrict__ scol,
                                const double* __restrict__ sw, const int* __restrict__ orp, int* __restrict__ oci,
                                double* __restrict__ ow) {
-    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    const int i = int((int64_t(blockIdx.x) * kBlock + threadIdx.x) / G);
+    const int lane = int(threadIdx.x) % G;
     if (i >= n) return;
+    const int shift = (int(threadIdx.x) & (kWave - 1)) / G * G;
+    const u64 groupMask = G == 64 ? ~0ull : ((1ull << G) - 1);
+    const int e = rp[i + 1];
     int o = orp[i];
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
-        const int c = scol[j];
+    for (int j0 = rp[i]; j0 < e; j0 += G) {
+        const int j = j0 + lane;
+        const int c = j < e ? scol[j] : -1;
+        const u64 m = (u64(__ballot(c >= 0)) >> shift) & groupMask;
         if (c >= 0) {
-            oci[o] = c;
-            ow[o] = sw[j];
-            ++o;
+            const int at = o + __popcll(m & ((1ull << lane) - 1));
+            oci[at] = c;
+            ow[at] = sw[j];
         }
+        o += __popcll(m);
     }
+}
+
+// Lanes per row for the row-parallel setup kernels: the smallest power of
+// two at least the average row length, 4..32.
+inline int lanes_per_row(int n, int64_t nnz) {
+    const double avg = n > 0 ? double(nnz) / double(n) : 1.0;
+    int g = 4;
+    while (g < 32 && g < avg) g <<= 1;
+    return g;
+}
+
+inline dim3 grid_rows(int64_t rows, int g) {
+    const int64_t threads = rows * g;
+    return dim3(unsigned((threads + kBlock - 1) / kBlock > 0 ? (threads + kBlock - 1) / kBlock : 1));
 }
 
 // Output row I gathers the rows of its members (all rows of `offsets`
@@ -739,8 +769,9 @@ __global__ __launch_bounds__(kBlock) void k_pack_members(int nc, const int* __re
 
 // ---------------- host helpers ----------------
 
-void match_round_flags(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal,
-                       const int* prevChanged, int* changed) {
+void match_round_flags(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* w, int* mate,
+                       int* proposal, const int* prevChanged, int* changed) {
+    (void)nnz;  // one thread per row measured faster here than a group of lanes per row
     hipLaunchKernelGGL(k_propose, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, w, mate, proposal, prevChanged);
     check(hipGetLastError(), "propose launch");
     hipLaunchKernelGGL(k_accept, grid1(n), dim3(kBlock), 0, c.stream, n, proposal, mate, prevChanged, changed);
@@ -969,8 +1000,16 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
         int* cnt = c.wst<int>(kCount, size_t(n) + 1);
         int* scol = c.wst<int>(kKeys, size_t(nnz) + 1);
         double* sw = c.wst<double>(kVals, size_t(nnz) + 1);
-        hipLaunchKernelGGL(k_strength_sym, grid1(int64_t(n) + 1), dim3(kBlock), 0, c.stream, n, rp, ci, v, rowmax,
-                           theta, scol, sw, cnt);
+        const int lanes = lanes_per_row(n, nnz);
+#define TF_STRENGTH(G)                                                                                               \
+    case G:                                                                                                          \
+        hipLaunchKernelGGL(k_strength_sym<G>, grid_rows(int64_t(n) + 1, G), dim3(kBlock), 0, c.stream, n, rp, ci, v, \
+                           rowmax, theta, scol, sw, cnt);                                                            \
+        break;
+        switch (lanes) {
+            TF_STRENGTH(4) TF_STRENGTH(8) TF_STRENGTH(16) TF_STRENGTH(32)
+        }
+#undef TF_STRENGTH
         check(hipGetLastError(), "strength launch");
         DevCsr g;
         g.n = n;
@@ -979,7 +1018,15 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
         g.nnz = c.read_int(g.rp + n);
         g.ci = dev_alloc<int>(c, size_t(g.nnz));
         g.v = dev_alloc<double>(c, size_t(g.nnz));
-        hipLaunchKernelGGL(k_compact_rows, grid1(n), dim3(kBlock), 0, c.stream, n, rp, scol, sw, g.rp, g.ci, g.v);
+#define TF_COMPACT(G)                                                                                            \
+    case G:                                                                                                      \
+        hipLaunchKernelGGL(k_compact_rows<G>, grid_rows(n, G), dim3(kBlock), 0, c.stream, n, rp, scol, sw, g.rp, g.ci, \
+                           g.v);                                                                                 \
+        break;
+        switch (lanes) {
+            TF_COMPACT(4) TF_COMPACT(8) TF_COMPACT(16) TF_COMPACT(32)
+        }
+#undef TF_COMPACT
         check(hipGetLastError(), "strength compaction launch");
         return g;
     }
@@ -1012,17 +1059,18 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
 }
 
 void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal) {
-    match_round_flags(c, n, rp, ci, w, mate, proposal, nullptr, nullptr);
+    match_round_flags(c, n, n > 0 ? c.read_int(rp + n) : 0, rp, ci, w, mate, proposal, nullptr, nullptr);
 }
 
-int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map) {
+int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map, int64_t nnz) {
+    if (nnz < 0) nnz = n > 0 ? c.read_int(rp + n) : 0;
     int* mate = c.wst<int>(kMate, size_t(n));
     int* proposal = c.wst<int>(kProposal, size_t(n));
     fill_i32(c, n, -1, mate);
     int* changed = c.wst<int>(kChanged, 8);
     fill_i32(c, 8, 0, changed);
     for (int r = 0; r < rounds; ++r)
-        match_round_flags(c, n, rp, ci, w, mate, proposal, r ? changed + r - 1 : nullptr, changed + r);
+        match_round_flags(c, n, nnz, rp, ci, w, mate, proposal, r ? changed + r - 1 : nullptr, changed + r);
     int* flags = c.wst<int>(kFlags, size_t(n) + 1);
     hipLaunchKernelGGL(k_roots, grid1(int64_t(n) + 1), dim3(kBlock), 0, c.stream, n, mate, flags);
     check(hipGetLastError(), "roots launch");
@@ -1052,12 +1100,12 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb) {
     t.n = n;
     t.map = dev_alloc<int>(c, size_t(n));
     try {
-        const int first = pair(c, n, g.rp, g.ci, g.v, rounds, t.map);
+        const int first = pair(c, n, g.rp, g.ci, g.v, rounds, t.map, g.nnz);
         DevCsr contracted = contract_graph(c, n, g.rp, g.ci, g.v, t.map, first);
         int* second = c.wst<int>(kSecond, size_t(first));
         int coarse = 0;
         try {
-            coarse = pair(c, first, contracted.rp, contracted.ci, contracted.v, rounds, second);
+            coarse = pair(c, first, contracted.rp, contracted.ci, contracted.v, rounds, second, contracted.nnz);
         }
         catch (...) {
             free_csr(c, contracted);

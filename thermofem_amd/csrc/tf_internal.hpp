@@ -292,7 +292,8 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
                       int symmetric = -1, bool prevalidated = false, int* symmetricOut = nullptr);
 double* rowmax_workspace(tf_ctx& c, int n);
 void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal);
-int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map);
+// nnz < 0: read from rp[n].
+int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map, int64_t nnz = -1);
 DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc);
 DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb);
 DevCsr galerkin(tf_ctx& c, const DevCsr& a, const int* map, int nc, const int* offsets, const int* members);

@@ -98,13 +98,24 @@ __global__ void k_dense_fill(int n, const int* __restrict__ rp, const int* __res
 // workgroup: the pivot is the first maximal |m[i][k]| over i >= k, rejected
 // when non-finite or <= 64*eps*max|A|; rows are swapped whole; multipliers
 // are stored below the diagonal.
-__global__ __launch_bounds__(kLuThreads) void k_lu_factor(int n, double* m, int* piv, int* bad) {
+// Orders whose matrix fits kLuLds doubles of LDS work there (the steps are
+// a chain of dependent loads; LDS makes each one short) with the same
+// operations, then write back.
+constexpr int kLuLds = 16384;  // 128 KB
+__global__ __launch_bounds__(kLuThreads) void k_lu_factor(int n, double* gm, int* piv, int* bad) {
     __shared__ double sv[kLuThreads / kWave];
     __shared__ int si[kLuThreads / kWave];
     __shared__ double scale;
     __shared__ int pivot, failed;
+    __shared__ double lds[kLuLds];
     const int tid = int(threadIdx.x);
     const int lane = tid & (kWave - 1), w = tid / kWave;
+    const bool inLds = n * n <= kLuLds;
+    double* m = inLds ? lds : gm;
+    if (inLds) {
+        for (int i = tid; i < n * n; i += kLuThreads) lds[i] = gm[i];
+        __syncthreads();
+    }
     double mx = 0;
     for (int i = tid; i < n * n; i += kLuThreads) mx = fmax(mx, fabs(m[i]));
 #pragma unroll
@@ -162,6 +173,8 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_factor(int n, double* m, int*
         __syncthreads();
         if (failed) {
             if (tid == 0) *bad = 1;
+            if (inLds)
+                for (int i = tid; i < n * n; i += kLuThreads) gm[i] = lds[i];
             return;
         }
         const int p = pivot;
@@ -184,8 +197,10 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_factor(int n, double* m, int*
         __syncthreads();
     }
     int nonfinite = 0;
-    for (int i = tid; i < n * n; i += kLuThreads)
+    for (int i = tid; i < n * n; i += kLuThreads) {
         if (!isfinite(m[i])) nonfinite = 1;
+        if (inLds) gm[i] = lds[i];
+    }
     if (nonfinite) atomicOr(bad, 1);
 }
 
@@ -193,10 +208,19 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_factor(int n, double* m, int*
 // workgroup): V = P I (the row swaps of the factorisation in order), then
 // for each k the unit-lower and upper updates run in parallel over rows and
 // columns.  Per column this is the operation order of the triangular solve.
-__global__ __launch_bounds__(kLuThreads) void k_lu_inverse(int n, const double* __restrict__ lu,
-                                                           const int* __restrict__ piv, double* inv) {
+__global__ __launch_bounds__(kLuThreads) void k_lu_inverse(int n, const double* __restrict__ glu,
+                                                           const int* __restrict__ piv, double* ginv) {
+    __shared__ double lds[kLuLds];
     const int tid = int(threadIdx.x);
     const int nn = n * n;
+    // both factors and the inverse in LDS when they fit
+    const bool inLds = 2 * nn <= kLuLds;
+    double* inv = inLds ? lds : ginv;
+    const double* lu = glu;
+    if (inLds) {
+        for (int i = tid; i < nn; i += kLuThreads) lds[nn + i] = glu[i];
+        lu = lds + nn;
+    }
     for (int idx = tid; idx < nn; idx += kLuThreads) inv[idx] = (idx / n == idx % n) ? 1.0 : 0.0;
     __syncthreads();
     for (int k = 0; k < n; ++k) {
@@ -228,6 +252,8 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_inverse(int n, const double* 
         }
         __syncthreads();
     }
+    if (inLds)
+        for (int i = tid; i < nn; i += kLuThreads) ginv[i] = lds[i];
 }
 
 // x = inv * b in FP64; b is staged in LDS first so rhs and x may alias.
