@@ -49,6 +49,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch each iteration's kernels without hipGraph")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the level-0 SpMV launches and a copy kernel of known size (for rocprofv3 --pmc)")
+    ap.add_argument("--workload", choices=["heat3d", "convdiff"], default="heat3d",
+                    help="convdiff: 96^3 upwind convection-diffusion, BiCGStab with V/W/F cycles, FP64 and FP32 "
+                         "levels, cell Peclet 10/30/100 (one JSON line of its own)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver runs on a single "
@@ -60,6 +63,10 @@ def main():
     from thermofem_amd._lib import check, lib
     from thermofem_amd.device import Context, DeviceArray, DeviceCsr
     from thermofem_amd.solver import AmgOptions, Cg, KrylovOptions, SolveStatus
+
+    if args.workload == "convdiff":
+        convdiff(args)
+        return
 
     nx = args.nx
     log(f"generating heat3d log-normal {nx}^3")
@@ -215,6 +222,70 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def convdiff(args):
+    """96^3 convection-diffusion (upwind, 8 Fourier velocity modes) at cell
+    Peclet 10, 30 and 100: BiCGStab + AMG (V(1,1) Jacobi omega 2/3, coarsest
+    <= 256 rows by dense LU) to a 1e-8 FP64 relative residual with V, W and F
+    cycles, levels in FP64 and in FP32.  Median seconds of setup+solve over
+    --steps runs after --warmup, one JSON line."""
+    import numpy as np
+    from thermofem_amd import problems as P
+    from thermofem_amd.device import DeviceArray, DeviceCsr
+    from thermofem_amd.solver import AmgOptions, BiCgStab, Cycle, KrylovOptions, SolveStatus
+
+    nx = 96 if args.nx == 128 else args.nx
+    rows = []
+    for pe in (10.0, 30.0, 100.0):
+        log(f"generating convection-diffusion {nx}^3 Pe {pe:g}")
+        (rp, ci, v), b = P.convection_diffusion3d(nx, pe)
+        n, nnz = len(rp) - 1, len(ci)
+        A = DeviceCsr(rp, ci, v)
+        db = DeviceArray.from_host(b)
+        dx = DeviceArray(n, np.float64)
+        zeros = np.zeros(n)
+        for cyc in (Cycle.V, Cycle.W, Cycle.F):
+            for prec in (64, 32):
+                s = BiCgStab(KrylovOptions(relative_tolerance=1e-8, use_graph=not args.no_graph),
+                             AmgOptions(coarse_rows=256, jacobi_weight=2.0 / 3.0, cycle=cyc, level_precision=prec))
+                times, setups, solves, res = [], [], [], None
+                for k in range(args.warmup + args.steps):
+                    dx.upload(zeros)
+                    t0 = time.perf_counter()
+                    s.setup_device(n, nnz, *A.args())
+                    res = s.apply_device(db, dx)
+                    t1 = time.perf_counter()
+                    if k >= args.warmup:
+                        st, sa = s.timings()
+                        times.append(t1 - t0)
+                        setups.append(st)
+                        solves.append(sa)
+                x = dx.get()
+                true_rel = float(np.linalg.norm(b - csr_matvec(rp, ci, v, x)) / np.linalg.norm(b))
+                row = {"peclet": pe, "cycle": cyc.name, "levels_fp": prec, "s": round(statistics.median(times), 6),
+                       "setup_s": round(statistics.median(setups), 6), "solve_s": round(statistics.median(solves), 6),
+                       "iterations": res.iterations, "status": res.status.name, "true_rel_residual": true_rel}
+                log(json.dumps(row))
+                rows.append(row)
+                if res.status != SolveStatus.Converged:
+                    log(f"warning: Pe {pe} {cyc.name} FP{prec} did not converge")
+                del s
+    best = min((r for r in rows if r["status"] == "Converged"), key=lambda r: r["s"], default=None)
+    print(json.dumps({"metric": f"BiCGStab+AMG setup+solve s to 1e-8 at {nx}^3 convection-diffusion",
+                      "unit": "s", "higher_is_better": False, "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "dtype": "f64 / f32 levels", "data": "synthetic",
+                      "config": {"workload": f"upwind convection-diffusion {nx}^3, 8 Fourier velocity modes, "
+                                             "cell Peclet 10/30/100, V(1,1) omega 2/3, coarsest <= 256",
+                                 "n": nx ** 3},
+                      "best": best, "runs": rows}), flush=True)
+
+
+def csr_matvec(rp, ci, v, x):
+    """y = A x on the host in FP64 (for the recomputed residual)."""
+    import numpy as np
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    return np.bincount(rows, weights=v * x[ci], minlength=len(rp) - 1)
 
 
 def make_sell(ctx, n, nnz, A, check, lib):

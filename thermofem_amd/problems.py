@@ -113,6 +113,73 @@ def heat3d_lognormal(nx=128, seed=1, sigma=1.0):
     return _compress(cols, vals, mask), b
 
 
+def fourier_velocity(nx, seed, modes=8, kmax=4):
+    """Velocity at the nodes ((i+1)h, (j+1)h, (k+1)h), h = 1/(nx+1), of an nx^3
+    interior grid: u(x) = sum_m a_m sin(2 pi (k_m . x) + phi_m) over `modes`
+    modes, integer wavenumbers k_m in {1..kmax}^3, amplitudes a_m ~ N(0,1)^3,
+    phases ~ U[0, 2 pi).  Draw order: the 3*modes wavenumbers, then the
+    3*modes amplitudes (one normal stream), then the modes phases.
+    Returns (n, 3), node index i + nx*(j + nx*k).
+    """
+    kv = (np.floor(kmax * uniform(seed, 3 * modes)).astype(np.int64) + 1).reshape(modes, 3)
+    amp = normal(seed + 1, 3 * modes).reshape(modes, 3)
+    phase = 2.0 * np.pi * uniform(seed + 2, modes)
+    h = 1.0 / (nx + 1)
+    t = (np.arange(nx) + 1) * h
+    z, y, x = np.meshgrid(t, t, t, indexing="ij")
+    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    u = np.zeros((nx ** 3, 3))
+    for m in range(modes):
+        s = np.sin(2.0 * np.pi * (pts @ kv[m].astype(np.float64)) + phase[m])
+        u += s[:, None] * amp[m][None, :]
+    return u
+
+
+def convection_diffusion3d(nx=96, peclet=10.0, seed=4, rhs_seed=5):
+    """Steady convection-diffusion -lap(c) + u.grad(c) = f on the unit cube,
+    7-point finite differences on an nx^3 interior node grid (kappa = 1,
+    homogeneous Dirichlet: neighbours outside the grid are dropped), first-
+    order upwind advection.  The velocity is fourier_velocity(nx, seed)
+    scaled so that the largest cell Peclet number max_d |u_d| h / kappa
+    equals `peclet`.  Rows are scaled by h^2: with q_d = h u_d the row holds
+    6 + sum_d |q_d| on the diagonal, -1 - |q_d| at the upwind neighbour of
+    axis d and -1 at the downwind one.  Non-symmetric M-matrix with a
+    symmetric pattern; b ~ U[0,1) from rhs_seed.  Returns ((rp, ci, v), b).
+    """
+    n = nx ** 3
+    u = fourier_velocity(nx, seed)
+    q = u * (peclet / np.max(np.abs(u)))
+    idx = np.arange(n)
+    ix = idx % nx
+    iy = (idx // nx) % nx
+    iz = idx // (nx * nx)
+    s = [nx * nx, nx, 1]
+    coords = [iz, iy, ix]
+    comp = [2, 1, 0]  # velocity component along each stride axis (z, y, x)
+    cols = np.empty((n, 7), np.int64)
+    vals = np.zeros((n, 7))
+    mask = np.zeros((n, 7), bool)
+    diag = np.full(n, 6.0)
+    order = [(0, -1), (1, -1), (2, -1), None, (2, +1), (1, +1), (0, +1)]
+    for slot, o in enumerate(order):
+        if o is None:
+            cols[:, slot] = idx
+            mask[:, slot] = True
+            continue
+        axis, sign = o
+        qd = q[:, comp[axis]]
+        valid = (coords[axis] > 0) if sign < 0 else (coords[axis] < nx - 1)
+        # upwind side: -axis neighbour when q_d > 0, +axis neighbour when q_d < 0
+        upwind = (qd > 0) if sign < 0 else (qd < 0)
+        cols[:, slot] = np.where(valid, idx + sign * s[axis], idx)
+        vals[:, slot] = np.where(upwind, -1.0 - np.abs(qd), -1.0)
+        mask[:, slot] = valid
+    for d in range(3):
+        diag += np.abs(q[:, d])
+    vals[:, 3] = diag
+    return _compress(cols, vals, mask), uniform(rhs_seed, n)
+
+
 def spmv_bytes(n, nnz, value_size):
     """Algorithmic bytes of one CSR SpMV: values+columns, offsets, x once, y."""
     return nnz * (value_size + 4) + 4 * (n + 1) + 2 * n * value_size
