@@ -49,9 +49,10 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch each iteration's kernels without hipGraph")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the level-0 SpMV launches and a copy kernel of known size (for rocprofv3 --pmc)")
-    ap.add_argument("--workload", choices=["heat3d", "convdiff"], default="heat3d",
+    ap.add_argument("--workload", choices=["heat3d", "convdiff", "fem"], default="heat3d",
                     help="convdiff: 96^3 upwind convection-diffusion, BiCGStab with V/W/F cycles, FP64 and FP32 "
-                         "levels, cell Peclet 10/30/100 (one JSON line of its own)")
+                         "levels, cell Peclet 10/30/100; fem: P1 tets on 64^3 cells, 64 right-hand sides in one "
+                         "block CG vs 64 single solves, and the k=64 SpMM (each one JSON line of its own)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver runs on a single "
@@ -66,6 +67,9 @@ def main():
 
     if args.workload == "convdiff":
         convdiff(args)
+        return
+    if args.workload == "fem":
+        fem_block(args)
         return
 
     nx = args.nx
@@ -279,6 +283,110 @@ def convdiff(args):
                                              "cell Peclet 10/30/100, V(1,1) omega 2/3, coarsest <= 256",
                                  "n": nx ** 3},
                       "best": best, "runs": rows}), flush=True)
+
+
+def fem_block(args):
+    """P1 tetrahedral heat operator M/dt + K (64^3 cells x 6 Kuhn tets,
+    274,625 nodes, 4,018,753 nonzeros, dt 1e-3, log-normal k_tet) with 64
+    right-hand sides: AMG (V(2,2) Jacobi omega 2/3, coarsest <= 256 by dense
+    LU) + CG to 1e-8 on all 64 at once (row-major n x 64 blocks, SpMM) against
+    64 single-vector solves with the same hierarchy; plus the k = 64 FP64
+    SpMM alone (algorithmic bytes nnz*12 + 4(n+1) + 2*64*n*8)."""
+    import numpy as np
+    from thermofem_amd import problems as P
+    from thermofem_amd._lib import check, lib
+    from thermofem_amd.device import Context, DeviceArray, DeviceCsr
+    from thermofem_amd.solver import AmgOptions, Cg, KrylovOptions, SolveStatus
+
+    nc, k = 64, 64
+    log(f"generating P1 tet heat operator {nc}^3 cells")
+    rp, ci, v = P.fem_p1_heat(nc)
+    n, nnz = len(rp) - 1, len(ci)
+    A = DeviceCsr(rp, ci, v)
+    B = P.block_rhs(n, k)
+    dB = DeviceArray.from_host(B.ravel())
+    dX = DeviceArray(n * k, np.float64)
+    zeros = np.zeros(n * k)
+    out = {"metric": f"setup + CG solve s of {k} right-hand sides to 1e-8, P1 tets {nc}^3 cells",
+           "unit": "s", "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "data": "synthetic", "config": {"workload": f"P1 tet M/dt+K {nc}^3 cells x 6 Kuhn tets, {k} RHS",
+                                           "n": n, "nnz": nnz, "k": k}}
+    for prec in (32, 64):
+        s = Cg(KrylovOptions(relative_tolerance=1e-8), AmgOptions(coarse_rows=256, pre_sweeps=2, post_sweeps=2,
+                                                                  jacobi_weight=2.0 / 3.0, level_precision=prec))
+        times, setups, solves, res = [], [], [], None
+        for it in range(args.warmup + args.steps):
+            dX.upload(zeros)
+            t0 = time.perf_counter()
+            s.setup_device(n, nnz, *A.args())
+            t1 = time.perf_counter()
+            res = s.apply_block_device(k, dB, dX)
+            t2 = time.perf_counter()
+            if it >= args.warmup:
+                times.append(t2 - t0)
+                setups.append(t1 - t0)
+                solves.append(t2 - t1)
+        X = dX.get().reshape(n, k)
+        worst = max(float(np.linalg.norm(B[:, j] - csr_matvec(rp, ci, v, X[:, j])) / np.linalg.norm(B[:, j]))
+                    for j in range(k))
+        its = [r.iterations for r in res]
+        ok = all(r.status == SolveStatus.Converged for r in res)
+        block = {"s": round(statistics.median(times), 6), "setup_s": round(statistics.median(setups), 6),
+                 "solve_s": round(statistics.median(solves), 6), "iterations_min_max": [min(its), max(its)],
+                 "all_converged": ok, "worst_true_rel_residual": worst}
+        log(f"fem fp{prec} block: {block}")
+        # the same 64 systems one at a time (same solver and hierarchy)
+        x1 = DeviceArray(n, np.float64)
+        single_solve, single_its = 0.0, []
+        for j in range(k):
+            b1 = DeviceArray.from_host(np.ascontiguousarray(B[:, j]))
+            x1.upload(np.zeros(n))
+            t0 = time.perf_counter()
+            r1 = s.apply_device(b1, x1)
+            single_solve += time.perf_counter() - t0
+            single_its.append(r1.iterations)
+        single = {"solve_s": round(single_solve, 6), "s": round(statistics.median(setups) + single_solve, 6),
+                  "iterations_min_max": [min(single_its), max(single_its)]}
+        log(f"fem fp{prec} 64 single solves: {single}")
+        out[f"levels_fp{prec}"] = {"block": block, "single_solves": single,
+                                   "block_speedup": round(single["s"] / block["s"], 3)}
+        del s
+    # k = 64 FP64 SpMM with HIP events on the library stream
+    ctx = Context(0)
+    Y = DeviceArray(n * k, np.float64)
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    check(lib().tf_event_create(C.byref(ev0)))
+    check(lib().tf_event_create(C.byref(ev1)))
+    L = lib()
+
+    v32 = DeviceArray.from_host(v.astype(np.float32))
+    B32 = DeviceArray.from_host(B.ravel().astype(np.float32))
+    Y32 = DeviceArray(n * k, np.float32)
+
+    def time_spmm(launch):
+        for _ in range(3):
+            check(launch())
+        reps = args.spmv_reps
+        check(L.tf_event_record(ctx.h, ev0))
+        for _ in range(reps):
+            check(launch())
+        check(L.tf_event_record(ctx.h, ev1))
+        ms = C.c_float()
+        check(L.tf_event_elapsed_ms(ev0, ev1, C.byref(ms)))
+        return ms.value / reps * 1e-3
+
+    for name, s_, launch in (
+            ("spmm_k64_f64", 8, lambda: L.tf_spmm_csr_f64(ctx.h, n, nnz, A.rp.ptr, A.ci.ptr, A.v.ptr, k, dB.ptr,
+                                                          Y.ptr)),
+            ("spmm_k64_f32", 4, lambda: L.tf_spmm_csr_f32(ctx.h, n, nnz, A.rp.ptr, A.ci.ptr, v32.ptr, k, B32.ptr,
+                                                          Y32.ptr))):
+        t = time_spmm(launch)
+        nbytes = nnz * (s_ + 4) + 4 * (n + 1) + 2 * k * n * s_
+        out[name] = {"us": round(t * 1e6, 2), "algorithmic_bytes": nbytes, "GBs": round(nbytes / t / 1e9, 1),
+                     "frac_of_8TBs": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"{name} {t * 1e6:.1f} us {nbytes / t / 1e9:.0f} GB/s")
+    out["value"] = out["levels_fp32"]["block"]["s"]
+    print(json.dumps(out), flush=True)
 
 
 def csr_matvec(rp, ci, v, x):

@@ -80,7 +80,8 @@ int tf_residual_csr_f64(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const in
                         const double* b, const double* x, double* r);
 int tf_residual_csr_f32(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const float* v,
                         const float* b, const float* x, float* r);
-/* Y = A X with X, Y row-major n x k (DenseMatrix.cuh:9-19 layout). */
+/* Y = A X with X, Y row-major n x k (DenseMatrix.cuh:9-19 layout); X != Y.
+ * Column j of Y equals tf_spmv_csr_* of column j bit for bit. */
 int tf_spmm_csr_f64(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v, int k,
                     const double* X, double* Y);
 int tf_spmm_csr_f32(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const float* v, int k,
@@ -213,6 +214,13 @@ int tf_amg_destroy(tf_amg* amg);
 int tf_amg_setup(tf_amg* amg, int n, int64_t nnz, const int* rp, const int* ci, const double* v, int host_memory);
 /* apply (Amg.cuh:149-161): x = cycle(rhs), rhs and x may alias. */
 int tf_amg_apply(tf_amg* amg, const double* rhs, double* x, int host_memory);
+/* k right-hand sides at once: R, Z distinct row-major n x k device blocks,
+ * Z = cycle(R) column by column.  Replaces k calls of tf_amg_apply (Amg.cuh:
+ * 149-161; the reference has no multi-vector cycle, its dense type is
+ * DenseMatrix.cuh:9-19); column j equals tf_amg_apply of column j bit for bit. */
+int tf_amg_apply_block(tf_amg* amg, int k, const double* R, double* Z);
+/* Per-level visit counts of the last block cycle (level_count entries). */
+int tf_amg_block_visits(tf_amg* amg, int64_t* visits);
 int tf_amg_level_count(tf_amg* amg);
 int tf_amg_level_info(tf_amg* amg, int level, int64_t* rows, int64_t* nnz, int64_t* histogram);
 /* Host copies of one level's native FP64 matrix and transfer (for parity checks). */
@@ -237,6 +245,12 @@ int tf_solver_apply(tf_solver* solver, const double* b, double* x, int host_memo
  * call preconditioner->setup (Cg.cuh:60-67, BiCgStab.cuh:70-77). */
 int tf_solver_create_with_amg(int kind, const tf_krylov_options* options, tf_amg* amg, tf_solver** solver);
 tf_amg* tf_solver_amg(tf_solver* solver);
+/* CG on k right-hand sides at once (Cg.cuh:123-189 per column, each column
+ * with its own scalars and stopping test): B, X distinct row-major n x k
+ * device blocks (X holds the initial guesses), results[k] host array.  The
+ * preconditioner and the products run on the whole block (SpMM,
+ * Backend.cu:256-281).  BiCGStab solvers return TF_E_INVALID. */
+int tf_solver_apply_block(tf_solver* solver, int k, const double* B, double* X, tf_solve_result* results);
 /* Seconds of the last setup / apply measured with HIP events on the solver stream. */
 int tf_solver_timings(tf_solver* solver, double* setup_seconds, double* apply_seconds);
 

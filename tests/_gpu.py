@@ -109,6 +109,20 @@ def spmv(a, x, dtype=np.float64):
     return y.get()
 
 
+def spmm(a, X, dtype=np.float64):
+    """Y = A X for a row-major n x k block."""
+    n = len(a[0]) - 1
+    X = np.ascontiguousarray(X, dtype)
+    k = X.shape[1]
+    rp, ci, v = up(a[0], np.int32), up(a[1], np.int32), up(a[2], dtype)
+    dx = up(X.ravel(), dtype)
+    y = DeviceArray(n * k, dtype)
+    fn = lib().tf_spmm_csr_f64 if dtype == np.float64 else lib().tf_spmm_csr_f32
+    check(fn(ctx().h, n, len(a[1]), rp.ptr, ci.ptr, v.ptr, k, dx.ptr, y.ptr))
+    ctx().sync()
+    return y.get().reshape(n, k)
+
+
 def lu_factor(a):
     n = len(a[0]) - 1
     rp, ci, v = up(a[0], np.int32), up(a[1], np.int32), up(a[2], np.float64)

@@ -119,6 +119,8 @@ _SIGS = {
     "tf_amg_destroy": (i32, [vp]),
     "tf_amg_setup": (i32, [vp, i32, i64, vp, vp, vp, i32]),
     "tf_amg_apply": (i32, [vp, vp, vp, i32]),
+    "tf_amg_apply_block": (i32, [vp, i32, vp, vp]),
+    "tf_amg_block_visits": (i32, [vp, C.POINTER(C.c_int64)]),
     "tf_amg_level_count": (i32, [vp]),
     "tf_amg_level_info": (i32, [vp, i32, I64P, I64P, I64P]),
     "tf_amg_level_matrix": (i32, [vp, i32, vp, vp, vp]),
@@ -130,6 +132,7 @@ _SIGS = {
     "tf_solver_destroy": (i32, [vp]),
     "tf_solver_setup": (i32, [vp, i32, i64, vp, vp, vp, i32]),
     "tf_solver_apply": (i32, [vp, vp, vp, i32, C.POINTER(SolveResultC)]),
+    "tf_solver_apply_block": (i32, [vp, i32, vp, vp, C.POINTER(SolveResultC)]),
     "tf_solver_create_with_amg": (i32, [i32, C.POINTER(KrylovOptions), vp, C.POINTER(vp)]),
     "tf_solver_amg": (vp, [vp]),
     "tf_solver_timings": (i32, [vp, F64P, F64P]),

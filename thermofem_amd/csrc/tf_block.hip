@@ -1,0 +1,701 @@
+// Multi-right-hand-side (block) operations on gfx950: SpMM with the fused
+// epilogues of the single-vector kernels, the AMG cycle over row-major
+// blocks, and CG with one control block per right-hand side.  A block is
+// row-major n x k: row i holds entry i of each of the k right-hand sides
+// (the DenseMatrix layout, DenseMatrix.cuh:9-19; the SpMM of
+// linalg::multiply(CsrMatrix, DenseMatrix), Operations.cuh:425-442 and
+// Backend.cu:256-281).
+//
+// Launch shape: G lanes per row (G = k rounded up to a power of two, at most
+// 64; lane = column inside a chunk of G columns, blockIdx.y = chunk) and
+// kBlock / G rows per block step.  At G = 64 a row is one wave: its column
+// indices and values are wave-uniform scalar loads and each gathered block
+// row is one coalesced 64-lane load.  Row-parallel launches are XCD-major:
+// XCD x takes the x-th contiguous slab of rows and its blocks walk the slab
+// together, so the block rows that neighbouring rows gather stay in that
+// XCD's L2.
+//
+// Every column is computed with the arithmetic of the single-vector kernels
+// (row sums of rounded products added in stored order, unfused epilogues,
+// sequential restriction sums, the coarse inverse's lane partials and wave
+// reduction), so column j of a block cycle equals the single-vector cycle
+// of column j bit for bit.  Krylov dot products are per column but reduced
+// in a different order than the single-vector solver's.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "tf_device.hpp"
+#include "tf_engine.hpp"
+#include "tf_krylov.hpp"
+
+namespace tf {
+
+namespace {
+
+// Most blocks of a row-parallel launch with a fused column reduction
+// (= partials per column): 8 blocks of 4 waves per CU fill the chip.
+constexpr int kBParts = 2048;
+
+// Row groups of this block: XCD-major slabs, strided by the blocks per XCD.
+__device__ __forceinline__ void block_groups(int groups, int& first, int& step, int& end) {
+    const int per = int(gridDim.x) / kXcds;
+    const int x = int(blockIdx.x) % kXcds, local = int(blockIdx.x) / kXcds;
+    const int slab = (groups + kXcds - 1) / kXcds;
+    const int begin = min(groups, x * slab);
+    first = begin + local;
+    step = per;
+    end = min(groups, begin + slab);
+}
+
+int lanes_for(int k) {
+    int g = 1;
+    while (g < k && g < kWave) g <<= 1;
+    return g;
+}
+
+// Blocks of a row-parallel launch over `groups` row groups: one group per
+// block without a reduction, at most kBParts blocks with one.
+int row_grid(int groups, bool reduce = true) {
+    int per = std::max(1, (groups + kXcds - 1) / kXcds);
+    if (reduce) per = std::min(per, kBParts / kXcds);
+    return per * kXcds;
+}
+
+// Row sum of row [s, e) against column col of X (k columns): products
+// rounded and added one by one in stored order; chunks of 8 issue their
+// loads first, slots past the row's end re-read its last entry and are not
+// added.
+template <class T>
+__device__ __forceinline__ T brow_sum(int s, int e, const int* __restrict__ ci, const T* __restrict__ v,
+                                      const T* __restrict__ X, int k, int col) {
+#pragma clang fp contract(off)
+    T acc = 0;
+    for (int j0 = s; j0 < e; j0 += 8) {
+        T pv[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = min(j0 + u, e - 1);
+            pv[u] = v[j];
+            xv[u] = X[size_t(ci[j]) * size_t(k) + size_t(col)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const T p = pv[u] * xv[u];
+            const T sum = acc + p;
+            acc = j0 + u < e ? sum : acc;
+        }
+    }
+    return acc;
+}
+
+// Reductions fused into a row-parallel launch (one partial per column and
+// block, written to partials[col * kBParts + block]).
+enum BDot {
+    kBNone = 0,
+    kBDotX = 1,    // sum X[r] * y[r]   (CG curvature p.Ap)
+    kBSquares = 2, // sum y[r]^2         (residual norms)
+    kBTrue = 3,    // residual + squares only in columns whose control block asks for a refresh
+};
+
+__device__ __forceinline__ void block_partials(double contrib, int G, int col, int k, double* red,
+                                               double* __restrict__ partials) {
+    red[threadIdx.x] = contrib;
+    __syncthreads();
+    if (int(threadIdx.x) < G && col < k) {
+        double s = 0;
+        for (int q = 0; q < kBlock / G; ++q) s += red[q * G + int(threadIdx.x)];
+        partials[size_t(col) * kBParts + blockIdx.x] = s;
+    }
+}
+
+// Y = epilogue(A X) over a block; E as the single-vector kernels
+// (kSpmv, kResidual, kJacobi), D a fused column reduction.
+template <class T, int G, int E, int D>
+__global__ __launch_bounds__(kBlock) void k_bspmm(int n, int k, const int* __restrict__ rp,
+                                                  const int* __restrict__ ci, const T* __restrict__ v,
+                                                  const T* __restrict__ X, T* __restrict__ Y,
+                                                  const T* __restrict__ B, const T* __restrict__ inv, T w,
+                                                  double* __restrict__ partials, const KCtl* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    __shared__ double red[kBlock];
+    constexpr int R = kBlock / G;
+    const int sub = int(threadIdx.x) / G, lane = int(threadIdx.x) % G;
+    const int col = int(blockIdx.y) * G + lane;
+    const bool live = col < k;
+    const int cc = live ? col : k - 1;
+    bool active = live;
+    if constexpr (D == kBTrue) active = live && !stopped(ctl + cc) && ctl[cc].check && !ctl[cc].early;
+    double contrib = 0;
+    int first, step, end;
+    block_groups((n + R - 1) / R, first, step, end);
+    for (int g = first; g < end; g += step) {
+        int row = g * R + sub;
+        if constexpr (G == kWave) row = __builtin_amdgcn_readfirstlane(row);
+        if (row >= n) continue;
+        int s = rp[row], e = rp[row + 1];
+        if constexpr (G == kWave) {
+            s = __builtin_amdgcn_readfirstlane(s);
+            e = __builtin_amdgcn_readfirstlane(e);
+        }
+        const T acc = brow_sum<T>(s, e, ci, v, X, k, cc);
+        const size_t at = size_t(row) * size_t(k) + size_t(cc);
+        T y;
+        if constexpr (E == kSpmv) {
+            y = acc;
+        }
+        else if constexpr (E == kResidual) {
+            y = B[at] - acc;
+        }
+        else {
+            const T wd = w * inv[row];
+            const T d = B[at] - acc;
+            const T upd = wd * d;
+            y = X[at] + upd;
+        }
+        if (active) {
+            Y[at] = y;
+            if constexpr (D == kBDotX) contrib += double(X[at]) * double(y);
+            if constexpr (D == kBSquares || D == kBTrue) contrib += double(y) * double(y);
+        }
+    }
+    if constexpr (D != kBNone) block_partials(contrib, G, col, k, red, partials);
+}
+
+// Column dot products sum X[r][j] * Y[r][j].
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_bdot(int n, int k, const double* __restrict__ X,
+                                                 const double* __restrict__ Y, double* __restrict__ partials) {
+    __shared__ double red[kBlock];
+    constexpr int R = kBlock / G;
+    const int sub = int(threadIdx.x) / G, lane = int(threadIdx.x) % G;
+    const int col = int(blockIdx.y) * G + lane;
+    double s = 0;
+    int first, step, end;
+    block_groups((n + R - 1) / R, first, step, end);
+    if (col < k)
+        for (int g = first; g < end; g += step) {
+            const int row = g * R + sub;
+            if (row < n) {
+                const size_t at = size_t(row) * size_t(k) + size_t(col);
+                s += X[at] * Y[at];
+            }
+        }
+    block_partials(s, G, col, k, red, partials);
+}
+
+// Finaliser: per column, the partials added in block order, then the
+// single-vector recurrence on that column's control block (mode >= 0), or
+// the plain sums written to out (mode < 0).
+__global__ __launch_bounds__(1024) void k_bfinal(int k, const double* __restrict__ partials, int nparts, KCtl* ctl,
+                                                 int mode, double* __restrict__ out) {
+    const int lane = int(threadIdx.x) & (kWave - 1), w = int(threadIdx.x) / kWave;
+    for (int col = w; col < k; col += 1024 / kWave) {
+        if (mode >= 0) {
+            const KCtl* c = ctl + col;
+            if (stopped(c)) continue;
+            if (mode == kFinTrue && (!c->check || c->early)) continue;
+        }
+        double s = 0;
+        for (int i = lane; i < nparts; i += kWave) s += partials[size_t(col) * kBParts + i];
+        s = wave_sum(s);
+        if (lane == 0) {
+            if (mode >= 0)
+                finalize(mode, s, ctl + col);
+            else
+                out[col] = s;
+        }
+    }
+}
+
+// CG direction per column: p = z (fresh) or p = z + beta p.
+__global__ void k_bcg_p(size_t nk, int k, const double* __restrict__ z, double* __restrict__ p,
+                        const KCtl* __restrict__ ctl) {
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
+        const KCtl* c = ctl + int(i % size_t(k));
+        if (stopped(c)) continue;
+        p[i] = c->fresh ? z[i] : z[i] + c->beta * p[i];
+    }
+}
+
+// x += alpha p; r -= alpha q; column sums of r^2.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_bcg_update(int n, int k, const double* __restrict__ p,
+                                                       const double* __restrict__ q, double* __restrict__ x,
+                                                       double* __restrict__ r, double* __restrict__ partials,
+                                                       const KCtl* __restrict__ ctl) {
+    __shared__ double red[kBlock];
+    constexpr int R = kBlock / G;
+    const int sub = int(threadIdx.x) / G, lane = int(threadIdx.x) % G;
+    const int col = int(blockIdx.y) * G + lane;
+    double s = 0;
+    int first, step, end;
+    block_groups((n + R - 1) / R, first, step, end);
+    if (col < k && !stopped(ctl + col)) {
+        const double alpha = ctl[col].alpha;
+        for (int g = first; g < end; g += step) {
+            const int row = g * R + sub;
+            if (row < n) {
+                const size_t at = size_t(row) * size_t(k) + size_t(col);
+                x[at] = alpha * p[at] + x[at];
+                const double ri = -alpha * q[at] + r[at];
+                r[at] = ri;
+                s += ri * ri;
+            }
+        }
+    }
+    block_partials(s, G, col, k, red, partials);
+}
+
+// First Jacobi sweep from a zero iterate: x = (w * inv) * b.
+template <class T>
+__global__ void k_bjacobi_zero(int n, int k, const T* __restrict__ inv, T w, const T* __restrict__ b,
+                               T* __restrict__ x) {
+#pragma clang fp contract(off)
+    const size_t nk = size_t(n) * size_t(k);
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
+        const T wd = w * inv[i / size_t(k)];
+        x[i] = wd * b[i];
+    }
+}
+
+// coarse[c][j] = sum over the members m of aggregate c (ascending) of f[m][j].
+template <class T>
+__global__ void k_brestrict(int nc, int k, const int* __restrict__ off, const int* __restrict__ mem,
+                            const T* __restrict__ f, T* __restrict__ coarse) {
+#pragma clang fp contract(off)
+    const size_t nk = size_t(nc) * size_t(k);
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
+        const int c = int(i / size_t(k)), j = int(i % size_t(k));
+        T s = 0;
+        for (int q = off[c]; q < off[c + 1]; ++q) s += f[size_t(mem[q]) * size_t(k) + size_t(j)];
+        coarse[i] = s;
+    }
+}
+
+// fine += P coarse, or fine = P coarse when the fine iterate is known zero.
+template <class T>
+__global__ void k_bprolong(int n, int k, const int* __restrict__ map, const T* __restrict__ coarse,
+                           T* __restrict__ fine, int set) {
+    const size_t nk = size_t(n) * size_t(k);
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
+        const T c = coarse[size_t(map[i / size_t(k)]) * size_t(k) + i % size_t(k)];
+        fine[i] = set ? c : fine[i] + c;
+    }
+}
+
+template <class S, class D>
+__global__ void k_bconvert(size_t count, const S* __restrict__ s, D* __restrict__ d) {
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < count; i += size_t(gridDim.x) * kBlock)
+        d[i] = D(s[i]);
+}
+
+// x = inv * b on the dense coarsest level, one wave per (row, column): lane
+// l adds inv[r][l + 64u] * b[l + 64u] for u = 0, 1, ... in FP64, then the
+// wave reduction -- the operations coarse_inverse_apply performs for one
+// vector.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bcoarse(int n, int k, const double* __restrict__ inv,
+                                                    const T* __restrict__ b, T* __restrict__ x) {
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    const int item = int(blockIdx.x) * (kBlock / kWave) + int(threadIdx.x) / kWave;
+    if (item >= n * k) return;
+    const int r = item / k, j = item % k;
+    double acc = 0;
+    for (int c = lane; c < n; c += kWave) acc += inv[size_t(r) * n + c] * double(b[size_t(c) * size_t(k) + j]);
+    acc = wave_sum(acc);
+    if (lane == 0) x[size_t(r) * size_t(k) + j] = T(acc);
+}
+
+unsigned flat_blocks(size_t count) {
+    return unsigned(std::max<size_t>(1, std::min<size_t>((count + kBlock - 1) / kBlock, 8192)));
+}
+
+template <class T, int E, int D>
+void bspmm(tf_ctx& c, int n, int k, const int* rp, const int* ci, const T* v, const T* X, T* Y, const T* B,
+           const T* inv, T w, double* partials, const KCtl* ctl) {
+    if (n <= 0 || k <= 0) return;
+    const int G = lanes_for(k);
+    const int R = kBlock / G;
+    const dim3 grid(row_grid((n + R - 1) / R, D != kBNone), (k + G - 1) / G);
+#define TF_BS(GG)                                                                                              \
+    case GG:                                                                                                   \
+        hipLaunchKernelGGL((k_bspmm<T, GG, E, D>), grid, dim3(kBlock), 0, c.stream, n, k, rp, ci, v, X, Y, B, \
+                           inv, w, partials, ctl);                                                             \
+        break;
+    switch (G) { TF_BS(1) TF_BS(2) TF_BS(4) TF_BS(8) TF_BS(16) TF_BS(32) TF_BS(64) }
+#undef TF_BS
+    check(hipGetLastError(), "block spmm launch");
+}
+
+// Number of partials a row-parallel launch over n rows writes per column.
+int row_parts(int n, int k) {
+    const int R = kBlock / lanes_for(k);
+    return row_grid((n + R - 1) / R);
+}
+
+void bdot(tf_ctx& c, int n, int k, const double* X, const double* Y, double* partials) {
+    const int G = lanes_for(k);
+    const int R = kBlock / G;
+    const dim3 grid(row_grid((n + R - 1) / R), (k + G - 1) / G);
+#define TF_BD(GG)                                                                                              \
+    case GG:                                                                                                   \
+        hipLaunchKernelGGL((k_bdot<GG>), grid, dim3(kBlock), 0, c.stream, n, k, X, Y, partials);               \
+        break;
+    switch (G) { TF_BD(1) TF_BD(2) TF_BD(4) TF_BD(8) TF_BD(16) TF_BD(32) TF_BD(64) }
+#undef TF_BD
+    check(hipGetLastError(), "block dot launch");
+}
+
+void bcg_update(tf_ctx& c, int n, int k, const double* p, const double* q, double* x, double* r, double* partials,
+                const KCtl* ctl) {
+    const int G = lanes_for(k);
+    const int R = kBlock / G;
+    const dim3 grid(row_grid((n + R - 1) / R), (k + G - 1) / G);
+#define TF_BU(GG)                                                                                              \
+    case GG:                                                                                                   \
+        hipLaunchKernelGGL((k_bcg_update<GG>), grid, dim3(kBlock), 0, c.stream, n, k, p, q, x, r, partials,    \
+                           ctl);                                                                               \
+        break;
+    switch (G) { TF_BU(1) TF_BU(2) TF_BU(4) TF_BU(8) TF_BU(16) TF_BU(32) TF_BU(64) }
+#undef TF_BU
+    check(hipGetLastError(), "block CG update launch");
+}
+
+void bfinal(tf_ctx& c, int k, const double* partials, int nparts, KCtl* ctl, int mode, double* out) {
+    hipLaunchKernelGGL(k_bfinal, dim3(1), dim3(1024), 0, c.stream, k, partials, nparts, ctl, mode, out);
+    check(hipGetLastError(), "block finaliser launch");
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ block SpMM
+
+template <class T>
+void block_spmm(tf_ctx& c, int n, const int* rp, const int* ci, const T* v, int k, const T* X, T* Y) {
+    bspmm<T, kSpmv, kBNone>(c, n, k, rp, ci, v, X, Y, nullptr, nullptr, T(0), nullptr, nullptr);
+}
+template void block_spmm<double>(tf_ctx&, int, const int*, const int*, const double*, int, const double*, double*);
+template void block_spmm<float>(tf_ctx&, int, const int*, const int*, const float*, int, const float*, float*);
+
+// ------------------------------------------------------------- block cycle
+
+BlockAmg::BlockAmg(tf_ctx* ctx, AmgEngine* amg) : c_(ctx), amg_(amg) {}
+
+BlockAmg::~BlockAmg() { release(); }
+
+void BlockAmg::release() {
+    if (mem_) {
+        (void)hipStreamSynchronize(c_->stream);
+        dev_free(*c_, mem_);
+        mem_ = nullptr;
+    }
+    lv_.clear();
+    shape_.clear();
+    k_ = 0;
+}
+
+void BlockAmg::reserve(int k) {
+    const size_t es = amg_->opt_.precision == 32 ? sizeof(float) : sizeof(double);
+    std::vector<int> shape;
+    for (auto& l : amg_->levels_) shape.push_back(l.native.n);
+    shape.push_back(int(es));
+    if (k == k_ && shape == shape_ && mem_) return;
+    release();
+    size_t total = 0;
+    auto round = [](size_t b) { return (b + 255) / 256 * 256; };
+    for (auto& l : amg_->levels_) total += 4 * round(size_t(l.native.n) * size_t(k) * es);
+    mem_ = dev_alloc<char>(*c_, total);
+    char* p = static_cast<char*>(mem_);
+    lv_.resize(amg_->levels_.size());
+    for (size_t d = 0; d < lv_.size(); ++d) {
+        const size_t bytes = round(size_t(amg_->levels_[d].native.n) * size_t(k) * es);
+        lv_[d].b = p;
+        lv_[d].x = p + bytes;
+        lv_[d].x2 = p + 2 * bytes;
+        lv_[d].r = p + 3 * bytes;
+        p += 4 * bytes;
+    }
+    k_ = k;
+    shape_ = shape;
+    visits.assign(lv_.size(), 0);
+}
+
+template <class L>
+void BlockAmg::smooth(size_t d, int sweeps) {
+    const AmgLevel& a = amg_->levels_[d];
+    Lvl& l = lv_[d];
+    const DevCsr& A = a.native;
+    const L w = L(amg_->opt_.omega);
+    for (int s = 0; s < sweeps; ++s) {
+        if (l.zero) {
+            const size_t nk = size_t(A.n) * size_t(k_);
+            hipLaunchKernelGGL(k_bjacobi_zero<L>, dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, A.n, k_,
+                               static_cast<const L*>(a.inv), w, static_cast<const L*>(l.b), static_cast<L*>(l.x));
+            check(hipGetLastError(), "block Jacobi launch");
+            l.zero = false;
+        }
+        else {
+            bspmm<L, kJacobi, kBNone>(*c_, A.n, k_, A.rp, A.ci, static_cast<const L*>(a.vals),
+                                      static_cast<const L*>(l.x), static_cast<L*>(l.x2), static_cast<const L*>(l.b),
+                                      static_cast<const L*>(a.inv), w, nullptr, nullptr);
+            std::swap(l.x, l.x2);
+        }
+    }
+}
+
+template <class L>
+void BlockAmg::restrict_to(size_t d, bool ofResidual) {
+    const AmgLevel& a = amg_->levels_[d];
+    Lvl& l = lv_[d];
+    const DevAgg& t = a.transfer;
+    const L* f = static_cast<const L*>(l.b);
+    if (ofResidual && !l.zero) {
+        bspmm<L, kResidual, kBNone>(*c_, a.native.n, k_, a.native.rp, a.native.ci, static_cast<const L*>(a.vals),
+                                    static_cast<const L*>(l.x), static_cast<L*>(l.r), static_cast<const L*>(l.b),
+                                    nullptr, L(0), nullptr, nullptr);
+        f = static_cast<const L*>(l.r);
+    }
+    const size_t nk = size_t(t.nc) * size_t(k_);
+    hipLaunchKernelGGL(k_brestrict<L>, dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, t.nc, k_, t.offsets,
+                       t.members, f, static_cast<L*>(lv_[d + 1].b));
+    check(hipGetLastError(), "block restrict launch");
+}
+
+template <class L>
+void BlockAmg::prolong(size_t d) {
+    const AmgLevel& a = amg_->levels_[d];
+    Lvl& l = lv_[d];
+    const size_t nk = size_t(a.native.n) * size_t(k_);
+    hipLaunchKernelGGL(k_bprolong<L>, dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, a.native.n, k_,
+                       a.transfer.map, static_cast<const L*>(lv_[d + 1].x), static_cast<L*>(l.x), int(l.zero));
+    check(hipGetLastError(), "block prolong launch");
+    l.zero = false;
+}
+
+template <class L>
+void BlockAmg::coarse(size_t d) {
+    const AmgLevel& a = amg_->levels_[d];
+    Lvl& l = lv_[d];
+    const double* inverse = amg_->coarse_inverse();
+    if (inverse) {
+        const int items = a.native.n * k_;
+        const int per = kBlock / kWave;
+        hipLaunchKernelGGL(k_bcoarse<L>, dim3((items + per - 1) / per), dim3(kBlock), 0, c_->stream, a.native.n, k_,
+                           inverse, static_cast<const L*>(l.b), static_cast<L*>(l.x));
+        check(hipGetLastError(), "block coarse solve launch");
+        l.zero = false;
+    }
+    else {
+        smooth<L>(d, amg_->opt_.coarseSweeps);
+    }
+}
+
+// executeCycle (Cycle.cuh:35-63), the operations of AmgEngine::cycle.
+template <class L>
+void BlockAmg::cycle(int kind, size_t d, bool zero) {
+    ++visits[d];
+    Lvl& l = lv_[d];
+    if (zero || kind == 2) l.zero = true;
+    if (d + 1 == lv_.size()) {
+        coarse<L>(d);
+        return;
+    }
+    const AmgOptionsC& o = amg_->opt_;
+    if (kind == 2) {
+        restrict_to<L>(d, false);
+        cycle<L>(2, d + 1, true);
+        prolong<L>(d);
+        cycle<L>(0, d, false);
+        return;
+    }
+    smooth<L>(d, o.pre);
+    restrict_to<L>(d, true);
+    cycle<L>(kind, d + 1, true);
+    if (kind == 1) cycle<L>(kind, d + 1, false);
+    prolong<L>(d);
+    smooth<L>(d, o.post);
+}
+
+template <class L>
+void BlockAmg::apply_typed(const double* R, double* Z) {
+    const int n = amg_->levels_[0].native.n;
+    const size_t nk = size_t(n) * size_t(k_);
+    hipLaunchKernelGGL((k_bconvert<double, L>), dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk, R,
+                       static_cast<L*>(lv_[0].b));
+    check(hipGetLastError(), "block enter launch");
+    for (auto& l : lv_) l.zero = false;
+    std::fill(visits.begin(), visits.end(), 0);
+    cycle<L>(amg_->opt_.cycle, 0, true);
+    if (lv_[0].zero)
+        fill_zero(*c_, Z, nk * sizeof(double));
+    else
+        hipLaunchKernelGGL((k_bconvert<L, double>), dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk,
+                           static_cast<const L*>(lv_[0].x), Z);
+    check(hipGetLastError(), "block exit launch");
+}
+
+void BlockAmg::apply(int k, const double* R, double* Z) {
+    require(amg_->ready(), "AMG requires setup", TF_E_INVALID);
+    require(k >= 1, "block apply needs at least one column");
+    reserve(k);
+    if (amg_->opt_.precision == 32)
+        apply_typed<float>(R, Z);
+    else
+        apply_typed<double>(R, Z);
+}
+
+// ---------------------------------------------------------- block Krylov
+
+void SolverEngine::reserve_block(int k) {
+    if (k == bk_ && bvec_) return;
+    release_block();
+    const size_t nk = size_t(n_) * size_t(k);
+    bvec_ = dev_alloc<double>(*c_, 6 * nk);
+    bctl_ = dev_alloc<KCtl>(*c_, size_t(k));
+    bparts_ = dev_alloc<double>(*c_, size_t(kBParts) * size_t(k));
+    bsums_ = dev_alloc<double>(*c_, size_t(k));
+    check(hipHostMalloc(&hbctl_, sizeof(KCtl) * size_t(k), hipHostMallocDefault), "hipHostMalloc block control");
+    check(hipHostMalloc(&hbsums_, sizeof(double) * size_t(k), hipHostMallocDefault), "hipHostMalloc block sums");
+    bk_ = k;
+}
+
+void SolverEngine::release_block() {
+    if (!bvec_) return;
+    (void)hipStreamSynchronize(c_->stream);
+    dev_free(*c_, bvec_);
+    dev_free(*c_, bctl_);
+    dev_free(*c_, bparts_);
+    dev_free(*c_, bsums_);
+    (void)hipHostFree(hbctl_);
+    (void)hipHostFree(hbsums_);
+    bvec_ = nullptr;
+    bctl_ = nullptr;
+    bparts_ = nullptr;
+    bsums_ = nullptr;
+    hbctl_ = nullptr;
+    hbsums_ = nullptr;
+    bk_ = 0;
+}
+
+// Column sums of a row-parallel reduction -> host.
+static void column_sums(tf_ctx& c, int k, const double* partials, int nparts, double* dsums, double* hsums) {
+    bfinal(c, k, partials, nparts, nullptr, -1, dsums);
+    check(hipMemcpyAsync(hsums, dsums, sizeof(double) * size_t(k), hipMemcpyDeviceToHost, c.stream),
+          "block sums download");
+    c.sync();
+}
+
+// CG (Cg.cuh:123-189) on k right-hand sides at once: every column follows
+// the single-vector recurrence on its own control block and stops on its
+// own; the preconditioner and the products run on the whole block.
+void SolverEngine::apply_block(int k, const double* B, double* X, tf_solve_result* out) {
+    require(n_ > 0, "CG requires setup", TF_E_INVALID);
+    require(kind_ == 0, "the multi-right-hand-side solve runs CG", TF_E_INVALID);
+    require(k >= 1, "block solve needs at least one right-hand side");
+    reserve_block(k);
+    const size_t nk = size_t(n_) * size_t(k);
+    double* bB = bvec_;
+    double* bX = bvec_ + nk;
+    double* bR = bvec_ + 2 * nk;
+    double* bZ = bvec_ + 3 * nk;
+    double* bP = bvec_ + 4 * nk;
+    double* bQ = bvec_ + 5 * nk;
+    KCtl* ctl = static_cast<KCtl*>(bctl_);
+    KCtl* h = static_cast<KCtl*>(hbctl_);
+    double* hs = static_cast<double*>(hbsums_);
+    const int np = row_parts(n_, k);
+    const unsigned fb = flat_blocks(nk);
+    hipLaunchKernelGGL((k_bconvert<double, double>), dim3(fb), dim3(kBlock), 0, c_->stream, nk, B, bB);
+    hipLaunchKernelGGL((k_bconvert<double, double>), dim3(fb), dim3(kBlock), 0, c_->stream, nk, X, bX);
+    check(hipGetLastError(), "block copy launch");
+
+    auto residual_norms = [&](std::vector<double>& res) {
+        bspmm<double, kResidual, kBSquares>(*c_, n_, k, A_.rp, A_.ci, A_.v, bX, bR, bB, nullptr, 0.0, bparts_,
+                                            nullptr);
+        column_sums(*c_, k, bparts_, np, bsums_, hs);
+        res.resize(size_t(k));
+        for (int j = 0; j < k; ++j) res[size_t(j)] = std::sqrt(hs[j]);
+    };
+    bdot(*c_, n_, k, bB, bB, bparts_);
+    column_sums(*c_, k, bparts_, np, bsums_, hs);
+    std::vector<double> normb(static_cast<size_t>(k)), tol(static_cast<size_t>(k)), res;
+    for (int j = 0; j < k; ++j) {
+        normb[size_t(j)] = std::sqrt(hs[j]);
+        tol[size_t(j)] = std::max(o_.atol, o_.rtol * normb[size_t(j)]);
+    }
+    residual_norms(res);
+
+    int running = 0;
+    for (int j = 0; j < k; ++j) {
+        KCtl& c = h[j];
+        std::memset(&c, 0, sizeof(KCtl));
+        c.normb = normb[size_t(j)];
+        c.tol = tol[size_t(j)];
+        c.res = res[size_t(j)];
+        c.rho_prev = 0.0;
+        c.alpha = 1.0;
+        c.omega = 1.0;
+        c.fresh = 1;
+        c.refresh = o_.refresh;
+        if (!std::isfinite(normb[size_t(j)]) || !std::isfinite(res[size_t(j)]))
+            c.status = kNonFinite;
+        else if (res[size_t(j)] <= tol[size_t(j)])
+            c.status = kConverged;
+        else {
+            c.status = kRunning;
+            ++running;
+        }
+    }
+    check(hipMemcpyAsync(ctl, h, sizeof(KCtl) * size_t(k), hipMemcpyHostToDevice, c_->stream), "block control upload");
+    auto download = [&]() {
+        check(hipMemcpyAsync(h, ctl, sizeof(KCtl) * size_t(k), hipMemcpyDeviceToHost, c_->stream),
+              "block control download");
+        c_->sync();
+    };
+    for (int it = 0; it < o_.maxit && running > 0; ++it) {
+        if (amg_)
+            amg_->block().apply(k, bR, bZ);
+        else
+            hipLaunchKernelGGL((k_bconvert<double, double>), dim3(fb), dim3(kBlock), 0, c_->stream, nk, bR, bZ);
+        bdot(*c_, n_, k, bR, bZ, bparts_);
+        bfinal(*c_, k, bparts_, np, ctl, kFinRhoCg, nullptr);
+        hipLaunchKernelGGL(k_bcg_p, dim3(fb), dim3(kBlock), 0, c_->stream, nk, k, bZ, bP, ctl);
+        bspmm<double, kSpmv, kBDotX>(*c_, n_, k, A_.rp, A_.ci, A_.v, bP, bQ, nullptr, nullptr, 0.0, bparts_, ctl);
+        bfinal(*c_, k, bparts_, np, ctl, kFinCurvCg, nullptr);
+        bcg_update(*c_, n_, k, bP, bQ, bX, bR, bparts_, ctl);
+        bfinal(*c_, k, bparts_, np, ctl, kFinUpdCg, nullptr);
+        download();
+        bool refresh = false;
+        for (int j = 0; j < k; ++j) refresh |= h[j].status == kRunning && h[j].check && !h[j].early;
+        if (refresh) {
+            bspmm<double, kResidual, kBTrue>(*c_, n_, k, A_.rp, A_.ci, A_.v, bX, bR, bB, nullptr, 0.0, bparts_, ctl);
+            bfinal(*c_, k, bparts_, np, ctl, kFinTrue, nullptr);
+            download();
+        }
+        running = 0;
+        for (int j = 0; j < k; ++j) running += h[j].status == kRunning;
+    }
+    // Final true residuals decide each column's status (SolverEngine::finish).
+    residual_norms(res);
+    for (int j = 0; j < k; ++j) {
+        int st = h[j].status == kRunning ? kMaxIt : h[j].status;
+        const int its = h[j].status == kRunning ? o_.maxit : h[j].iter;
+        const double r = res[size_t(j)];
+        if (!std::isfinite(r) || !std::isfinite(normb[size_t(j)]))
+            st = kNonFinite;
+        else if (r <= tol[size_t(j)])
+            st = kConverged;
+        else if (st == kConverged)
+            st = kMaxIt;
+        out[j].status = st;
+        out[j].iterations = its;
+        out[j].residual_norm = r;
+        out[j].relative_residual = normb[size_t(j)] > 0 ? r / normb[size_t(j)] : r;
+    }
+    hipLaunchKernelGGL((k_bconvert<double, double>), dim3(fb), dim3(kBlock), 0, c_->stream, nk, bX, X);
+    check(hipGetLastError(), "block copy launch");
+    c_->sync();
+}
+
+}  // namespace tf

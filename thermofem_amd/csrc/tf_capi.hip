@@ -349,7 +349,7 @@ int tf_fill_zero(tf_ctx* ctx, void* dst, size_t bytes) { TF_CTX_GUARD(ctx, fill_
     int tf_spmm_csr_##SUF(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const T* v, int k,          \
                           const T* X, T* Y) {                                                                        \
         TF_CTX_GUARD(ctx, require(k >= 0 && (X != Y || n == 0), "SpMM shape or alias mismatch");                    \
-                     spmm<T>(*ctx, n, nnz, rp, ci, v, k, X, Y));                                                     \
+                     (void)nnz; block_spmm<T>(*ctx, n, rp, ci, v, k, X, Y));                                        \
     }                                                                                                                \
     int tf_jacobi_update_##SUF(tf_ctx* ctx, int n, T w, const T* inv, const T* b, const T* product, T* x) {          \
         TF_CTX_GUARD(ctx, jacobi_update<T>(*ctx, n, w, inv, b, product, x));                                        \
@@ -780,6 +780,40 @@ int tf_solver_apply(tf_solver* s, const double* b, double* x, int host_memory, t
                   "download x");
             s->ctx->sync();
         }
+    });
+}
+
+int tf_amg_apply_block(tf_amg* amg, int k, const double* R, double* Z) {
+    return guard([&] {
+        require(amg != nullptr && amg->eng->ready(), "AMG requires setup");
+        require(k >= 1 && R != nullptr && Z != nullptr && static_cast<const void*>(R) != Z,
+                "block apply needs k >= 1 and distinct device blocks");
+        DeviceScope scope(amg->ctx->device);
+        amg->eng->block().apply(k, R, Z);
+        amg->ctx->sync();
+    });
+}
+
+int tf_amg_block_visits(tf_amg* amg, int64_t* visits) {
+    return guard([&] {
+        require(amg != nullptr && visits != nullptr, "null handle");
+        const auto& v = amg->eng->block().visits;
+        for (size_t d = 0; d < v.size(); ++d) visits[d] = v[d];
+    });
+}
+
+int tf_solver_apply_block(tf_solver* s, int k, const double* B, double* X, tf_solve_result* results) {
+    return guard([&] {
+        require(s != nullptr && results != nullptr, "null handle");
+        require(k >= 1 && B != nullptr && X != nullptr && static_cast<const void*>(B) != X,
+                "block solve needs k >= 1 and distinct device blocks");
+        DeviceScope scope(s->ctx->device);
+        Events ev;
+        check(hipEventRecord(ev.a, s->ctx->stream), "event record");
+        s->eng->apply_block(k, B, X, results);
+        check(hipEventRecord(ev.b, s->ctx->stream), "event record");
+        check(hipEventSynchronize(ev.b), "event sync");
+        s->eng->applySeconds = elapsed(ev.a, ev.b);
     });
 }
 

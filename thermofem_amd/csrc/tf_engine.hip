@@ -16,22 +16,9 @@
 
 #include "tf_device.hpp"
 #include "tf_engine.hpp"
+#include "tf_krylov.hpp"
 
 namespace tf {
-
-// ---------------------------------------------------------------- control
-
-struct KCtl {
-    double normb, tol, res, rho, rho_prev, alpha, beta, omega, den, ns;
-    int status;  // -1 running, otherwise the status handed to finish()
-    int iter;
-    int fresh;
-    int check;  // true-residual refresh requested
-    int early;  // BiCGStab: ||s|| <= tol in this iteration
-    int refresh;
-};
-
-enum { kRunning = -1, kConverged = 0, kMaxIt = 1, kBreakdown = 2, kNonFinite = 3 };
 
 namespace {
 
@@ -193,101 +180,6 @@ struct SellRows {
         for_sell_rows(m, v, x, f);
     }
 };
-
-// Finalisers run by thread 0 of the last block of a reduction.
-enum Fin { kFinRhoCg, kFinCurvCg, kFinUpdCg, kFinTrue, kFinRhoBi, kFinDenBi, kFinSBi, kFinUpdBi };
-
-__device__ void finalize(int mode, double total, KCtl* c) {
-    switch (mode) {
-    case kFinRhoCg: {  // Cg.cuh:139-160
-        c->iter += 1;
-        const double rho = total;
-        c->rho = rho;
-        if (!isfinite(rho)) c->status = kNonFinite;
-        else if (rho <= 0) c->status = kBreakdown;
-        else if (!c->fresh) {
-            c->beta = rho / c->rho_prev;
-            if (!isfinite(c->beta)) c->status = kNonFinite;
-        }
-        break;
-    }
-    case kFinCurvCg: {  // Cg.cuh:161-172
-        const double curv = total;
-        if (!isfinite(curv)) c->status = kNonFinite;
-        else if (curv <= 0) c->status = kBreakdown;
-        else {
-            c->alpha = c->rho / curv;
-            if (!isfinite(c->alpha)) c->status = kNonFinite;
-        }
-        c->fresh = 0;
-        break;
-    }
-    case kFinUpdCg: {  // Cg.cuh:173-187
-        const double res = sqrt(total);
-        c->res = res;
-        if (!isfinite(res)) c->status = kNonFinite;
-        else {
-            c->check = (res <= c->tol || c->iter % c->refresh == 0) ? 1 : 0;
-            c->rho_prev = c->rho;
-        }
-        break;
-    }
-    case kFinTrue: {  // true-residual refresh, Cg.cuh:178-185 / BiCgStab.cuh:202-211,235-243
-        const double res = sqrt(total);
-        c->res = res;
-        c->check = 0;
-        if (!isfinite(res) || res <= c->tol) c->status = kMaxIt;
-        else c->fresh = 1;
-        break;
-    }
-    case kFinRhoBi: {  // BiCgStab.cuh:160-180
-        c->iter += 1;
-        c->early = 0;
-        const double rho = total;
-        c->rho = rho;
-        if (!isfinite(rho)) c->status = kNonFinite;
-        else if (rho == 0) c->status = kBreakdown;
-        else if (!c->fresh) {
-            c->beta = (rho / c->rho_prev) * (c->alpha / c->omega);
-            if (!isfinite(c->beta)) c->status = kNonFinite;
-        }
-        break;
-    }
-    case kFinDenBi: {  // BiCgStab.cuh:182-195
-        const double den = total;
-        c->den = den;
-        if (!isfinite(den)) c->status = kNonFinite;
-        else if (den == 0) c->status = kBreakdown;
-        else {
-            c->alpha = c->rho / den;
-            if (!isfinite(c->alpha)) c->status = kNonFinite;
-        }
-        c->fresh = 0;
-        break;
-    }
-    case kFinSBi: {  // BiCgStab.cuh:196-212
-        const double ns = sqrt(total);
-        c->ns = ns;
-        if (!isfinite(ns)) c->status = kNonFinite;
-        else c->early = ns <= c->tol ? 1 : 0;
-        break;
-    }
-    case kFinUpdBi: {  // BiCgStab.cuh:229-245
-        const double res = sqrt(total);
-        c->res = res;
-        if (!isfinite(res)) c->status = kNonFinite;
-        else {
-            c->check = (res <= c->tol || c->iter % c->refresh == 0) ? 1 : 0;
-            c->rho_prev = c->rho;
-        }
-        break;
-    }
-    default:
-        break;
-    }
-}
-
-__device__ __forceinline__ bool stopped(const KCtl* c) { return c->status != kRunning; }
 
 // Block partial of a reduction whose finaliser runs in k_kfinal.
 __device__ __forceinline__ void reduce_fin(double contrib, double* partials, unsigned* ticket, KCtl* ctl, int mode) {
@@ -494,7 +386,15 @@ AmgOptionsC amg_options_from(const tf_amg_options* o) {
 }
 
 AmgEngine::AmgEngine(tf_ctx* ctx, const AmgOptionsC& o) : opt_(o), c_(ctx) {}
-AmgEngine::~AmgEngine() { release(); }
+AmgEngine::~AmgEngine() {
+    delete block_;
+    release();
+}
+
+BlockAmg& AmgEngine::block() {
+    if (!block_) block_ = new BlockAmg(c_, this);
+    return *block_;
+}
 
 namespace {
 void free_level(tf_ctx& c, AmgLevel& l, bool f32) {
@@ -1075,6 +975,7 @@ void SolverEngine::release_graphs() {
 SolverEngine::~SolverEngine() {
     if (c_) (void)hipStreamSynchronize(c_->stream);
     release_graphs();
+    release_block();
     free_csr(*c_, A_);
     free_sell(*c_, As_);
     dev_free(*c_, Asv_);
@@ -1086,6 +987,7 @@ SolverEngine::~SolverEngine() {
 void SolverEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v) {
     require(n >= 1, "Solver requires a nonempty square matrix");
     release_graphs();
+    release_block();
     DevCsr A = canonicalize(*c_, n, nnz, rp, ci, v);
     if (vec_) {
         dev_free(*c_, vec_);

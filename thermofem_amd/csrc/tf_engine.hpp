@@ -68,6 +68,8 @@ template <class L>
 void tail_run(tf_ctx& c, const TailLevel* levels, int top, int count, const void* image, int imageBytes,
               const TailOp* ops, int nops, const double* inverse, L w);
 
+class BlockAmg;
+
 class AmgEngine {
 public:
     AmgEngine(tf_ctx* ctx, const AmgOptionsC& o);
@@ -78,6 +80,10 @@ public:
     // z = M^-1 r (FP64 device vectors; may alias).  Enqueued on the context stream.
     void apply(const double* rhs, double* x);
     bool ready() const { return !levels_.empty(); }
+    // Explicit inverse of the coarsest operator (nullptr: coarse Jacobi).
+    const double* coarse_inverse() const { return factored_ ? inverse_ : nullptr; }
+    // The multi-right-hand-side cycle over this hierarchy (made on first use).
+    BlockAmg& block();
     int n() const { return levels_.empty() ? 0 : levels_[0].native.n; }
 
     std::vector<AmgLevel> levels_;
@@ -120,6 +126,45 @@ private:
     double* lu_ = nullptr;
     int* piv_ = nullptr;
     double* inverse_ = nullptr;
+    BlockAmg* block_ = nullptr;
+};
+
+// The cycle of an AmgEngine's hierarchy on k right-hand sides at once
+// (row-major n x k blocks, tf_block.hip): AmgEngine::cycle's operations,
+// column by column in the same arithmetic, without the coarse tail.
+class BlockAmg {
+public:
+    BlockAmg(tf_ctx* ctx, AmgEngine* amg);
+    ~BlockAmg();
+    void apply(int k, const double* R, double* Z);  // Z = M^-1 R per column; R, Z distinct
+    std::vector<int64_t> visits;
+
+private:
+    struct Lvl {
+        void *b = nullptr, *x = nullptr, *x2 = nullptr, *r = nullptr;
+        bool zero = false;
+    };
+    template <class L>
+    void apply_typed(const double* R, double* Z);
+    template <class L>
+    void cycle(int kind, size_t d, bool zero);
+    template <class L>
+    void smooth(size_t d, int sweeps);
+    template <class L>
+    void restrict_to(size_t d, bool ofResidual);
+    template <class L>
+    void prolong(size_t d);
+    template <class L>
+    void coarse(size_t d);
+    void reserve(int k);
+    void release();
+
+    tf_ctx* c_;
+    AmgEngine* amg_;
+    int k_ = 0;
+    std::vector<int> shape_;  // level sizes and element size the buffers were made for
+    std::vector<Lvl> lv_;
+    void* mem_ = nullptr;
 };
 
 struct KrylovOptionsC {
@@ -135,6 +180,9 @@ public:
     ~SolverEngine();
     void setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v);
     tf_solve_result apply(const double* b, double* x);  // device vectors
+    // k right-hand sides at once (CG only): B, X row-major n x k device blocks,
+    // out[k] host results, one per column.
+    void apply_block(int k, const double* B, double* X, tf_solve_result* out);
     AmgEngine* amg() { return amg_; }
     int n() const { return n_; }
     double setupSeconds = 0, applySeconds = 0;
@@ -150,6 +198,8 @@ private:
     bool sell_spmv_pays();
     double norm_host(const double* x);
     void release_graphs();
+    void reserve_block(int k);
+    void release_block();
 
     tf_ctx* c_;
     int kind_;
@@ -166,6 +216,14 @@ private:
     void* hctl_ = nullptr;     // pinned host mirror
     hipGraphExec_t graph_[2] = {nullptr, nullptr};
     double normb_ = 0, tol_ = 0;
+    // multi-right-hand-side work (allocated by the first block solve)
+    int bk_ = 0;
+    double* bvec_ = nullptr;  // B, X, R, Z, P, Q blocks
+    void* bctl_ = nullptr;    // one control block per column
+    void* hbctl_ = nullptr;
+    double* bparts_ = nullptr;
+    double* bsums_ = nullptr;
+    void* hbsums_ = nullptr;
 };
 
 }  // namespace tf

@@ -193,8 +193,10 @@ void jacobi_fused(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, c
                   const T* b, const T* xin, T* xout);
 template <class T>
 void jacobi_update(tf_ctx& c, int n, T w, const T* inv, const T* b, const T* product, T* x);
+// Y = A X for row-major n x k blocks (tf_block.hip); every column's row
+// sums are the single-vector SpMV's (rounded products in stored order).
 template <class T>
-void spmm(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, int k, const T* X, T* Y);
+void block_spmm(tf_ctx& c, int n, const int* rp, const int* ci, const T* v, int k, const T* X, T* Y);
 template <class T>
 void axpy(tf_ctx& c, int n, T alpha, const T* x, T* y);
 template <class T>

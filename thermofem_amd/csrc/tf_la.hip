@@ -1,6 +1,6 @@
 // Linear-algebra kernels for gfx950: CSR SpMV over LDS-staged row blocks
-// (or sub-wave row groups) with fused epilogues (residual, weighted Jacobi), SpMM over row-major blocks of
-// right-hand sides, level-1 BLAS with deterministic reductions, aggregate
+// (or sub-wave row groups) with fused epilogues (residual, weighted Jacobi),
+// level-1 BLAS with deterministic reductions, aggregate
 // restriction/prolongation and element conversions.
 #include "tf_device.hpp"
 #include "tf_internal.hpp"
@@ -91,27 +91,6 @@ void launch_rows(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, co
     }
 #undef TF_ROWS
     check(hipGetLastError(), "csr row kernel launch");
-}
-
-// Y = A X, X and Y row-major n x k.  G lanes per row cover the k columns
-// (coalesced row reads of X); matrix entries are broadcast to the group.
-template <int G, class T>
-__global__ __launch_bounds__(kBlock) void k_spmm(int n, int k, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                 const T* __restrict__ v, const T* __restrict__ X,
-                                                 T* __restrict__ Y) {
-    constexpr int R = kBlock / G;
-    const int row = int(blockIdx.x) * R + int(threadIdx.x) / G;
-    const int lane = int(threadIdx.x) % G;
-    if (row >= n) return;
-    const int s = rp[row], e = rp[row + 1];
-    for (int c0 = 0; c0 < k; c0 += G) {
-        const int col = c0 + lane;
-        T acc = 0;
-        if (col < k) {
-            for (int j = s; j < e; ++j) acc += v[j] * X[size_t(ci[j]) * size_t(k) + col];
-            Y[size_t(row) * size_t(k) + col] = acc;
-        }
-    }
 }
 
 template <class T>
@@ -322,24 +301,6 @@ void jacobi_update(tf_ctx& c, int n, T w, const T* inv, const T* b, const T* pro
     check(hipGetLastError(), "jacobi update launch");
 }
 template <class T>
-void spmm(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const T* v, int k, const T* X, T* Y) {
-    (void)nnz;
-    if (n <= 0 || k <= 0) return;
-    int g = 1;
-    while (g < 64 && g < k) g <<= 1;
-    const int rows = kBlock / g;
-    const dim3 grid((n + rows - 1) / rows);
-#define TF_MM(G)                                                                                                \
-    case G:                                                                                                     \
-        hipLaunchKernelGGL((k_spmm<G, T>), grid, dim3(kBlock), 0, c.stream, n, k, rp, ci, v, X, Y);              \
-        break;
-    switch (g) {
-        TF_MM(1) TF_MM(2) TF_MM(4) TF_MM(8) TF_MM(16) TF_MM(32) TF_MM(64)
-    }
-#undef TF_MM
-    check(hipGetLastError(), "spmm launch");
-}
-template <class T>
 void axpy(tf_ctx& c, int n, T alpha, const T* x, T* y) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_axpy<T>, dim3(blocks_for(size_t(n))), dim3(kBlock), 0, c.stream, n, alpha, x, y);
@@ -433,7 +394,6 @@ void fill_zero(tf_ctx& c, void* dst, size_t bytes) {
     template void jacobi_fused<T>(tf_ctx&, int, int64_t, const int*, const int*, const T*, const T*, T, const T*, \
                                   const T*, T*);                                                                 \
     template void jacobi_update<T>(tf_ctx&, int, T, const T*, const T*, const T*, T*);                           \
-    template void spmm<T>(tf_ctx&, int, int64_t, const int*, const int*, const T*, int, const T*, T*);           \
     template void axpy<T>(tf_ctx&, int, T, const T*, T*);                                                        \
     template void scal<T>(tf_ctx&, int, T, T*);                                                                  \
     template void dot_device<T>(tf_ctx&, int, const T*, const T*, double*, int);                                 \

@@ -180,6 +180,60 @@ def convection_diffusion3d(nx=96, peclet=10.0, seed=4, rhs_seed=5):
     return _compress(cols, vals, mask), uniform(rhs_seed, n)
 
 
+# Kuhn (Freudenthal) subdivision of the unit cube into 6 tetrahedra: tet p
+# walks from corner (0,0,0) to (1,1,1) along the axes in the order of the
+# permutation p; every tet has volume 1/6.
+_KUHN = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+
+
+def fem_p1_heat(nc=64, dt=1e-3, sigma=0.5, seed=2):
+    """P1 finite elements for one backward-Euler step of the heat equation,
+    A = M / dt + K, on the unit cube split into nc^3 cells x 6 Kuhn tets
+    ((nc+1)^3 nodes, natural boundary conditions).  k_tet = exp(sigma N(0,1))
+    per tet (tet index 6 * cell + p, cell = i + nc*(j + nc*k)).  Element
+    matrices: K_e = k_tet vol grad(l_a).grad(l_b), M_e = vol/20 (1 + delta_ab),
+    vol = h^3/6.  Entries are summed in the order cell, tet, local row, local
+    column.  SPD.  Node index i + (nc+1)*(j + (nc+1)*k).  Returns (rp, ci, v).
+    """
+    h = 1.0 / nc
+    m = nc + 1
+    n = m ** 3
+    kt = np.exp(sigma * normal(seed, 6 * nc ** 3))
+    cell = np.arange(nc ** 3)
+    ci_, cj, ck = cell % nc, (cell // nc) % nc, cell // (nc * nc)
+    base = ci_ + m * (cj + m * ck)
+    stride = np.array([1, m, m * m])
+    # barycentric gradients (times h) of the path tet in its own axis frame:
+    # l0 = 1 - x_a, l1 = x_a - x_b, l2 = x_b - x_c, l3 = x_c
+    gram = np.array([[1, -1, 0, 0], [-1, 2, -1, 0], [0, -1, 2, -1], [0, 0, -1, 1]], np.float64)
+    vol = h ** 3 / 6.0
+    mass = vol / 20.0 * (np.ones((4, 4)) + np.eye(4))
+    rows, cols, vals = [], [], []
+    nodes = np.empty((nc ** 3, 6, 4), np.int64)
+    for p, perm in enumerate(_KUHN):
+        off = np.zeros(4, np.int64)
+        for q in range(3):
+            off[q + 1] = off[q] + stride[perm[q]]
+        nodes[:, p, :] = base[:, None] + off[None, :]
+    stiff = kt.reshape(nc ** 3, 6)[:, :, None, None] * (vol / h ** 2) * gram[None, None]
+    elem = stiff + (mass / dt)[None, None]
+    rows = np.repeat(nodes[:, :, :, None], 4, axis=3).ravel()
+    cols = np.repeat(nodes[:, :, None, :], 4, axis=2).ravel()
+    key = rows * n + cols
+    uniq, inverse = np.unique(key, return_inverse=True)
+    v = np.bincount(inverse.ravel(), weights=elem.ravel(), minlength=len(uniq))
+    r = (uniq // n).astype(np.int64)
+    c = (uniq % n).astype(np.int32)
+    rp = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(r, minlength=n), out=rp[1:])
+    return rp.astype(np.int32), c, v
+
+
+def block_rhs(n, k, seed=6):
+    """k right-hand sides U[0,1), row-major n x k (entry (i, j) at i*k + j)."""
+    return uniform(seed, n * k).reshape(n, k)
+
+
 def spmv_bytes(n, nnz, value_size):
     """Algorithmic bytes of one CSR SpMV: values+columns, offsets, x once, y."""
     return nnz * (value_size + 4) + 4 * (n + 1) + 2 * n * value_size

@@ -171,6 +171,23 @@ class Amg(_AmgView):
             _raise(e)
         return x
 
+    def apply_block(self, rhs):
+        """One cycle on each column of a row-major n x k block (host arrays)."""
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        n, k = rhs.shape
+        dr = DeviceArray.from_host(rhs.ravel())
+        dz = DeviceArray(n * k, np.float64)
+        try:
+            check(lib().tf_amg_apply_block(self.handle, k, dr.ptr, dz.ptr))
+        except TfError as e:
+            _raise(e)
+        return dz.get().reshape(n, k)
+
+    def last_block_visits(self):
+        v = (C.c_int64 * self.level_count())()
+        check(lib().tf_amg_block_visits(self.handle, v))
+        return list(v)
+
 
 class _Krylov:
     kind = 0
@@ -229,6 +246,26 @@ class _Krylov:
         r = _lib.SolveResultC()
         check(lib().tf_solver_apply(self.handle, b.ptr, x.ptr, 0, C.byref(r)))
         return SolveResult(SolveStatus(r.status), r.iterations, r.residual_norm, r.relative_residual)
+
+    def apply_block(self, b, x0=None):
+        """CG on the k columns of a row-major n x k block at once (host arrays):
+        returns (x, [SolveResult per column])."""
+        b = np.ascontiguousarray(b, np.float64)
+        n, k = b.shape
+        x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64)
+        db = DeviceArray.from_host(b.ravel())
+        dx = DeviceArray.from_host(x.ravel())
+        res = self.apply_block_device(k, db, dx)
+        return dx.get().reshape(n, k), res
+
+    def apply_block_device(self, k, b, x):
+        """Block solve with device blocks (DeviceArray, row-major n x k)."""
+        r = (_lib.SolveResultC * k)()
+        try:
+            check(lib().tf_solver_apply_block(self.handle, k, b.ptr, x.ptr, r))
+        except TfError as e:
+            _raise(e)
+        return [SolveResult(SolveStatus(q.status), q.iterations, q.residual_norm, q.relative_residual) for q in r]
 
     def amg(self):
         h = lib().tf_solver_amg(self.handle)
