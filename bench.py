@@ -49,10 +49,12 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch each iteration's kernels without hipGraph")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the level-0 SpMV launches and a copy kernel of known size (for rocprofv3 --pmc)")
-    ap.add_argument("--workload", choices=["heat3d", "convdiff", "fem"], default="heat3d",
+    ap.add_argument("--workload", choices=["heat3d", "convdiff", "fem", "batch"], default="heat3d",
                     help="convdiff: 96^3 upwind convection-diffusion, BiCGStab with V/W/F cycles, FP64 and FP32 "
                          "levels, cell Peclet 10/30/100; fem: P1 tets on 64^3 cells, 64 right-hand sides in one "
-                         "block CG vs 64 single solves, and the k=64 SpMM (each one JSON line of its own)")
+                         "block CG vs 64 single solves, and the k=64 SpMM; batch: 32 independent heat problems "
+                         "(edges U{16..96}) in one segmented solve vs 32 separate solves (each one JSON line of "
+                         "its own)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver runs on a single "
@@ -70,6 +72,9 @@ def main():
         return
     if args.workload == "fem":
         fem_block(args)
+        return
+    if args.workload == "batch":
+        batch(args)
         return
 
     nx = args.nx
@@ -387,6 +392,71 @@ def fem_block(args):
         log(f"{name} {t * 1e6:.1f} us {nbytes / t / 1e9:.0f} GB/s")
     out["value"] = out["levels_fp32"]["block"]["s"]
     print(json.dumps(out), flush=True)
+
+
+def batch(args):
+    """32 independent log-normal heat problems, cube edges ~ U{16..96} (seed
+    3), as one block-diagonal matrix: one AMG setup + segmented CG (FP32
+    V(2,2) cycle, FP64 CG, to 1e-8 per system) against 32 separate
+    setup + solve runs of the same solver configuration."""
+    import numpy as np
+    from thermofem_amd import problems as P
+    from thermofem_amd.device import DeviceArray, DeviceCsr
+    from thermofem_amd.solver import AmgOptions, Cg, KrylovOptions, SolveStatus
+
+    log("generating 32 heat problems")
+    (rp, ci, v), b, off, edges = P.heat3d_batch()
+    n, nnz = len(rp) - 1, len(ci)
+    A = DeviceCsr(rp, ci, v)
+    db = DeviceArray.from_host(b)
+    dx = DeviceArray(n, np.float64)
+    amg = AmgOptions(coarse_rows=256, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0, level_precision=32)
+    s = Cg(KrylovOptions(relative_tolerance=1e-8), amg)
+    times, res = [], None
+    for it in range(args.warmup + args.steps):
+        dx.upload(np.zeros(n))
+        t0 = time.perf_counter()
+        s.setup_device(n, nnz, *A.args())
+        res = s.apply_segmented_device(off, db, dx)
+        t1 = time.perf_counter()
+        if it >= args.warmup:
+            times.append(t1 - t0)
+    x = dx.get()
+    worst = 0.0
+    for k in range(len(edges)):
+        lo, hi = off[k], off[k + 1]
+        rs = b[lo:hi] - csr_matvec(rp[lo:hi + 1] - rp[lo], ci[rp[lo]:rp[hi]] - lo, v[rp[lo]:rp[hi]], x[lo:hi])
+        worst = max(worst, float(np.linalg.norm(rs) / np.linalg.norm(b[lo:hi])))
+    its = [r.iterations for r in res]
+    seg = {"s": round(statistics.median(times), 6), "iterations_min_max": [min(its), max(its)],
+           "all_converged": all(r.status == SolveStatus.Converged for r in res), "worst_true_rel_residual": worst}
+    log(f"segmented: {seg}")
+    del s
+    # the same systems one at a time (setup + solve each)
+    one = Cg(KrylovOptions(relative_tolerance=1e-8), amg)
+    total, sits = 0.0, []
+    for k in range(len(edges)):
+        lo, hi = int(off[k]), int(off[k + 1])
+        Ak = DeviceCsr(rp[lo:hi + 1] - rp[lo], ci[rp[lo]:rp[hi]] - lo, v[rp[lo]:rp[hi]])
+        bk = DeviceArray.from_host(np.ascontiguousarray(b[lo:hi]))
+        xk = DeviceArray(hi - lo, np.float64)
+        for rep in range(2):  # second run timed (first warms the tuner caches)
+            xk.upload(np.zeros(hi - lo))
+            t0 = time.perf_counter()
+            one.setup_device(hi - lo, int(rp[hi] - rp[lo]), *Ak.args())
+            r1 = one.apply_device(bk, xk)
+            t1 = time.perf_counter()
+        total += t1 - t0
+        sits.append(r1.iterations)
+    single = {"s": round(total, 6), "iterations_min_max": [min(sits), max(sits)]}
+    log(f"32 separate solves: {single}")
+    print(json.dumps({"metric": "setup + CG solve s of 32 independent heat problems (edges U{16..96}) to 1e-8",
+                      "unit": "s", "higher_is_better": False, "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "dtype": "f64+f32", "data": "synthetic",
+                      "config": {"workload": "32 log-normal heat3d cubes, edges U{16..96} seed 3, segmented CG + "
+                                             "AMG FP32 V(2,2)", "n": n, "nnz": nnz, "edges": edges.tolist()},
+                      "value": seg["s"], "segmented": seg, "separate": single,
+                      "speedup": round(single["s"] / seg["s"], 3)}), flush=True)
 
 
 def csr_matvec(rp, ci, v, x):

@@ -251,6 +251,17 @@ tf_amg* tf_solver_amg(tf_solver* solver);
  * preconditioner and the products run on the whole block (SpMM,
  * Backend.cu:256-281).  BiCGStab solvers return TF_E_INVALID. */
 int tf_solver_apply_block(tf_solver* solver, int k, const double* B, double* X, tf_solve_result* results);
+/* Independent systems in one solve: the solver's matrix is block diagonal
+ * with segment s = rows [host_offsets[s], host_offsets[s+1]) (offsets from 0
+ * to n, nondecreasing; an entry coupling two segments is TF_E_INVALID).  CG
+ * (Cg.cuh:123-189) runs on every segment with its own scalars and stopping
+ * test, each kernel of an iteration one persistent launch over all segments;
+ * the AMG hierarchy of a block-diagonal matrix is block diagonal, so one
+ * cycle preconditions all of them.  b, x distinct device vectors of n;
+ * results[segments] host array.  No reference counterpart (one system per
+ * Cg::apply, Cg.cuh:123-189).  BiCGStab solvers return TF_E_INVALID. */
+int tf_solver_apply_segmented(tf_solver* solver, int segments, const int* host_offsets, const double* b, double* x,
+                              tf_solve_result* results);
 /* Seconds of the last setup / apply measured with HIP events on the solver stream. */
 int tf_solver_timings(tf_solver* solver, double* setup_seconds, double* apply_seconds);
 

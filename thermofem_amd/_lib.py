@@ -133,6 +133,7 @@ _SIGS = {
     "tf_solver_setup": (i32, [vp, i32, i64, vp, vp, vp, i32]),
     "tf_solver_apply": (i32, [vp, vp, vp, i32, C.POINTER(SolveResultC)]),
     "tf_solver_apply_block": (i32, [vp, i32, vp, vp, C.POINTER(SolveResultC)]),
+    "tf_solver_apply_segmented": (i32, [vp, i32, vp, vp, vp, C.POINTER(SolveResultC)]),
     "tf_solver_create_with_amg": (i32, [i32, C.POINTER(KrylovOptions), vp, C.POINTER(vp)]),
     "tf_solver_amg": (vp, [vp]),
     "tf_solver_timings": (i32, [vp, F64P, F64P]),

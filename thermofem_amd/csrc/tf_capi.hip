@@ -817,6 +817,22 @@ int tf_solver_apply_block(tf_solver* s, int k, const double* B, double* X, tf_so
     });
 }
 
+int tf_solver_apply_segmented(tf_solver* s, int segments, const int* host_offsets, const double* b, double* x,
+                              tf_solve_result* results) {
+    return guard([&] {
+        require(s != nullptr && results != nullptr && host_offsets != nullptr, "null handle");
+        require(b != nullptr && x != nullptr && static_cast<const void*>(b) != x,
+                "segmented solve needs distinct device vectors");
+        DeviceScope scope(s->ctx->device);
+        Events ev;
+        check(hipEventRecord(ev.a, s->ctx->stream), "event record");
+        s->eng->apply_segmented(segments, host_offsets, b, x, results);
+        check(hipEventRecord(ev.b, s->ctx->stream), "event record");
+        check(hipEventSynchronize(ev.b), "event sync");
+        s->eng->applySeconds = elapsed(ev.a, ev.b);
+    });
+}
+
 tf_amg* tf_solver_amg(tf_solver* s) { return s && s->amg ? &s->view : nullptr; }
 
 int tf_solver_timings(tf_solver* s, double* setup_seconds, double* apply_seconds) {

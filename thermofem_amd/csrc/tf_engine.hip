@@ -976,6 +976,7 @@ SolverEngine::~SolverEngine() {
     if (c_) (void)hipStreamSynchronize(c_->stream);
     release_graphs();
     release_block();
+    release_segments();
     free_csr(*c_, A_);
     free_sell(*c_, As_);
     dev_free(*c_, Asv_);
@@ -988,6 +989,7 @@ void SolverEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const
     require(n >= 1, "Solver requires a nonempty square matrix");
     release_graphs();
     release_block();
+    release_segments();
     DevCsr A = canonicalize(*c_, n, nnz, rp, ci, v);
     if (vec_) {
         dev_free(*c_, vec_);

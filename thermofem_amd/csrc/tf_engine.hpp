@@ -183,6 +183,9 @@ public:
     // k right-hand sides at once (CG only): B, X row-major n x k device blocks,
     // out[k] host results, one per column.
     void apply_block(int k, const double* B, double* X, tf_solve_result* out);
+    // Independent systems on the diagonal blocks of the operator (CG only):
+    // segment s = rows [off[s], off[s+1]) (host offsets); out[segments].
+    void apply_segmented(int segments, const int* off, const double* b, double* x, tf_solve_result* out);
     AmgEngine* amg() { return amg_; }
     int n() const { return n_; }
     double setupSeconds = 0, applySeconds = 0;
@@ -200,6 +203,7 @@ private:
     void release_graphs();
     void reserve_block(int k);
     void release_block();
+    void release_segments();
 
     tf_ctx* c_;
     int kind_;
@@ -224,6 +228,10 @@ private:
     double* bparts_ = nullptr;
     double* bsums_ = nullptr;
     void* hbsums_ = nullptr;
+    // segmented batch work (tile table, control blocks, partials, vectors)
+    void* sbuf_ = nullptr;
+    void* hsbuf_ = nullptr;
+    int sSegments_ = 0, sTiles_ = 0;
 };
 
 }  // namespace tf

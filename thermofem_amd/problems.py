@@ -234,6 +234,36 @@ def block_rhs(n, k, seed=6):
     return uniform(seed, n * k).reshape(n, k)
 
 
+def block_diagonal(systems):
+    """Concatenate CSR matrices [(rp, ci, v), ...] into one block-diagonal CSR
+    and the row offsets of its segments."""
+    rps, cis, vs, off = [np.zeros(1, np.int64)], [], [], [0]
+    nnz = 0
+    for rp, ci, v in systems:
+        n = len(rp) - 1
+        rps.append(np.asarray(rp[1:], np.int64) + nnz)
+        cis.append(np.asarray(ci, np.int64) + off[-1])
+        vs.append(np.asarray(v, np.float64))
+        nnz += len(ci)
+        off.append(off[-1] + n)
+    return ((np.concatenate(rps).astype(np.int32), np.concatenate(cis).astype(np.int32), np.concatenate(vs)),
+            np.array(off, np.int32))
+
+
+def heat3d_batch(count=32, lo=16, hi=96, seed=3):
+    """`count` independent log-normal heat problems (heat3d_lognormal) with
+    cube edges ~ U{lo..hi} drawn from `seed`; system s uses field seed
+    100 + s.  Returns ((rp, ci, v) block diagonal, b, offsets, edges)."""
+    edges = (lo + np.floor((hi - lo + 1) * uniform(seed, count))).astype(np.int64)
+    systems, rhs = [], []
+    for s, e in enumerate(edges):
+        a, b = heat3d_lognormal(int(e), seed=100 + s)
+        systems.append(a)
+        rhs.append(b)
+    a, off = block_diagonal(systems)
+    return a, np.concatenate(rhs), off, edges
+
+
 def spmv_bytes(n, nnz, value_size):
     """Algorithmic bytes of one CSR SpMV: values+columns, offsets, x once, y."""
     return nnz * (value_size + 4) + 4 * (n + 1) + 2 * n * value_size

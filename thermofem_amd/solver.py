@@ -267,6 +267,26 @@ class _Krylov:
             _raise(e)
         return [SolveResult(SolveStatus(q.status), q.iterations, q.residual_norm, q.relative_residual) for q in r]
 
+    def apply_segmented(self, offsets, b, x0=None):
+        """CG on the independent diagonal blocks [offsets[s], offsets[s+1]) of
+        the set-up matrix at once (host arrays): (x, [SolveResult per segment])."""
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64)
+        db, dx = DeviceArray.from_host(b), DeviceArray.from_host(x)
+        res = self.apply_segmented_device(offsets, db, dx)
+        return dx.get(), res
+
+    def apply_segmented_device(self, offsets, b, x):
+        off = np.ascontiguousarray(offsets, np.int32)
+        S = len(off) - 1
+        r = (_lib.SolveResultC * max(S, 1))()
+        try:
+            check(lib().tf_solver_apply_segmented(self.handle, S, off.ctypes.data_as(C.c_void_p), b.ptr, x.ptr, r))
+        except TfError as e:
+            _raise(e)
+        return [SolveResult(SolveStatus(q.status), q.iterations, q.residual_norm, q.relative_residual)
+                for q in r[:S]]
+
     def amg(self):
         h = lib().tf_solver_amg(self.handle)
         return _AmgView(C.c_void_p(h)) if h else None
