@@ -191,6 +191,8 @@ typedef struct tf_amg_options {
     int cycle;              /* 0 V, 1 W, 2 F (Cycle.cuh:10) */
     int level_precision;    /* 64: cycle in FP64 (reference); 32: FP32 cycle under an FP64 caller */
     int coarse_tail;        /* coarse levels as one one-workgroup program: 0 timed at setup, 1 always, 2 never */
+    int learned;            /* matching weights: 0 strength (reference), 1 learned affinities (needs
+                               tf_amg_set_affinity_model), 2 learned only where PCG converges faster with it */
 } tf_amg_options;
 
 typedef struct tf_krylov_options {
@@ -260,6 +262,43 @@ int tf_solver_apply_block(tf_solver* solver, int k, const double* B, double* X, 
  * cycle preconditions all of them.  b, x distinct device vectors of n;
  * results[segments] host array.  No reference counterpart (one system per
  * Cg::apply, Cg.cuh:123-189).  BiCGStab solvers return TF_E_INVALID. */
+/* ---- Learned edge affinities (no reference counterpart: the reference's
+ * matching reads the strength weights of GpuAggregation.cu:339-364 at
+ * :402-431; the network's weights replace them, the matching is unchanged).
+ * A two-layer message-passing network (hidden 32, bfloat16 weights, FP32
+ * accumulation) scores every off-diagonal entry of a matrix with a symmetric
+ * pattern; forward, backward and Adam run on the GPU.  All matrix and vector
+ * pointers are device pointers. ---- */
+typedef struct tf_gnn tf_gnn;
+int tf_gnn_param_count(void);
+int tf_gnn_create(int device, tf_gnn** model); /* zero weights; set them with tf_gnn_set_params */
+int tf_gnn_destroy(tf_gnn* model);
+int tf_gnn_set_params(tf_gnn* model, const float* host_params, int count); /* resets the optimizer */
+int tf_gnn_get_params(tf_gnn* model, float* host_params, int count);
+/* weights[nnz]: the network's weight of every stored entry, 0 on the diagonal. */
+int tf_gnn_infer(tf_gnn* model, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double* weights);
+/* loss = mean over off-diagonal entries of (w - target)^2 (target[nnz] per stored
+ * entry, device); host_grad (param_count floats, may be NULL) its gradient. */
+int tf_gnn_loss_grad(tf_gnn* model, int n, int64_t nnz, const int* rp, const int* ci, const double* v,
+                     const float* target, double* loss, float* host_grad);
+/* one Adam step on that loss; loss receives the value before the step. */
+int tf_gnn_train_step(tf_gnn* model, int n, int64_t nnz, const int* rp, const int* ci, const double* v,
+                      const float* target, float learning_rate, double* loss);
+/* Training targets: `sweeps` weighted-Jacobi sweeps on A X = 0 from the n x k
+ * row-major block X0, then out[p] = (x_i.x_j)^2 / ((x_i.x_i)(x_j.x_j)) for
+ * every stored entry p = (i, j) (0 on the diagonal). */
+int tf_affinities(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v, int k, int sweeps,
+                  double omega, const double* X0, float* out);
+/* The network an AMG handle uses when options.learned != 0 (not owned; NULL: none). */
+int tf_amg_set_affinity_model(tf_amg* amg, tf_gnn* model);
+/* After setup: whether the learned hierarchy is in use, and the convergence
+ * estimates (tf_amg_convergence_estimate) of both (-1 when not measured). */
+int tf_amg_learned_info(tf_amg* amg, int* used, double* rho_learned, double* rho_plain);
+/* Convergence of the current cycle as a CG preconditioner: the mean residual
+ * reduction per iteration (|r_10|/|r_0|)^(1/10) of 10 PCG iterations on
+ * A x = b, x0 = 0, b fixed pseudo-random (smaller is better). */
+int tf_amg_convergence_estimate(tf_amg* amg, double* rho);
+
 int tf_solver_apply_segmented(tf_solver* solver, int segments, const int* host_offsets, const double* b, double* x,
                               tf_solve_result* results);
 /* Seconds of the last setup / apply measured with HIP events on the solver stream. */

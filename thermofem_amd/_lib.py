@@ -26,7 +26,7 @@ class AmgOptions(C.Structure):
     _fields_ = [("max_levels", i32), ("coarse_rows", i32), ("matching_rounds", i32), ("pre_sweeps", i32),
                 ("post_sweeps", i32), ("coarse_sweeps", i32), ("jacobi_weight", f64),
                 ("strength_threshold", f64), ("absorb_singletons", i32), ("cycle", i32),
-                ("level_precision", i32), ("coarse_tail", i32)]
+                ("level_precision", i32), ("coarse_tail", i32), ("learned", i32)]
 
 
 class KrylovOptions(C.Structure):
@@ -134,6 +134,18 @@ _SIGS = {
     "tf_solver_apply": (i32, [vp, vp, vp, i32, C.POINTER(SolveResultC)]),
     "tf_solver_apply_block": (i32, [vp, i32, vp, vp, C.POINTER(SolveResultC)]),
     "tf_solver_apply_segmented": (i32, [vp, i32, vp, vp, vp, C.POINTER(SolveResultC)]),
+    "tf_gnn_param_count": (i32, []),
+    "tf_gnn_create": (i32, [i32, C.POINTER(vp)]),
+    "tf_gnn_destroy": (i32, [vp]),
+    "tf_gnn_set_params": (i32, [vp, vp, i32]),
+    "tf_gnn_get_params": (i32, [vp, vp, i32]),
+    "tf_gnn_infer": (i32, [vp, i32, i64, vp, vp, vp, vp]),
+    "tf_gnn_loss_grad": (i32, [vp, i32, i64, vp, vp, vp, vp, C.POINTER(f64), vp]),
+    "tf_gnn_train_step": (i32, [vp, i32, i64, vp, vp, vp, vp, C.c_float, C.POINTER(f64)]),
+    "tf_affinities": (i32, [vp, i32, i64, vp, vp, vp, i32, i32, f64, vp, vp]),
+    "tf_amg_set_affinity_model": (i32, [vp, vp]),
+    "tf_amg_learned_info": (i32, [vp, C.POINTER(i32), C.POINTER(f64), C.POINTER(f64)]),
+    "tf_amg_convergence_estimate": (i32, [vp, C.POINTER(f64)]),
     "tf_solver_create_with_amg": (i32, [i32, C.POINTER(KrylovOptions), vp, C.POINTER(vp)]),
     "tf_solver_amg": (vp, [vp]),
     "tf_solver_timings": (i32, [vp, F64P, F64P]),

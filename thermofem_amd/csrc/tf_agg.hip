@@ -228,6 +228,29 @@ __global__ void k_strength_sym(int n, const int* __restrict__ rp, const int* __r
     if (lane == 0) cnt[i] = kept;
 }
 
+// Learned weights in place of the strength measure: every off-diagonal
+// entry with a positive learned weight is an edge of that weight (the
+// weights are symmetric, so the graph is).
+__global__ void k_learned_graph(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                const double* __restrict__ lw, int* __restrict__ scol, double* __restrict__ sw,
+                                int* __restrict__ cnt) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i > n) return;
+    if (i == n) {
+        cnt[n] = 0;
+        return;
+    }
+    int kept = 0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const int c = ci[j];
+        const double w = c != i && lw[j] > 0 ? lw[j] : 0.0;
+        scol[j] = w > 0 ? c : -1;
+        sw[j] = w;
+        kept += w > 0 ? 1 : 0;
+    }
+    cnt[i] = kept;
+}
+
 // Packs the kept entries of each row in order: G lanes per row, positions
 // from a ballot within the group.
 template <int G>
@@ -982,7 +1005,7 @@ bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const doubl
 double* rowmax_workspace(tf_ctx& c, int n) { return c.wst<double>(kRowmax, size_t(n)); }
 
 DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta,
-                      int symmetric, bool prevalidated, int* symmetricOut) {
+                      int symmetric, bool prevalidated, int* symmetricOut, const double* learned) {
     require(std::isfinite(theta) && theta >= 0 && theta <= 1, "Invalid strength threshold");
     if (nnz > int64_t(0x3fffffff)) throw Error(TF_E_CAPACITY, "Native AMG: strength graph exceeds 32-bit capacity");
     double* rowmax = rowmax_workspace(c, n);
@@ -996,20 +1019,27 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
         symmetric = c.hflags[0] == 0 ? 1 : 0;
     }
     if (symmetricOut) *symmetricOut = symmetric;
+    require(!learned || symmetric, "learned affinities need a matrix with a symmetric pattern");
     if (symmetric) {
         int* cnt = c.wst<int>(kCount, size_t(n) + 1);
         int* scol = c.wst<int>(kKeys, size_t(nnz) + 1);
         double* sw = c.wst<double>(kVals, size_t(nnz) + 1);
         const int lanes = lanes_per_row(n, nnz);
+        if (learned) {
+            hipLaunchKernelGGL(k_learned_graph, grid1(n + 1), dim3(kBlock), 0, c.stream, n, rp, ci, learned, scol, sw,
+                               cnt);
+        }
+        else {
 #define TF_STRENGTH(G)                                                                                               \
     case G:                                                                                                          \
         hipLaunchKernelGGL(k_strength_sym<G>, grid_rows(int64_t(n) + 1, G), dim3(kBlock), 0, c.stream, n, rp, ci, v, \
                            rowmax, theta, scol, sw, cnt);                                                            \
         break;
-        switch (lanes) {
-            TF_STRENGTH(4) TF_STRENGTH(8) TF_STRENGTH(16) TF_STRENGTH(32)
-        }
+            switch (lanes) {
+                TF_STRENGTH(4) TF_STRENGTH(8) TF_STRENGTH(16) TF_STRENGTH(32)
+            }
 #undef TF_STRENGTH
+        }
         check(hipGetLastError(), "strength launch");
         DevCsr g;
         g.n = n;

@@ -375,6 +375,13 @@ template <class T>
 void block_spmm(tf_ctx& c, int n, const int* rp, const int* ci, const T* v, int k, const T* X, T* Y) {
     bspmm<T, kSpmv, kBNone>(c, n, k, rp, ci, v, X, Y, nullptr, nullptr, T(0), nullptr, nullptr);
 }
+template <class T>
+void block_jacobi(tf_ctx& c, int n, const int* rp, const int* ci, const T* v, const T* inv, T w, int k, const T* B,
+                  const T* X, T* Y) {
+    bspmm<T, kJacobi, kBNone>(c, n, k, rp, ci, v, X, Y, B, inv, w, nullptr, nullptr);
+}
+template void block_jacobi<double>(tf_ctx&, int, const int*, const int*, const double*, const double*, double, int,
+                                   const double*, const double*, double*);
 template void block_spmm<double>(tf_ctx&, int, const int*, const int*, const double*, int, const double*, double*);
 template void block_spmm<float>(tf_ctx&, int, const int*, const int*, const float*, int, const float*, float*);
 

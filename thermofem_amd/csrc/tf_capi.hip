@@ -107,6 +107,11 @@ struct tf_sell {
     float* v32 = nullptr;
 };
 
+struct tf_gnn {
+    tf_ctx* ctx = nullptr;
+    tf::GnnModel* model = nullptr;
+};
+
 struct tf_solver {
     tf_ctx* ctx = nullptr;
     tf::AmgEngine* amg = nullptr;
@@ -830,6 +835,108 @@ int tf_solver_apply_segmented(tf_solver* s, int segments, const int* host_offset
         check(hipEventRecord(ev.b, s->ctx->stream), "event record");
         check(hipEventSynchronize(ev.b), "event sync");
         s->eng->applySeconds = elapsed(ev.a, ev.b);
+    });
+}
+
+// ------------------------------------------------------- learned affinities
+
+int tf_gnn_param_count(void) { return GnnModel::param_count(); }
+
+int tf_gnn_create(int device, tf_gnn** out) {
+    return guard([&] {
+        require(out != nullptr, "null handle pointer");
+        auto h = std::make_unique<tf_gnn>();
+        h->ctx = make_ctx(device);
+        DeviceScope scope(device);
+        h->model = new GnnModel(h->ctx);
+        *out = h.release();
+    });
+}
+
+int tf_gnn_destroy(tf_gnn* g) {
+    return guard([&] {
+        if (!g) return;
+        {
+            DeviceScope scope(g->ctx->device);
+            delete g->model;
+        }
+        destroy_ctx(g->ctx);
+        delete g;
+    });
+}
+
+int tf_gnn_set_params(tf_gnn* g, const float* host, int count) {
+    return guard([&] {
+        require(g != nullptr && host != nullptr && count == GnnModel::param_count(), "parameter count mismatch");
+        DeviceScope scope(g->ctx->device);
+        g->model->set_params(host);
+    });
+}
+
+int tf_gnn_get_params(tf_gnn* g, float* host, int count) {
+    return guard([&] {
+        require(g != nullptr && host != nullptr && count == GnnModel::param_count(), "parameter count mismatch");
+        DeviceScope scope(g->ctx->device);
+        g->model->get_params(host);
+    });
+}
+
+int tf_gnn_infer(tf_gnn* g, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double* weights) {
+    return guard([&] {
+        require(g != nullptr && weights != nullptr, "null handle");
+        DeviceScope scope(g->ctx->device);
+        g->model->infer(n, nnz, rp, ci, v, weights);
+        g->ctx->sync();
+    });
+}
+
+int tf_gnn_loss_grad(tf_gnn* g, int n, int64_t nnz, const int* rp, const int* ci, const double* v,
+                     const float* target, double* loss, float* host_grad) {
+    return guard([&] {
+        require(g != nullptr && target != nullptr && loss != nullptr, "null handle");
+        DeviceScope scope(g->ctx->device);
+        *loss = g->model->loss_grad(n, nnz, rp, ci, v, target, host_grad);
+    });
+}
+
+int tf_gnn_train_step(tf_gnn* g, int n, int64_t nnz, const int* rp, const int* ci, const double* v,
+                      const float* target, float learning_rate, double* loss) {
+    return guard([&] {
+        require(g != nullptr && target != nullptr && loss != nullptr, "null handle");
+        require(std::isfinite(learning_rate) && learning_rate > 0, "learning rate must be positive");
+        DeviceScope scope(g->ctx->device);
+        *loss = g->model->train_step(n, nnz, rp, ci, v, target, learning_rate);
+    });
+}
+
+int tf_affinities(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v, int k, int sweeps,
+                  double omega, const double* X0, float* out) {
+    TF_CTX_GUARD(ctx, require(X0 != nullptr && out != nullptr, "null pointer");
+                 affinities(*ctx, n, nnz, rp, ci, v, k, sweeps, omega, X0, out));
+}
+
+int tf_amg_set_affinity_model(tf_amg* amg, tf_gnn* model) {
+    return guard([&] {
+        require(amg != nullptr, "null handle");
+        require(!model || model->ctx->device == amg->ctx->device, "the model must live on the AMG's device");
+        amg->eng->set_affinity_model(model ? model->model : nullptr);
+    });
+}
+
+int tf_amg_learned_info(tf_amg* amg, int* used, double* rho_learned, double* rho_plain) {
+    return guard([&] {
+        require(amg != nullptr, "null handle");
+        if (used) *used = amg->eng->learned_used() ? 1 : 0;
+        if (rho_learned) *rho_learned = amg->eng->rho_learned();
+        if (rho_plain) *rho_plain = amg->eng->rho_plain();
+    });
+}
+
+int tf_amg_convergence_estimate(tf_amg* amg, double* rho) {
+    return guard([&] {
+        require(amg != nullptr && rho != nullptr && amg->eng->ready(), "AMG requires setup");
+        DeviceScope scope(amg->ctx->device);
+        *rho = amg->eng->convergence_estimate();
     });
 }
 

@@ -374,13 +374,14 @@ AmgOptionsC amg_options_from(const tf_amg_options* o) {
         a.cycle = o->cycle;
         a.precision = o->level_precision;
         a.tail = o->coarse_tail;
+        a.learned = o->learned;
     }
     // Amg.cuh:66-75 (coarse_rows widened to 1..1024).
     require(a.maxLevels >= 1 && a.maxLevels <= 20 && a.coarseRows >= 1 && a.coarseRows <= 1024 && a.rounds >= 1 &&
                 a.rounds <= 8 && a.pre >= 0 && a.post >= 0 && a.coarseSweeps >= 1 && std::isfinite(a.theta) &&
                 a.theta >= 0 && a.theta <= 1 && std::isfinite(a.omega) && a.omega > 0 && a.omega < 2 &&
                 a.cycle >= 0 && a.cycle <= 2 && (a.precision == 32 || a.precision == 64) && a.tail >= 0 &&
-                a.tail <= 2,
+                a.tail <= 2 && a.learned >= 0 && a.learned <= 2,
             "Invalid AMG options");
     return a;
 }
@@ -430,7 +431,114 @@ void AmgEngine::release() {
     factored_ = singular_ = false;
 }
 
+namespace {
+// Deterministic pseudo-random start vector in [-1, 1) (SplitMix64 of the index).
+__global__ void k_hash_fill(int n, uint64_t seed, double* __restrict__ x) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    uint64_t z = seed + uint64_t(i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    x[i] = 2.0 * (double(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+}
+}  // namespace
+
+// Hierarchy choice with learned affinities: 1 builds on the network's
+// weights, 2 builds both hierarchies and keeps the learned one only when its
+// cycle makes PCG converge faster (convergence_estimate; the plain hierarchy
+// is the fallback).
 void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v, bool canonicalInput) {
+    learnedUsed_ = false;
+    rhoLearned_ = rhoPlain_ = -1.0;
+    if (opt_.learned == 0 || !gnn_) {
+        setup_hierarchy(n, nnz, rp, ci, v, canonicalInput, false);
+        return;
+    }
+    setup_hierarchy(n, nnz, rp, ci, v, canonicalInput, true);
+    if (opt_.learned == 1) {
+        learnedUsed_ = true;
+        return;
+    }
+    rhoLearned_ = convergence_estimate();
+    AmgOptionsC po = opt_;
+    po.learned = 0;
+    AmgEngine plain(c_, po);
+    plain.setup(n, nnz, rp, ci, v, canonicalInput);
+    rhoPlain_ = plain.convergence_estimate();
+    if (rhoLearned_ < rhoPlain_)
+        learnedUsed_ = true;
+    else
+        swap_hierarchy(plain);
+}
+
+void AmgEngine::swap_hierarchy(AmgEngine& o) {
+    (void)hipStreamSynchronize(c_->stream);
+    std::swap(levels_, o.levels_);
+    std::swap(visits_, o.visits_);
+    std::swap(stop_, o.stop_);
+    std::swap(coarseSolver_, o.coarseSolver_);
+    std::swap(tail_, o.tail_);
+    std::swap(tailLevels_, o.tailLevels_);
+    std::swap(tailOps_, o.tailOps_);
+    std::swap(tailImage_, o.tailImage_);
+    std::swap(tailImageBytes_, o.tailImageBytes_);
+    std::swap(tailPrograms_, o.tailPrograms_);
+    std::swap(factored_, o.factored_);
+    std::swap(singular_, o.singular_);
+    std::swap(lu_, o.lu_);
+    std::swap(piv_, o.piv_);
+    std::swap(inverse_, o.inverse_);
+}
+
+// Convergence of the cycle as a CG preconditioner: 10 PCG iterations on
+// A x = b from x = 0 with a fixed pseudo-random b, and the mean residual
+// reduction per iteration (|r_10| / |r_0|)^(1/10) (smaller is better; the
+// cycle alone need not contract as a stationary iteration).
+double AmgEngine::convergence_estimate() {
+    const DevCsr& A = levels_[0].native;
+    const int n = A.n;
+    double* r = dev_alloc<double>(*c_, size_t(n));
+    double* z = dev_alloc<double>(*c_, size_t(n));
+    double* p = dev_alloc<double>(*c_, size_t(n));
+    double* q = dev_alloc<double>(*c_, size_t(n));
+    hipLaunchKernelGGL(k_hash_fill, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c_->stream, n,
+                       uint64_t(0x5EEDull), r);
+    check(hipGetLastError(), "start vector launch");
+    auto dot = [&](const double* x, const double* y) {
+        dot_device<double>(*c_, n, x, y, c_->dscalars, 0);
+        check(hipMemcpyAsync(c_->hscalars, c_->dscalars, sizeof(double), hipMemcpyDeviceToHost, c_->stream),
+              "dot download");
+        c_->sync();
+        return c_->hscalars[0];
+    };
+    const double r0 = std::sqrt(dot(r, r));
+    apply(r, z);
+    convert<double, double>(*c_, size_t(n), z, p);
+    double rz = dot(r, z);
+    constexpr int kIts = 10;
+    double rk = r0;
+    for (int k = 0; k < kIts && rz > 0 && std::isfinite(rz); ++k) {
+        spmv<double>(*c_, n, A.nnz, A.rp, A.ci, A.v, p, q);
+        const double pq = dot(p, q);
+        if (!(pq > 0) || !std::isfinite(pq)) break;
+        const double alpha = rz / pq;
+        axpy<double>(*c_, n, -alpha, q, r);
+        rk = std::sqrt(dot(r, r));
+        apply(r, z);
+        const double rzNew = dot(r, z);
+        scal<double>(*c_, n, rzNew / rz, p);
+        axpy<double>(*c_, n, 1.0, z, p);
+        rz = rzNew;
+    }
+    c_->sync();
+    for (double* v : {r, z, p, q}) dev_free(*c_, v);
+    const double rho = r0 > 0 ? std::pow(rk / r0, 1.0 / kIts) : 0.0;
+    return std::isfinite(rho) ? rho : 1e300;
+}
+
+void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci, const double* v,
+                                bool canonicalInput, bool learned) {
     const bool f32 = opt_.precision == 32;
     const size_t es = f32 ? sizeof(float) : sizeof(double);
     std::vector<AmgLevel> next;
@@ -487,8 +595,25 @@ void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const do
             }
             // Galerkin products of a pattern-symmetric matrix are pattern-symmetric, so
             // the check runs on the finest level only.
-            DevCsr g = strength_graph(*c_, ln, native.nnz, native.rp, native.ci, native.v, opt_.theta, symmetric,
-                                      true, &symmetric);
+            double* lw = nullptr;
+            if (learned) {
+                // the network's weights of this level's entries (its own stream)
+                lw = dev_alloc<double>(*c_, size_t(native.nnz));
+                c_->sync();
+                gnn_->infer(ln, native.nnz, native.rp, native.ci, native.v, lw);
+                gnn_->context()->sync();
+            }
+            DevCsr g;
+            try {
+                g = strength_graph(*c_, ln, native.nnz, native.rp, native.ci, native.v, opt_.theta, symmetric, true,
+                                   &symmetric, lw);
+            }
+            catch (...) {
+                dev_free(*c_, lw);
+                throw;
+            }
+            c_->sync();
+            dev_free(*c_, lw);
             if (g.nnz == 0) {
                 free_csr(*c_, g);
                 stop = "empty strength graph";

@@ -15,7 +15,8 @@ struct AmgOptionsC {
     bool absorb = true;
     int cycle = 0;
     int precision = 64;
-    int tail = 0;  // 0 timed, 1 always, 2 never
+    int tail = 0;     // 0 timed, 1 always, 2 never
+    int learned = 0;  // 0 strength weights, 1 learned affinities, 2 learned where PCG converges faster with it
 };
 AmgOptionsC amg_options_from(const tf_amg_options* o);
 
@@ -84,6 +85,13 @@ public:
     const double* coarse_inverse() const { return factored_ ? inverse_ : nullptr; }
     // The multi-right-hand-side cycle over this hierarchy (made on first use).
     BlockAmg& block();
+    // Network whose edge weights replace the strength weights (not owned;
+    // used when options.learned != 0).
+    void set_affinity_model(GnnModel* m) { gnn_ = m; }
+    bool learned_used() const { return learnedUsed_; }
+    double rho_learned() const { return rhoLearned_; }
+    double rho_plain() const { return rhoPlain_; }
+    double convergence_estimate();
     int n() const { return levels_.empty() ? 0 : levels_[0].native.n; }
 
     std::vector<AmgLevel> levels_;
@@ -107,6 +115,9 @@ private:
     void build_tail();
     void tune_tail();
     void release();
+    void setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci, const double* v, bool canonicalInput,
+                         bool learned);
+    void swap_hierarchy(AmgEngine& o);
 
     // One-workgroup tail: first level it covers (-1: none), level table,
     // and the programs for every (cycle kind, zero start, pre-applied sweep).
@@ -127,6 +138,9 @@ private:
     int* piv_ = nullptr;
     double* inverse_ = nullptr;
     BlockAmg* block_ = nullptr;
+    GnnModel* gnn_ = nullptr;
+    bool learnedUsed_ = false;
+    double rhoLearned_ = -1.0, rhoPlain_ = -1.0;
 };
 
 // The cycle of an AmgEngine's hierarchy on k right-hand sides at once

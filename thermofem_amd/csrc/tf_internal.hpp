@@ -197,6 +197,39 @@ void jacobi_update(tf_ctx& c, int n, T w, const T* inv, const T* b, const T* pro
 // sums are the single-vector SpMV's (rounded products in stored order).
 template <class T>
 void block_spmm(tf_ctx& c, int n, const int* rp, const int* ci, const T* v, int k, const T* X, T* Y);
+// One weighted-Jacobi sweep on every column: Y = X + (w inv) (B - A X).
+template <class T>
+void block_jacobi(tf_ctx& c, int n, const int* rp, const int* ci, const T* v, const T* inv, T w, int k, const T* B,
+                  const T* X, T* Y);
+
+// Learned edge-affinity network (tf_gnn.hip).  Matrices must have a
+// symmetric pattern; weights are per stored entry (0 on the diagonal).
+class GnnModel {
+public:
+    explicit GnnModel(tf_ctx* ctx);
+    ~GnnModel();
+    GnnModel(const GnnModel&) = delete;
+    GnnModel& operator=(const GnnModel&) = delete;
+    static int param_count();
+    void set_params(const float* host);  // FP32 master weights; resets Adam
+    void get_params(float* host);
+    // out[nnz] (device): the network's weight of every stored entry
+    void infer(int n, int64_t nnz, const int* rp, const int* ci, const double* v, double* out);
+    // mean squared error against target[nnz] (device, per stored entry) and its gradient
+    double loss_grad(int n, int64_t nnz, const int* rp, const int* ci, const double* v, const float* target,
+                     float* hostGrad);
+    double train_step(int n, int64_t nnz, const int* rp, const int* ci, const double* v, const float* target,
+                      float lr);
+    tf_ctx* context() const;
+
+private:
+    struct Impl;
+    Impl* p_;
+};
+// Affinities of the stored entries after `sweeps` weighted-Jacobi sweeps on
+// A X = 0 from the n x k block X0 (device): out[nnz] (device).
+void affinities(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, int k, int sweeps,
+                double omega, const double* X0, float* out);
 template <class T>
 void axpy(tf_ctx& c, int n, T alpha, const T* x, T* y);
 template <class T>
@@ -291,7 +324,8 @@ bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const doubl
 // symmetric: -1 unknown (checked here), 0/1 known pattern symmetry.
 // prevalidated: the caller ran validate_rowmax with rowmax_workspace(c, n).
 DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta,
-                      int symmetric = -1, bool prevalidated = false, int* symmetricOut = nullptr);
+                      int symmetric = -1, bool prevalidated = false, int* symmetricOut = nullptr,
+                      const double* learned = nullptr);
 double* rowmax_workspace(tf_ctx& c, int n);
 void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal);
 // nnz < 0: read from rp[n].

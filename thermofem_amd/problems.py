@@ -264,6 +264,120 @@ def heat3d_batch(count=32, lo=16, hi=96, seed=3):
     return a, np.concatenate(rhs), off, edges
 
 
+def _q1_reference_stiffness():
+    """T[d, e, a, b] = integral over the unit cube of dN_a/dx_d * dN_b/dx_e for
+    the 8 trilinear shape functions (local node a = ax + 2 ay + 4 az); the
+    element matrix of a constant tensor K is sum_de K[d, e] T[d, e]."""
+    bits = np.array([[(a >> q) & 1 for q in range(3)] for a in range(8)])
+    sgn = np.where(bits == 1, 1.0, -1.0)
+    T = np.zeros((3, 3, 8, 8))
+    for d in range(3):
+        for e in range(3):
+            for a in range(8):
+                for b in range(8):
+                    prod = 1.0
+                    for q in range(3):
+                        same = bits[a, q] == bits[b, q]
+                        if q == d and q == e:
+                            f = 1.0 if same else -1.0          # int phi_a' phi_b'
+                        elif q == d:
+                            f = sgn[a, q] / 2.0                # int phi_a' phi_b
+                        elif q == e:
+                            f = sgn[b, q] / 2.0                # int phi_a phi_b'
+                        else:
+                            f = 1.0 / 3.0 if same else 1.0 / 6.0
+                        prod *= f
+                    T[d, e, a, b] = prod
+    return T
+
+
+def euler_rotation(alpha, beta, gamma):
+    """R = Rz(alpha) Ry(beta) Rz(gamma)."""
+    def rz(t):
+        c, s = np.cos(t), np.sin(t)
+        return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+    def ry(t):
+        c, s = np.cos(t), np.sin(t)
+        return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+    return rz(alpha) @ ry(beta) @ rz(gamma)
+
+
+def q1_diffusion(nc, tensors, block=16):
+    """Trilinear (Q1) finite elements for -div(K grad u) = f on the unit cube
+    with nc^3 cells, homogeneous Dirichlet on the boundary: unknowns are the
+    (nc-1)^3 interior nodes, 27-point rows with the cross terms of a full
+    tensor.  tensors[bz, by, bx] is the 3x3 SPD tensor of the block of
+    `block`^3 cells containing the cell.  Element matrices are the exact
+    integrals (no quadrature), scaled by 1/h (a uniform factor).  Node index
+    i + m (j + m k) with m = nc - 1 over the interior nodes, columns ascending.
+    Returns (rp, ci, v)."""
+    T = _q1_reference_stiffness()
+    nb = tensors.shape[0]
+    Ke = np.einsum("zyxde,deab->zyxab", tensors, T).reshape(-1, 8, 8)
+    c = np.arange(nc)
+    bz, by, bx = np.meshgrid(c // block, c // block, c // block, indexing="ij")
+    cell_block = (bz * nb + by) * nb + bx  # (nc, nc, nc)
+    m1 = nc + 1
+    S = np.zeros((m1, m1, m1, 27))
+    for a in range(8):
+        az, ay, ax = (a >> 2) & 1, (a >> 1) & 1, a & 1
+        for b in range(8):
+            dz, dy, dx = ((b >> 2) & 1) - az, ((b >> 1) & 1) - ay, (b & 1) - ax
+            slot = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)
+            S[az:az + nc, ay:ay + nc, ax:ax + nc, slot] += Ke[cell_block, a, b]
+    m = nc - 1
+    n = m ** 3
+    idx = np.arange(n)
+    ix, iy, iz = idx % m, (idx // m) % m, idx // (m * m)
+    cols = np.empty((n, 27), np.int64)
+    vals = np.empty((n, 27))
+    mask = np.empty((n, 27), bool)
+    for slot in range(27):
+        dz, dy, dx = slot // 9 - 1, (slot // 3) % 3 - 1, slot % 3 - 1
+        jx, jy, jz = ix + dx, iy + dy, iz + dz
+        ok = (jx >= 0) & (jx < m) & (jy >= 0) & (jy < m) & (jz >= 0) & (jz < m)
+        cols[:, slot] = np.where(ok, jx + m * (jy + m * jz), 0)
+        vals[:, slot] = S[iz + 1, iy + 1, ix + 1, slot]
+        mask[:, slot] = ok
+    return _compress(cols, vals, mask)
+
+
+def anisotropic_tensors(nb, seed, eps_lo=1e-4, eps_hi=1.0):
+    """nb^3 tensors R diag(1, eps, eps) R^T: Euler angles ~ U[0, 2 pi) from
+    seed (alpha, beta, gamma per block), eps log-uniform in [eps_lo, eps_hi]
+    from seed + 1."""
+    count = nb ** 3
+    ang = 2.0 * np.pi * uniform(seed, 3 * count).reshape(count, 3)
+    le = np.log(eps_lo) + (np.log(eps_hi) - np.log(eps_lo)) * uniform(seed + 1, count)
+    eps = np.exp(le)
+    out = np.empty((count, 3, 3))
+    for q in range(count):
+        R = euler_rotation(*ang[q])
+        out[q] = R @ np.diag([1.0, eps[q], eps[q]]) @ R.T
+    return out.reshape(nb, nb, nb, 3, 3)
+
+
+def anisotropic_diffusion(nc=128, seed=7, block=16):
+    """Rotated anisotropic diffusion: K = R diag(1, eps, eps) R^T per block of
+    16^3 cells (anisotropic_tensors), Q1 elements, (nc-1)^3 unknowns.
+    Returns ((rp, ci, v), b) with b ~ U[0,1) from seed + 2."""
+    nb = (nc + block - 1) // block
+    a = q1_diffusion(nc, anisotropic_tensors(nb, seed), block)
+    return a, uniform(seed + 2, (nc - 1) ** 3)
+
+
+def jump_diffusion(nc=128, seed=9, block=16):
+    """Four isotropic materials k in {1e-3, 1e-1, 1e1, 1e3} assigned to the
+    blocks of 16^3 cells from seed, Q1 elements.  Returns ((rp, ci, v), b)."""
+    nb = (nc + block - 1) // block
+    mat = np.floor(4 * uniform(seed, nb ** 3)).astype(np.int64)
+    k = 10.0 ** (-3.0 + 2.0 * mat)
+    tensors = (k[:, None, None] * np.eye(3)[None]).reshape(nb, nb, nb, 3, 3)
+    a = q1_diffusion(nc, tensors, block)
+    return a, uniform(seed + 2, (nc - 1) ** 3)
+
+
 def spmv_bytes(n, nnz, value_size):
     """Algorithmic bytes of one CSR SpMV: values+columns, offsets, x once, y."""
     return nnz * (value_size + 4) + 4 * (n + 1) + 2 * n * value_size

@@ -52,12 +52,13 @@ class AmgOptions:
     cycle: Cycle = Cycle.V
     level_precision: int = 64  # 32: FP32 cycle under the FP64 caller
     coarse_tail: int = 0  # coarse levels as one one-workgroup program: 0 timed at setup, 1 always, 2 never
+    learned: int = 0  # matching weights: 0 strength, 1 learned affinities, 2 learned where PCG converges faster with it
 
     def c(self):
         return _lib.AmgOptions(self.max_levels, self.coarse_rows, self.matching_rounds, self.pre_sweeps,
                                self.post_sweeps, self.coarse_sweeps, self.jacobi_weight, self.strength_threshold,
                                int(self.absorb_singletons), int(self.cycle), self.level_precision,
-                               self.coarse_tail)
+                               self.coarse_tail, self.learned)
 
 
 @dataclass
@@ -116,6 +117,23 @@ class _AmgView:
         v = (C.c_int64 * self.level_count())()
         check(lib().tf_amg_visits(self._h, v))
         return list(v)
+
+    def set_affinity_model(self, model):
+        """Network (gnn.AffinityModel, or None) whose weights the matching uses
+        when the options ask for learned affinities; keep it alive meanwhile."""
+        self._model = model
+        check(lib().tf_amg_set_affinity_model(self._h, model.handle if model is not None else None))
+
+    def learned_info(self):
+        """(learned hierarchy in use, its contraction factor, the plain one's); -1: not measured."""
+        u, a, b = C.c_int(), C.c_double(), C.c_double()
+        check(lib().tf_amg_learned_info(self._h, C.byref(u), C.byref(a), C.byref(b)))
+        return bool(u.value), a.value, b.value
+
+    def convergence_estimate(self):
+        r = C.c_double()
+        check(lib().tf_amg_convergence_estimate(self._h, C.byref(r)))
+        return r.value
 
     def level_matrix(self, d):
         r, z, _ = self.level_statistics()[d]
