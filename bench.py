@@ -49,12 +49,13 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch each iteration's kernels without hipGraph")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the level-0 SpMV launches and a copy kernel of known size (for rocprofv3 --pmc)")
-    ap.add_argument("--workload", choices=["heat3d", "convdiff", "fem", "batch"], default="heat3d",
+    ap.add_argument("--workload", choices=["heat3d", "convdiff", "fem", "batch", "learned"], default="heat3d",
                     help="convdiff: 96^3 upwind convection-diffusion, BiCGStab with V/W/F cycles, FP64 and FP32 "
                          "levels, cell Peclet 10/30/100; fem: P1 tets on 64^3 cells, 64 right-hand sides in one "
                          "block CG vs 64 single solves, and the k=64 SpMM; batch: 32 independent heat problems "
-                         "(edges U{16..96}) in one segmented solve vs 32 separate solves (each one JSON line of "
-                         "its own)")
+                         "(edges U{16..96}) in one segmented solve vs 32 separate solves; learned: held-out "
+                         "rotated anisotropic and jump-coefficient Q1 problems, strength vs learned affinities "
+                         "(each one JSON line of its own)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver runs on a single "
@@ -75,6 +76,9 @@ def main():
         return
     if args.workload == "batch":
         batch(args)
+        return
+    if args.workload == "learned":
+        learned(args)
         return
 
     nx = args.nx
@@ -457,6 +461,69 @@ def batch(args):
                                              "AMG FP32 V(2,2)", "n": n, "nnz": nnz, "edges": edges.tolist()},
                       "value": seg["s"], "segmented": seg, "separate": single,
                       "speedup": round(single["s"] / seg["s"], 3)}), flush=True)
+
+
+def learned(args):
+    """Held-out Q1 diffusion problems on nc^3 cells (default 128: 127^3
+    unknowns, 27-point rows): rotated anisotropic tensors diag(1, eps, eps)
+    per 16^3-cell block (seeds 7 and 8) and a four-material jump cube (seed
+    9).  AMG-PCG (FP32 V(2,2) cycle, FP64 CG, Jacobi omega 0.45 for the
+    anisotropic operators whose D^-1 A reaches ~3.9, 2/3 otherwise) to 1e-8
+    with the strength weights (plain), the committed network's weights
+    (learned = 1) and the learned hierarchy only where its PCG estimate wins
+    (learned = 2): setup and solve seconds, iterations, which hierarchy the
+    fallback kept."""
+    import numpy as np
+    from thermofem_amd import problems as P
+    from thermofem_amd.device import DeviceArray, DeviceCsr
+    from thermofem_amd.gnn import AffinityModel
+    from thermofem_amd.solver import AmgOptions, Cg, KrylovOptions
+
+    nc = args.nx
+    model = AffinityModel.trained()
+    cases = [("anisotropic seed 7", lambda: P.anisotropic_diffusion(nc, seed=7), 0.45),
+             ("anisotropic seed 8", lambda: P.anisotropic_diffusion(nc, seed=8), 0.45),
+             ("four-material jump seed 9", lambda: P.jump_diffusion(nc, seed=9), 2.0 / 3.0)]
+    rows, fallbacks = [], 0
+    for name, make, omega in cases:
+        log(f"generating {name} {nc}^3 cells")
+        (rp, ci, v), b = make()
+        n, nnz = len(rp) - 1, len(ci)
+        A = DeviceCsr(rp, ci, v)
+        db = DeviceArray.from_host(b)
+        dx = DeviceArray(n, np.float64)
+        for mode in (0, 1, 2):
+            s = Cg(KrylovOptions(relative_tolerance=1e-8, max_iterations=3000),
+                   AmgOptions(coarse_rows=256, pre_sweeps=2, post_sweeps=2, jacobi_weight=omega, level_precision=32,
+                              learned=mode))
+            view = s.amg()
+            if mode:
+                view.set_affinity_model(model)
+            times = []
+            for it in range(args.warmup + args.steps):
+                dx.upload(np.zeros(n))
+                t0 = time.perf_counter()
+                s.setup_device(n, nnz, *A.args())
+                res = s.apply_device(db, dx)
+                t1 = time.perf_counter()
+                if it >= args.warmup:
+                    times.append((t1 - t0, *s.timings()))
+            used, rl, rpl = view.learned_info()
+            fallbacks += mode == 2 and not used
+            row = {"problem": name, "learned": mode, "s": round(statistics.median(t[0] for t in times), 6),
+                   "setup_s": round(statistics.median(t[1] for t in times), 6),
+                   "solve_s": round(statistics.median(t[2] for t in times), 6), "iterations": res.iterations,
+                   "status": res.status.name, "learned_hierarchy_used": used, "pcg_estimate_learned": rl,
+                   "pcg_estimate_plain": rpl}
+            log(json.dumps(row))
+            rows.append(row)
+            del s
+    print(json.dumps({"metric": f"AMG-PCG setup+solve s to 1e-8, Q1 diffusion {nc}^3 cells, strength vs learned "
+                                "affinities", "unit": "s", "higher_is_better": False, "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "dtype": "f64+f32 (network bf16/fp32)",
+                      "data": "synthetic", "config": {"workload": f"held-out Q1 anisotropic/jump {nc}^3 cells",
+                                                      "n": (nc - 1) ** 3},
+                      "fallback_rate": fallbacks / len(cases), "runs": rows}), flush=True)
 
 
 def csr_matvec(rp, ci, v, x):

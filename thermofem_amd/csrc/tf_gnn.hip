@@ -474,6 +474,7 @@ struct GnnModel::Impl {
     float *xe = nullptr, *z = nullptr, *w = nullptr, *t = nullptr, *ds = nullptr;
     float *dE = nullptr, *dE2 = nullptr, *dN = nullptr, *dN2 = nullptr, *dNu = nullptr;
     std::vector<void*> owned;
+    bool capTrain = false;  // training (gradient) buffers allocated
 
     float* falloc(int64_t count) {
         float* p = dev_alloc<float>(*c, size_t(std::max<int64_t>(count, 1)));
@@ -485,10 +486,12 @@ struct GnnModel::Impl {
         for (void* p : owned) dev_free(*c, p);
         owned.clear();
         capN = capE = 0;
+        capTrain = false;
     }
-    void reserve(int nn, int64_t ee) {
-        if (nn <= capN && ee <= capE) return;
+    void reserve(int nn, int64_t ee, bool training) {
+        if (nn <= capN && ee <= capE && (capTrain || !training)) return;
         free_graph();
+        capTrain = training;
         capN = std::max<int64_t>(nn, 1);
         capE = std::max<int64_t>(ee, 1);
         auto ialloc = [&](int64_t count) {
@@ -521,6 +524,7 @@ struct GnnModel::Impl {
         xe = falloc(capE * (2 * H + FE));
         z = falloc(capE * H);
         w = falloc(capE);
+        if (!training) return;  // gradients only for training
         t = falloc(capE);
         ds = falloc(capE);
         dE = falloc(capE * (2 * H + FE));
@@ -530,7 +534,7 @@ struct GnnModel::Impl {
         dNu = falloc(capN * H);
     }
 
-    void graph(int nn, int64_t nnz, const int* rp, const int* ci, const double* v) {
+    void graph(int nn, int64_t nnz, const int* rp, const int* ci, const double* v, bool training) {
         require(nn >= 1, "the network needs a nonempty matrix");
         // edge count
         int* tmp = dev_alloc<int>(*c, size_t(nn) + 1);
@@ -540,7 +544,7 @@ struct GnnModel::Impl {
         scan_excl(*c, nn + 1, tmp, trp);
         const int64_t ee = c->read_int(trp + nn);
         (void)nnz;
-        reserve(nn, ee);
+        reserve(nn, ee, training);
         check(hipMemcpyAsync(erp, trp, (size_t(nn) + 1) * sizeof(int), hipMemcpyDeviceToDevice, c->stream),
               "edge offsets copy");
         c->sync();
@@ -695,7 +699,7 @@ void GnnModel::get_params(float* host) {
 }
 
 void GnnModel::infer(int n, int64_t nnz, const int* rp, const int* ci, const double* v, double* out) {
-    p_->graph(n, nnz, rp, ci, v);
+    p_->graph(n, nnz, rp, ci, v, false);
     p_->forward(nullptr);
     hipLaunchKernelGGL(k_to_positions, dim3(blocks_of(n)), dim3(kBlock), 0, p_->c->stream, n, rp, ci, p_->erp, p_->w,
                        out);
@@ -705,7 +709,7 @@ void GnnModel::infer(int n, int64_t nnz, const int* rp, const int* ci, const dou
 double GnnModel::loss_grad(int n, int64_t nnz, const int* rp, const int* ci, const double* v, const float* target,
                            float* hostGrad) {
     require(nnz >= 0, "negative nonzero count");
-    p_->graph(n, nnz, rp, ci, v);
+    p_->graph(n, nnz, rp, ci, v, true);
     hipLaunchKernelGGL(k_from_positions, dim3(blocks_of(n)), dim3(kBlock), 0, p_->c->stream, n, rp, ci, p_->erp, target,
                        p_->t);
     require(p_->E > 0 && p_->E <= (int64_t(1) << 22) * kBlock, "training needs 1 .. 2^30 edges");
