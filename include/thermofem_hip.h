@@ -87,6 +87,34 @@ int tf_spmm_csr_f64(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* c
 int tf_spmm_csr_f32(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const float* v, int k,
                     const float* X, float* Y);
 
+/* ---- Remaining linalg functions of Operations.cuh:160-525 (cuBLAS / cuSPARSE in Backend.cu) ---- */
+/* sum |x_i| (norm1, Operations.cuh:251-261) */
+int tf_asum_f64(tf_ctx* ctx, int n, const double* x, double* host_result);
+int tf_asum_f32(tf_ctx* ctx, int n, const float* x, float* host_result);
+/* At (cols x rows) = A^T for row-major A (rows x cols) (transpose, Operations.cuh:309-321) */
+int tf_dense_transpose_f64(tf_ctx* ctx, int rows, int cols, const double* A, double* At);
+int tf_dense_transpose_f32(tf_ctx* ctx, int rows, int cols, const float* A, float* At);
+/* y = A x, A row-major rows x cols (Operations.cuh:325-344) */
+int tf_gemv_f64(tf_ctx* ctx, int rows, int cols, const double* A, const double* x, double* y);
+int tf_gemv_f32(tf_ctx* ctx, int rows, int cols, const float* A, const float* x, float* y);
+/* C = A B, row-major m x k and k x n (Operations.cuh:347-367) */
+int tf_gemm_f64(tf_ctx* ctx, int m, int n, int k, const double* A, const double* B, double* C);
+int tf_gemm_f32(tf_ctx* ctx, int m, int n, int k, const float* A, const float* B, float* C);
+/* Sparse add A + beta B (Operations.cuh:449-467), transpose (:477-491) and
+ * product A B (:494-513) of CSR operands (any column order, duplicates
+ * allowed): results are canonical (columns ascending, duplicates summed in
+ * candidate order: A's entries then B's for the sum, a_ik b_kj by k in A's
+ * stored order for the product).  out_rp has result rows + 1 entries in
+ * caller memory; out_ci / out_v are library allocations (tf_free). */
+int tf_csr_add_f64(tf_ctx* ctx, int rows, int cols, int64_t nnz_a, const int* rp_a, const int* ci_a, const double* v_a,
+                   int64_t nnz_b, const int* rp_b, const int* ci_b, const double* v_b, double beta, int* out_rp,
+                   int** out_ci, double** out_v, int64_t* out_nnz);
+int tf_csr_transpose_f64(tf_ctx* ctx, int rows, int cols, int64_t nnz, const int* rp, const int* ci, const double* v,
+                         int* out_rp, int** out_ci, double** out_v);
+int tf_csr_multiply_f64(tf_ctx* ctx, int rows_a, int cols_a, int cols_b, int64_t nnz_a, const int* rp_a,
+                        const int* ci_a, const double* v_a, int64_t nnz_b, const int* rp_b, const int* ci_b,
+                        const double* v_b, int* out_rp, int** out_ci, double** out_v, int64_t* out_nnz);
+
 /* ---- Jacobi: update kernel IterationContext.cu:26-32 and one sweep of Jacobi.cuh:60-74 ---- */
 /* x += w * inv * (b - product) */
 int tf_jacobi_update_f64(tf_ctx* ctx, int n, double w, const double* inv, const double* b, const double* product,

@@ -885,4 +885,198 @@ TEST_CASE(engineAndFactoryHierarchiesAgree, true) {
     EXPECT_LE(diff, 1e-12 * norm);
 }
 
+// ---- linalg free functions and operators (linear_algebra_test.cpp) ----
+
+namespace {
+namespace la = thermofem::linalg;
+
+template <class T>
+void expectBuffer(const Buffer<T>& b, std::initializer_list<T> expected) {
+    const auto actual = b.hostCopy();
+    ASSERT_EQ(actual.size(), expected.size());
+    const double tolerance = std::same_as<T, float> ? 1e-5 : 1e-12;
+    std::size_t i = 0;
+    for (const auto value : expected) EXPECT_NEAR(double(actual[i++]), double(value), tolerance);
+}
+template <class T>
+void expectVector(const Vector<T>& v, std::initializer_list<T> expected) {
+    expectBuffer(v.values(), expected);
+}
+// the mathematical matrix of a CSR (duplicates summed), row-major
+template <class T, class I>
+void expectSparse(const CsrMatrix<T, I>& m, std::initializer_list<T> expected) {
+    const auto rp = m.rowOffsets().hostCopy();
+    const auto ci = m.columnIndices().hostCopy();
+    const auto v = m.values().hostCopy();
+    std::vector<double> dense(m.rowCount() * m.columnCount(), 0.0);
+    for (std::size_t r = 0; r < m.rowCount(); ++r)
+        for (auto j = rp[r]; j < rp[r + 1]; ++j) dense[r * m.columnCount() + std::size_t(ci[std::size_t(j)])] += v[std::size_t(j)];
+    ASSERT_EQ(dense.size(), expected.size());
+    const double tolerance = std::same_as<T, float> ? 1e-5 : 1e-12;
+    std::size_t i = 0;
+    for (const auto value : expected) EXPECT_NEAR(dense[i++], double(value), tolerance);
+}
+
+// linear_algebra_test.cpp:97-131
+template <class T>
+void run_vectorArithmetic() {
+    for (const auto& ls : strategies())
+        for (const auto& rs : strategies()) {
+            Vector<T> x(std::span<const T>(std::vector<T>{1, -2, 3}), ls), y(std::span<const T>(std::vector<T>{4, 5, -6}), rs);
+            auto sum = x + y;
+            EXPECT_EQ(sum.memoryStrategy().index(), ls.index());
+            expectVector<T>(sum, {5, 3, -3});
+            expectVector<T>(x - y, {-3, -7, 9});
+            expectVector<T>(-x, {-1, 2, -3});
+            expectVector<T>(+x, {1, -2, 3});
+            expectVector<T>(T(2) * x, {2, -4, 6});
+            expectVector<T>(x * T(2), {2, -4, 6});
+            expectVector<T>(x / T(2), {T(0.5), -1, T(1.5)});
+            EXPECT_NEAR(double(x * y), -24.0, 1e-5);
+            EXPECT_NEAR(double(la::dot(x, y)), -24.0, 1e-5);
+            EXPECT_NEAR(double(la::norm1(x)), 6.0, 1e-5);
+            EXPECT_NEAR(double(la::norm2(x)), std::sqrt(14.0), 1e-5);
+            expectVector<T>(la::axpy(T(2), x, y), {6, 1, 0});
+            auto copy = x;
+            copy += copy;
+            copy -= x;
+            copy *= T(2);
+            copy /= T(2);
+            expectVector<T>(copy, {1, -2, 3});
+            EXPECT_EQ(copy.memoryStrategy().index(), ls.index());
+            EXPECT_THROW(x / T(0), std::domain_error);
+            expectVector<T>(x, {1, -2, 3});
+        }
+}
+
+// linear_algebra_test.cpp:133-170
+template <class T>
+void run_denseArithmetic() {
+    for (const auto& strategy : strategies()) {
+        DenseMatrix<T> a(2, 3, strategy), b(3, 2, PinnedMemory{});
+        a.copyFromHost(std::vector<T>{1, 2, 3, 4, 5, 6});
+        b.copyFromHost(std::vector<T>{7, 8, 9, 10, 11, 12});
+        Vector<T> x(std::span<const T>(std::vector<T>{1, 2, 3}), PinnedMemory{});
+        expectVector<T>(a * x, {14, 32});
+        auto product = a * b;
+        EXPECT_EQ(product.rowCount(), 2u);
+        EXPECT_EQ(product.columnCount(), 2u);
+        EXPECT_EQ(product.values().memoryStrategy().index(), strategy.index());
+        expectBuffer<T>(product.values(), {58, 64, 139, 154});
+        auto transposed = la::transpose(a);
+        EXPECT_EQ(transposed.rowCount(), 3u);
+        EXPECT_EQ(transposed.columnCount(), 2u);
+        expectBuffer<T>(transposed.values(), {1, 4, 2, 5, 3, 6});
+        expectBuffer<T>((a + a).values(), {2, 4, 6, 8, 10, 12});
+        expectBuffer<T>((a - a).values(), {0, 0, 0, 0, 0, 0});
+        expectBuffer<T>((-a).values(), {-1, -2, -3, -4, -5, -6});
+        auto copy = a;
+        copy += a;
+        copy -= a;
+        copy *= T(2);
+        copy /= T(2);
+        expectBuffer<T>(copy.values(), {1, 2, 3, 4, 5, 6});
+        copy *= b;
+        expectBuffer<T>(copy.values(), {58, 64, 139, 154});
+        EXPECT_NEAR(double(la::frobeniusNorm(a)), std::sqrt(91.0), 1e-5);
+        EXPECT_THROW(a / T(0), std::domain_error);
+    }
+}
+
+// linear_algebra_test.cpp:176-220 (duplicate column 2 in row 0 of a)
+template <class T, class I>
+void checkSparse(MemoryStrategy strategy) {
+    CsrMatrix<T, I> a(2, 3, std::vector<I>{0, 3, 4}, std::vector<I>{2, 0, 2, 1}, std::vector<T>{1, 2, 3, 5}, strategy);
+    CsrMatrix<T, I> b(3, 2, std::vector<I>{0, 1, 2, 4}, std::vector<I>{0, 1, 1, 0}, std::vector<T>{1, 2, 4, 3},
+                      PinnedMemory{});
+    Vector<T> x(std::span<const T>(std::vector<T>{1, 2, 3}), PinnedMemory{});
+    expectVector<T>(a * x, {14, 10});
+    DenseMatrix<T> dense(3, 2, PinnedMemory{});
+    dense.copyFromHost(std::vector<T>{1, 0, 0, 2, 3, 4});
+    expectBuffer<T>((a * dense).values(), {14, 16, 0, 10});
+    auto product = a * b;
+    expectSparse<T, I>(product, {14, 16, 0, 10});
+    EXPECT_EQ(product.values().memoryStrategy().index(), strategy.index());
+    expectSparse<T, I>(a + a, {4, 0, 8, 0, 10, 0});
+    expectSparse<T, I>(a - a, {0, 0, 0, 0, 0, 0});
+    auto transposed = la::transpose(a);
+    expectSparse<T, I>(transposed, {2, 0, 0, 5, 4, 0});
+    expectSparse<T, I>(-a, {-2, 0, -4, 0, -5, 0});
+    expectSparse<T, I>(a * T(2), {4, 0, 8, 0, 10, 0});
+    EXPECT_NEAR(double(la::frobeniusNorm(a)), std::sqrt(45.0), 1e-5);
+    auto copy = a;
+    copy += copy;
+    copy -= a;
+    copy *= T(2);
+    copy /= T(2);
+    expectSparse<T, I>(copy, {2, 0, 4, 0, 5, 0});
+    copy *= b;
+    expectSparse<T, I>(copy, {14, 16, 0, 10});
+    EXPECT_THROW(a / T(0), std::domain_error);
+}
+
+// linear_algebra_test.cpp:222-254
+template <class T>
+void run_zeroDimensions() {
+    DenseMatrix<T> a(2, 0, PinnedMemory{}), b(0, 3);
+    expectBuffer<T>((a * b).values(), {0, 0, 0, 0, 0, 0});
+    Vector<T> empty;
+    expectVector<T>(a * empty, {0, 0});
+    EXPECT_EQ(double(la::dot(empty, empty)), 0.0);
+    EXPECT_EQ(double(la::norm1(empty)), 0.0);
+    EXPECT_EQ(double(la::norm2(empty)), 0.0);
+    EXPECT_EQ(la::transpose(a).rowCount(), 0u);
+    CsrMatrix<T> sparse(2, 3, std::vector<int>{0, 0, 0}, std::vector<int>{}, std::vector<T>{}, PinnedMemory{});
+    Vector<T> x(std::span<const T>(std::vector<T>{1, 2, 3}));
+    expectVector<T>(sparse * x, {0, 0});
+    expectSparse<T, int>(sparse + sparse, {0, 0, 0, 0, 0, 0});
+    expectSparse<T, int>(la::transpose(sparse), {0, 0, 0, 0, 0, 0});
+    CsrMatrix<T> right(3, 2, std::vector<int>{0, 0, 0, 0}, std::vector<int>{}, std::vector<T>{});
+    expectSparse<T, int>(sparse * right, {0, 0, 0, 0});
+    CsrMatrix<T> defaultEmpty;
+    EXPECT_EQ((defaultEmpty + defaultEmpty).rowCount(), 0u);
+    EXPECT_EQ(la::transpose(defaultEmpty).rowCount(), 0u);
+    CsrMatrix<T> one(2, 3, std::vector<int>{0, 1, 1}, std::vector<int>{1}, std::vector<T>{4});
+    expectSparse<T, int>(sparse + one, {0, 4, 0, 0, 0, 0});
+    expectSparse<T, int>(one + sparse, {0, 4, 0, 0, 0, 0});
+    // two nonzero operands whose product has no structural nonzero
+    CsrMatrix<T> disjoint(3, 2, std::vector<int>{0, 1, 1, 1}, std::vector<int>{0}, std::vector<T>{2});
+    expectSparse<T, int>(one * disjoint, {0, 0, 0, 0});
+}
+}  // namespace
+
+TEST_CASE(vectorArithmetic, true) {
+    run_vectorArithmetic<float>();
+    run_vectorArithmetic<double>();
+}
+TEST_CASE(denseArithmetic, true) {
+    run_denseArithmetic<float>();
+    run_denseArithmetic<double>();
+}
+TEST_CASE(sparseArithmetic, true) {
+    for (const auto& s : strategies()) {
+        checkSparse<float, std::int32_t>(s);
+        checkSparse<float, std::int64_t>(s);
+        checkSparse<double, std::int32_t>(s);
+        checkSparse<double, std::int64_t>(s);
+    }
+}
+TEST_CASE(zeroDimensions, true) {
+    run_zeroDimensions<float>();
+    run_zeroDimensions<double>();
+}
+// linear_algebra_test.cpp:256-269 (the matrices are allocated on the device)
+TEST_CASE(linalgInvalidShapes, true) {
+    Vector<double> x;
+    EXPECT_THROW(x / 0.0, std::domain_error);
+    DenseMatrix<double> a(2, 0), b(3, 0);
+    EXPECT_THROW(a + b, std::invalid_argument);
+    EXPECT_THROW(a * b, std::invalid_argument);
+    EXPECT_THROW(b * DenseMatrix<double>(1, 0), std::invalid_argument);
+    EXPECT_THROW(la::detail::librarySize(static_cast<std::size_t>(std::numeric_limits<int>::max()) + 1),
+                 std::length_error);
+    CsrMatrix<double> sparse;
+    EXPECT_THROW(sparse * b, std::invalid_argument);
+}
+
 int main(int argc, char** argv) { return harness::run(argc, argv); }

@@ -994,6 +994,11 @@ DevCsr canonicalize(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci,
     return gather_merge(c, n, nullptr, nullptr, nullptr, rp, ci, v, false, false);
 }
 
+DevCsr merge_rows(tf_ctx& c, int rows, int64_t nnz, const int* rp, const int* ci, const double* v) {
+    require(rows >= 1 && nnz >= 1 && nnz <= 0x3fffffff, "row merge needs rows >= 1 and 1..2^30 entries");
+    return gather_merge(c, rows, nullptr, nullptr, nullptr, rp, ci, v, false, false);
+}
+
 bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax) {
     c.clear_flags(1);
     hipLaunchKernelGGL(k_validate, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, v, c.dflags, rowmax);

@@ -440,6 +440,75 @@ int tf_csr_canonicalize(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const in
     });
 }
 
+namespace {
+// Hands a library-made CSR to the caller: offsets into caller memory, columns
+// and values as library allocations (released with tf_free).
+void hand_out(tf_ctx* ctx, DevCsr& a, int* out_rp, int** out_ci, double** out_v, int64_t* out_nnz) {
+    check(hipMemcpyAsync(out_rp, a.rp, (size_t(a.n) + 1) * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream),
+          "result offsets");
+    ctx->sync();
+    dev_free(*ctx, a.rp);
+    *out_ci = a.ci;
+    *out_v = a.v;
+    if (out_nnz) *out_nnz = a.nnz;
+}
+}  // namespace
+
+int tf_csr_add_f64(tf_ctx* ctx, int rows, int cols, int64_t nnz_a, const int* rp_a, const int* ci_a, const double* v_a,
+                   int64_t nnz_b, const int* rp_b, const int* ci_b, const double* v_b, double beta, int* out_rp,
+                   int** out_ci, double** out_v, int64_t* out_nnz) {
+    TF_CTX_GUARD(ctx, {
+        require(rows >= 1 && cols >= 1 && out_rp && out_ci && out_v && out_nnz, "sparse add needs a nonempty shape");
+        require(csr_structure_ok(*ctx, rows, cols, nnz_a, rp_a, ci_a) &&
+                    csr_structure_ok(*ctx, rows, cols, nnz_b, rp_b, ci_b),
+                "invalid CSR operand");
+        DevCsr r = csr_add(*ctx, rows, rp_a, ci_a, v_a, rp_b, ci_b, v_b, beta);
+        hand_out(ctx, r, out_rp, out_ci, out_v, out_nnz);
+    });
+}
+
+int tf_csr_transpose_f64(tf_ctx* ctx, int rows, int cols, int64_t nnz, const int* rp, const int* ci, const double* v,
+                         int* out_rp, int** out_ci, double** out_v) {
+    TF_CTX_GUARD(ctx, {
+        require(rows >= 1 && cols >= 1 && out_rp && out_ci && out_v, "sparse transpose needs a nonempty shape");
+        require(csr_structure_ok(*ctx, rows, cols, nnz, rp, ci), "invalid CSR operand");
+        DevCsr r = csr_transpose(*ctx, rows, cols, nnz, rp, ci, v);
+        hand_out(ctx, r, out_rp, out_ci, out_v, nullptr);
+    });
+}
+
+int tf_csr_multiply_f64(tf_ctx* ctx, int rows_a, int cols_a, int cols_b, int64_t nnz_a, const int* rp_a,
+                        const int* ci_a, const double* v_a, int64_t nnz_b, const int* rp_b, const int* ci_b,
+                        const double* v_b, int* out_rp, int** out_ci, double** out_v, int64_t* out_nnz) {
+    TF_CTX_GUARD(ctx, {
+        require(rows_a >= 1 && cols_a >= 1 && cols_b >= 1 && out_rp && out_ci && out_v && out_nnz,
+                "sparse product needs nonempty shapes");
+        require(csr_structure_ok(*ctx, rows_a, cols_a, nnz_a, rp_a, ci_a) &&
+                    csr_structure_ok(*ctx, cols_a, cols_b, nnz_b, rp_b, ci_b),
+                "invalid CSR operand");
+        DevCsr r = csr_multiply(*ctx, rows_a, rp_a, ci_a, v_a, rp_b, ci_b, v_b);
+        hand_out(ctx, r, out_rp, out_ci, out_v, out_nnz);
+    });
+}
+
+#define TF_DENSE_ABI(T, SUF)                                                                                       \
+    int tf_asum_##SUF(tf_ctx* ctx, int n, const T* x, T* host_result) {                                            \
+        TF_CTX_GUARD(ctx, require(host_result != nullptr, "null result"); *host_result = T(asum<T>(*ctx, n, x))); \
+    }                                                                                                              \
+    int tf_dense_transpose_##SUF(tf_ctx* ctx, int rows, int cols, const T* A, T* At) {                             \
+        TF_CTX_GUARD(ctx, require(rows >= 0 && cols >= 0 && (A != At || rows * cols == 0), "transpose shape or alias"); \
+                     dense_transpose<T>(*ctx, rows, cols, A, At));                                                 \
+    }                                                                                                              \
+    int tf_gemv_##SUF(tf_ctx* ctx, int rows, int cols, const T* A, const T* x, T* y) {                             \
+        TF_CTX_GUARD(ctx, require(rows >= 0 && cols >= 0, "gemv shape"); gemv<T>(*ctx, rows, cols, A, x, y));     \
+    }                                                                                                              \
+    int tf_gemm_##SUF(tf_ctx* ctx, int m, int n, int k, const T* A, const T* B, T* C) {                            \
+        TF_CTX_GUARD(ctx, require(m >= 0 && n >= 0 && k >= 0, "gemm shape"); gemm<T>(*ctx, m, n, k, A, B, C));    \
+    }
+TF_DENSE_ABI(double, f64)
+TF_DENSE_ABI(float, f32)
+#undef TF_DENSE_ABI
+
 int tf_agg_validate_rowmax(tf_ctx* ctx, int n, const int* rp, const int* ci, const double* v, double* rowmax,
                            int* host_bad) {
     TF_CTX_GUARD(ctx, *host_bad = validate_rowmax(*ctx, n, rp, ci, v, rowmax) ? 0 : 1);

@@ -612,8 +612,10 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
                 dev_free(*c_, lw);
                 throw;
             }
-            c_->sync();
-            dev_free(*c_, lw);
+            if (lw) {
+                c_->sync();
+                dev_free(*c_, lw);
+            }
             if (g.nnz == 0) {
                 free_csr(*c_, g);
                 stop = "empty strength graph";

@@ -319,6 +319,22 @@ inline void dev_free(tf_ctx& c, void* p) { c.pool.put(p); }
 int csr_check(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v);
 bool csr_structure_ok(tf_ctx& c, int rows, int cols, int64_t nnz, const int* rp, const int* ci);
 DevCsr canonicalize(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v);
+// Rows of any column range sorted by column, duplicates summed in stored order.
+DevCsr merge_rows(tf_ctx& c, int rows, int64_t nnz, const int* rp, const int* ci, const double* v);
+// Level-1 API of Operations.cuh (tf_dense.hip).
+template <class T>
+double asum(tf_ctx& c, int n, const T* x);
+template <class T>
+void dense_transpose(tf_ctx& c, int rows, int cols, const T* A, T* At);
+template <class T>
+void gemv(tf_ctx& c, int rows, int cols, const T* A, const T* x, T* y);
+template <class T>
+void gemm(tf_ctx& c, int m, int n, int k, const T* A, const T* B, T* C);
+DevCsr csr_add(tf_ctx& c, int rows, const int* ra, const int* ca, const double* va, const int* rb, const int* cb,
+               const double* vb, double beta);
+DevCsr csr_transpose(tf_ctx& c, int rows, int cols, int64_t nnz, const int* rp, const int* ci, const double* v);
+DevCsr csr_multiply(tf_ctx& c, int rows, const int* ra, const int* ca, const double* va, const int* rb, const int* cb,
+                    const double* vb);
 DevCsr copy_csr(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v);
 bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax);
 // symmetric: -1 unknown (checked here), 0/1 known pattern symmetry.

@@ -200,6 +200,7 @@ private:
 template <BufferElement T>
 class Vector {
 public:
+    using ValueType = T;
     explicit Vector(MemoryStrategy strategy = GlobalMemory{}) noexcept : values_(strategy) {}
     explicit Vector(std::size_t count, MemoryStrategy strategy = GlobalMemory{}) : values_(count, strategy) {}
     explicit Vector(std::span<const T> values, MemoryStrategy strategy = GlobalMemory{}) : values_(values, strategy) {}
