@@ -315,13 +315,29 @@ __global__ void k_gather_count(int nc, const int* __restrict__ offsets, const in
     int mb, me;
     member_range(I, offsets, mb, me);
     int total = 0, raw = 0;
-    for (int k = mb; k < me; ++k) {
-        const int m = offsets ? members[k] : k;
-        raw += rp[m + 1] - rp[m];
-        if (!graph) continue;
-        for (int j = rp[m]; j < rp[m + 1]; ++j) {
-            const int J = map ? map[ci[j]] : ci[j];
-            if (J != I && v[j] > 0) ++total;
+    if (!graph) {
+        // four member rows per step, loads clamped into range and issued together
+        for (int k = mb; k < me; k += 4) {
+            int rows[4], len[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = min(k + u, me - 1);
+                rows[u] = offsets ? members[q] : q;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) len[u] = rp[rows[u] + 1] - rp[rows[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) raw += k + u < me ? len[u] : 0;
+        }
+    }
+    else {
+        for (int k = mb; k < me; ++k) {
+            const int m = offsets ? members[k] : k;
+            raw += rp[m + 1] - rp[m];
+            for (int j = rp[m]; j < rp[m + 1]; ++j) {
+                const int J = map ? map[ci[j]] : ci[j];
+                if (J != I && v[j] > 0) ++total;
+            }
         }
     }
     cnt[I] = graph ? total : raw;
@@ -386,6 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_dedup_rows(int nc, const int* __rest
     __shared__ int lcol[NG][CAP];
     __shared__ double lval[NG][CAP];
     __shared__ unsigned char lfirst[NG][CAP];
+    __shared__ int mstart[NG][G], moff[NG][G];
     const int g = int(threadIdx.x) / G, lane = int(threadIdx.x) % G;
     const int I = int(blockIdx.x) * NG + g;
     const bool active = I < nc;
@@ -393,28 +410,58 @@ __global__ __launch_bounds__(kBlock) void k_dedup_rows(int nc, const int* __rest
     double* V = lval[g];
     unsigned char* F = lfirst[g];
     int m = 0;
-    if (active) {
-        int mb, me;
-        member_range(I, offsets, mb, me);
-        for (int k = mb; k < me; ++k) {
+    int mb = 0, me = 0;
+    if (active) member_range(I, offsets, mb, me);
+    // Up to G member rows at a time: lane k looks up member k's row range,
+    // a group prefix sum places the rows one after another (member order),
+    // then all lanes gather the entries of all those rows together.
+    for (int k0 = mb; k0 < me; k0 += G) {
+        const int k = k0 + lane;
+        int s = 0, len = 0;
+        if (k < me) {
             const int row = offsets ? members[k] : k;
-            const int s = rp[row], len = rp[row + 1] - s;
-            for (int t = lane; t < len; t += G) {
-                const int J = map ? map[ci[s + t]] : ci[s + t];
-                const double x = v[s + t];
-                const bool keep = !graph || (J != I && x > 0);
-                C[m + t] = keep ? J : -1;
-                V[m + t] = x;
-            }
-            m += len;
+            s = rp[row];
+            len = rp[row + 1] - s;
         }
+        int incl = len;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const int t = __shfl_up(incl, o, G);
+            if (lane >= o) incl += t;
+        }
+        const int total = __shfl(incl, G - 1, G);
+        mstart[g][lane] = s;
+        moff[g][lane] = incl - len;
+        wave_lds_sync();
+        const int nm = min(G, me - k0);
+        for (int t = lane; t < total; t += G) {
+            int q = 0;
+            for (int b = 1; b < nm; ++b)
+                if (moff[g][b] <= t) q = b;
+            const int at = mstart[g][q] + (t - moff[g][q]);
+            const int J = map ? map[ci[at]] : ci[at];
+            const double x = v[at];
+            const bool keep = !graph || (J != I && x > 0);
+            C[m + t] = keep ? J : -1;
+            V[m + t] = x;
+        }
+        m += total;
+        wave_lds_sync();
     }
     wave_lds_sync();
+    // The scans below read 4 LDS entries per step (indices clamped, extra
+    // slots masked) so that their loads overlap.
     for (int e = lane; e < m; e += G) {
         const int J = C[e];
         bool first = J >= 0;
-        for (int q = 0; q < e && first; ++q)
-            if (C[q] == J) first = false;
+        for (int q0 = 0; q0 < e && first; q0 += 4) {
+            int cq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cq[u] = C[min(q0 + u, e - 1)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (q0 + u < e && cq[u] == J) first = false;
+        }
         F[e] = first ? 1 : 0;
     }
     wave_lds_sync();
@@ -424,15 +471,39 @@ __global__ __launch_bounds__(kBlock) void k_dedup_rows(int nc, const int* __rest
         if (!F[e]) continue;
         const int J = C[e];
         double acc = V[e];
-        for (int q = e + 1; q < m; ++q) {
-            if (C[q] != J) continue;
-            const double x = V[q];
-            acc = Max ? (acc > x ? acc : x) : acc + x;
-            if (!isfinite(acc)) atomicOr(bad, 1);
-        }
         int rank = 0;
-        for (int q = 0; q < m; ++q)
-            if (F[q] && C[q] < J) ++rank;
+        bool merged = false;
+        // duplicates after e merge into acc in encounter order
+        for (int q0 = e + 1; q0 < m; q0 += 4) {
+            int cq[4];
+            double vq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = min(q0 + u, m - 1);
+                cq[u] = C[q];
+                vq[u] = V[q];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (q0 + u >= m || cq[u] != J) continue;
+                const double x = vq[u];
+                acc = Max ? (acc > x ? acc : x) : acc + x;
+                merged = true;
+            }
+        }
+        if (merged && !isfinite(acc)) atomicOr(bad, 1);
+        for (int q0 = 0; q0 < m; q0 += 4) {
+            int cq[4];
+            unsigned char fq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = min(q0 + u, m - 1);
+                cq[u] = C[q];
+                fq[u] = F[q];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rank += (q0 + u < m && fq[u] && cq[u] < J) ? 1 : 0;
+        }
         scol[base + rank] = J;
         sval[base + rank] = acc;
         ++mine;
@@ -640,16 +711,33 @@ __global__ void k_propose(int n, const int* __restrict__ rp, const int* __restri
     double score = 0;
     u64 priority = 0;
     if (mate[i] < 0) {
-        for (int j = rp[i]; j < rp[i + 1]; ++j) {
-            const int c = ci[j];
-            const double s = w[j];
-            if (c == i || mate[c] >= 0 || s <= 0) continue;
-            if (s < score) continue;
-            const u64 cand = edge_priority(i, c);
-            if (s > score || (s == score && (best < 0 || cand > priority || (cand == priority && c < best)))) {
-                best = c;
-                score = s;
-                priority = cand;
+        // The best candidate is the maximum of a total order (weight, then
+        // priority, then the smaller column), so entries can be examined in
+        // chunks: the loads of 8 entries and their mates' lookups go out
+        // together (indices clamped into the row, extra slots ignored).
+        const int e = rp[i + 1];
+        for (int j0 = rp[i]; j0 < e; j0 += 8) {
+            int cc[8], mm[8];
+            double ss[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int j = min(j0 + u, e - 1);
+                cc[u] = ci[j];
+                ss[u] = w[j];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) mm[u] = mate[cc[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int c = cc[u];
+                const double s = ss[u];
+                if (j0 + u >= e || c == i || mm[u] >= 0 || s <= 0 || s < score) continue;
+                const u64 cand = edge_priority(i, c);
+                if (s > score || (s == score && (best < 0 || cand > priority || (cand == priority && c < best)))) {
+                    best = c;
+                    score = s;
+                    priority = cand;
+                }
             }
         }
     }
@@ -880,7 +968,10 @@ DevCsr gather_merge(tf_ctx& c, int nc, const int* offsets, const int* members, c
     hipLaunchKernelGGL((k_dedup_rows<GG, MX>), dim3((nc + kBlock / GG - 1) / (kBlock / GG)), dim3(kBlock), 0,    \
                        c.stream, nc, offsets, members, map, rp, ci, v, int(graph), start, scol, sval, ucnt,      \
                        c.dflags)
-        if (maxRow <= 16 * 8) {
+        // 16 lanes per output row while rows are short; a wave per row once
+        // the average row gathers more than 40 entries (the quadratic merge
+        // then dominates and needs the lanes more than the rows do)
+        if (maxRow <= 16 * 8 && total <= 40 * int64_t(nc)) {
             if (maximum)
                 TF_DEDUP(16, true);
             else
@@ -994,7 +1085,7 @@ DevCsr canonicalize(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci,
     return gather_merge(c, n, nullptr, nullptr, nullptr, rp, ci, v, false, false);
 }
 
-DevCsr merge_rows(tf_ctx& c, int rows, int64_t nnz, const int* rp, const int* ci, const double* v) {
+DevCsr merge_csr_rows(tf_ctx& c, int rows, int64_t nnz, const int* rp, const int* ci, const double* v) {
     require(rows >= 1 && nnz >= 1 && nnz <= 0x3fffffff, "row merge needs rows >= 1 and 1..2^30 entries");
     return gather_merge(c, rows, nullptr, nullptr, nullptr, rp, ci, v, false, false);
 }

@@ -257,7 +257,7 @@ static DevCsr merge_candidates(tf_ctx& c, int rows, int64_t total, const int* st
         out.v = dev_alloc<double>(c, 1);
         return out;
     }
-    return merge_rows(c, rows, total, start, oc, ov);
+    return merge_csr_rows(c, rows, total, start, oc, ov);
 }
 
 DevCsr csr_add(tf_ctx& c, int rows, const int* ra, const int* ca, const double* va, const int* rb, const int* cb,
@@ -284,7 +284,7 @@ DevCsr csr_transpose(tf_ctx& c, int rows, int cols, int64_t nnz, const int* rp, 
     // the arbitrary slot order below cannot change a sum
     DevCsr in;
     if (nnz > 0) {
-        in = merge_rows(c, rows, nnz, rp, ci, v);
+        in = merge_csr_rows(c, rows, nnz, rp, ci, v);
         nnz = in.nnz;
         rp = in.rp;
         ci = in.ci;

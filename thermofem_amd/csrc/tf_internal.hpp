@@ -320,7 +320,7 @@ int csr_check(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const
 bool csr_structure_ok(tf_ctx& c, int rows, int cols, int64_t nnz, const int* rp, const int* ci);
 DevCsr canonicalize(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v);
 // Rows of any column range sorted by column, duplicates summed in stored order.
-DevCsr merge_rows(tf_ctx& c, int rows, int64_t nnz, const int* rp, const int* ci, const double* v);
+DevCsr merge_csr_rows(tf_ctx& c, int rows, int64_t nnz, const int* rp, const int* ci, const double* v);
 // Level-1 API of Operations.cuh (tf_dense.hip).
 template <class T>
 double asum(tf_ctx& c, int n, const T* x);
