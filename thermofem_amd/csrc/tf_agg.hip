@@ -333,10 +333,24 @@ __global__ void k_gather_count(int nc, const int* __restrict__ offsets, const in
     else {
         for (int k = mb; k < me; ++k) {
             const int m = offsets ? members[k] : k;
-            raw += rp[m + 1] - rp[m];
-            for (int j = rp[m]; j < rp[m + 1]; ++j) {
-                const int J = map ? map[ci[j]] : ci[j];
-                if (J != I && v[j] > 0) ++total;
+            const int s = rp[m], e = rp[m + 1];
+            raw += e - s;
+            // 8 entries per step: column, weight and mapped column loads overlap
+            for (int j0 = s; j0 < e; j0 += 8) {
+                int cc[8];
+                double vv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int j = min(j0 + u, e - 1);
+                    cc[u] = ci[j];
+                    vv[u] = v[j];
+                }
+                if (map) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) cc[u] = map[cc[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) total += (j0 + u < e && cc[u] != I && vv[u] > 0) ? 1 : 0;
             }
         }
     }
@@ -810,13 +824,30 @@ __global__ void k_propose_absorption(int n, const int* __restrict__ rp, const in
     int best = -1;
     double score = 0;
     if (sizes[map[i]] == 1) {
-        for (int j = rp[i]; j < rp[i + 1]; ++j) {
-            const int c = map[ci[j]];
-            const double s = w[j];
-            if (sizes[c] < 2 || sizes[c] >= 4 || s <= 0) continue;
-            if (s > score || (s == score && (best < 0 || c < best))) {
-                best = c;
-                score = s;
+        // maximum under (weight desc, aggregate asc): chunks of 8 entries
+        const int e = rp[i + 1];
+        for (int j0 = rp[i]; j0 < e; j0 += 8) {
+            int cc[8], sz[8];
+            double ww[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int j = min(j0 + u, e - 1);
+                cc[u] = ci[j];
+                ww[u] = w[j];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) cc[u] = map[cc[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) sz[u] = sizes[cc[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int c = cc[u];
+                const double s = ww[u];
+                if (j0 + u >= e || sz[u] < 2 || sz[u] >= 4 || s <= 0) continue;
+                if (s > score || (s == score && (best < 0 || c < best))) {
+                    best = c;
+                    score = s;
+                }
             }
         }
     }
@@ -833,12 +864,27 @@ __global__ void k_accept_absorption(int nc, const int* __restrict__ rp, const in
     int selected[2] = {-1, -1};
     double weights[2] = {-1, -1};
     const int capacity = 4 - sizes[c];
+    // The chosen rows are the top `capacity` proposers under (score desc,
+    // row asc), a total order, so entries can be visited in chunks of 8
+    // whose loads overlap (indices clamped into the row, extra slots ignored).
     for (int k = 0; k < sizes[c]; ++k) {
         const int row = slots[size_t(c) * 4 + k];
-        for (int j = rp[row]; j < rp[row + 1]; ++j) {
-            const int i = ci[j];
-            if (proposal[i] != c || i == selected[0] || i == selected[1]) continue;
-            const double s = scores[i];
+        const int e = rp[row + 1];
+        for (int j0 = rp[row]; j0 < e; j0 += 8) {
+            int ii[8], pp[8];
+            double ss[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ii[u] = ci[min(j0 + u, e - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                pp[u] = proposal[ii[u]];
+                ss[u] = scores[ii[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+            const int i = ii[u];
+            if (j0 + u >= e || pp[u] != c || i == selected[0] || i == selected[1]) continue;
+            const double s = ss[u];
             for (int q = 0; q < capacity; ++q) {
                 if (s > weights[q] || (s == weights[q] && i < selected[q])) {
                     for (int r = capacity - 1; r > q; --r) {
@@ -849,6 +895,7 @@ __global__ void k_accept_absorption(int nc, const int* __restrict__ rp, const in
                     weights[q] = s;
                     break;
                 }
+            }
             }
         }
     }
