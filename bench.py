@@ -321,8 +321,9 @@ def fem_block(args):
            "data": "synthetic", "config": {"workload": f"P1 tet M/dt+K {nc}^3 cells x 6 Kuhn tets, {k} RHS",
                                            "n": n, "nnz": nnz, "k": k}}
     for prec in (32, 64):
-        s = Cg(KrylovOptions(relative_tolerance=1e-8), AmgOptions(coarse_rows=256, pre_sweeps=2, post_sweeps=2,
-                                                                  jacobi_weight=2.0 / 3.0, level_precision=prec))
+        s = Cg(KrylovOptions(relative_tolerance=1e-8, use_graph=not args.no_graph),
+               AmgOptions(coarse_rows=256, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0,
+                          level_precision=prec))
         times, setups, solves, res = [], [], [], None
         for it in range(args.warmup + args.steps):
             dX.upload(zeros)
@@ -415,7 +416,7 @@ def batch(args):
     db = DeviceArray.from_host(b)
     dx = DeviceArray(n, np.float64)
     amg = AmgOptions(coarse_rows=256, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0, level_precision=32)
-    s = Cg(KrylovOptions(relative_tolerance=1e-8), amg)
+    s = Cg(KrylovOptions(relative_tolerance=1e-8, use_graph=not args.no_graph), amg)
     times, res = [], None
     for it in range(args.warmup + args.steps):
         dx.upload(np.zeros(n))
@@ -437,7 +438,7 @@ def batch(args):
     log(f"segmented: {seg}")
     del s
     # the same systems one at a time (setup + solve each)
-    one = Cg(KrylovOptions(relative_tolerance=1e-8), amg)
+    one = Cg(KrylovOptions(relative_tolerance=1e-8, use_graph=not args.no_graph), amg)
     total, sits = 0.0, []
     for k in range(len(edges)):
         lo, hi = int(off[k]), int(off[k + 1])

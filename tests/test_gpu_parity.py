@@ -129,8 +129,8 @@ def test_coarse_lu_256_matches_oracle():
 
 @pytest.mark.parametrize("n", [3, 37, 88, 89, 128, 129])
 def test_coarse_lu_orders_match_oracle(n):
-    """Both LU kernels (LDS-resident up to order 128, inverse up to 88; the
-    general kernels above): pivots equal to the oracle's, factors within
+    """The LU kernels (matrix in LDS up to order 128, in global memory above,
+    general kernels above; the inverse in LDS up to order 90): pivots equal to the oracle's, factors within
     1e-10, and the explicit inverse times b equal to the oracle's solve."""
     rng = np.random.default_rng(n)
     dense = rng.standard_normal((n, n))
