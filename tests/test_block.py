@@ -108,6 +108,31 @@ def test_spmm_columns_bit_exact(k):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("k", [33, 64])
+def test_spmm_empty_and_uneven_rows(k):
+    """Rows of different lengths (longer than a wave's 64-entry staging
+    window too) and empty rows, also as the first and the last row."""
+    import _gpu as G
+    rng = np.random.default_rng(5)
+    n = 203
+    lens = rng.integers(0, 40, n)
+    lens[[0, 1, 2, 57, 58, 131, n - 1]] = 0
+    lens[[10, 11, 12, 200]] = [150, 64, 65, 129]
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(n, int(m), replace=False)) for m in lens]).astype(np.int32)
+    v = rng.standard_normal(int(rp[-1]))
+    a = (rp, ci, v)
+    X = rng.standard_normal((n, k))
+    Y = G.spmm(a, X)
+    for j in range(k):
+        assert np.array_equal(Y[:, j], O.spmv(a, X[:, j])), j
+    Y32 = G.spmm(a, X.astype(np.float32), np.float32)
+    a32 = (rp, ci, v.astype(np.float32))
+    for j in range(k):
+        assert np.array_equal(Y32[:, j], O.spmv_f32(a32, X[:, j].astype(np.float32))), j
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("tail", [1, 2])
 @pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("cycle", [0, 1, 2])

@@ -127,10 +127,11 @@ def test_coarse_lu_256_matches_oracle():
     assert np.allclose(x.get(), xh, rtol=1e-11, atol=1e-12)
 
 
-@pytest.mark.parametrize("n", [3, 37, 88, 89, 128, 129])
+@pytest.mark.parametrize("n", [3, 16, 17, 37, 88, 129, 200, 256, 1000])
 def test_coarse_lu_orders_match_oracle(n):
-    """The LU kernels (matrix in LDS up to order 128, in global memory above,
-    general kernels above; the inverse in LDS up to order 90): pivots equal to the oracle's, factors within
+    """The blocked LU (16-column panels, partial last panels when 16 does not
+    divide n, matrix-core trailing updates) and the blocked inverse (one
+    workgroup per 16 columns): pivots equal to the oracle's, factors within
     1e-10, and the explicit inverse times b equal to the oracle's solve."""
     rng = np.random.default_rng(n)
     dense = rng.standard_normal((n, n))
@@ -321,7 +322,11 @@ def test_bicgstab_amg_c1_converges(graph):
     assert np.linalg.norm(b - O.spmv(a, x)) <= 1.0001e-8 * np.linalg.norm(b)
     host = O.Amg(coarse_rows=256, jacobi_weight=2 / 3).setup(a)
     xh, st, it, _, _ = O.krylov("bicgstab", host, a, b)
-    assert st == 0 and abs(it - res.iterations) <= 2
+    # BiCGStab's count on this problem is sensitive to rounding: relative
+    # perturbations of 1e-15 in b move the oracle's own count over 47..52
+    # (48 unperturbed), and the coarse solve applies an explicit inverse
+    # where the oracle substitutes, so the counts agree to 10 %, not exactly
+    assert st == 0 and abs(it - res.iterations) <= max(2, 0.1 * it)
 
 
 def test_krylov_failure_paths():

@@ -75,6 +75,19 @@ _SIGS = {
     "tf_residual_csr_f32": (i32, [vp, i32, i64, vp, vp, vp, vp, vp, vp]),
     "tf_spmm_csr_f64": (i32, [vp, i32, i64, vp, vp, vp, i32, vp, vp]),
     "tf_spmm_csr_f32": (i32, [vp, i32, i64, vp, vp, vp, i32, vp, vp]),
+    "tf_asum_f64": (i32, [vp, i32, vp, F64P]),
+    "tf_asum_f32": (i32, [vp, i32, vp, C.POINTER(f32)]),
+    "tf_dense_transpose_f64": (i32, [vp, i32, i32, vp, vp]),
+    "tf_dense_transpose_f32": (i32, [vp, i32, i32, vp, vp]),
+    "tf_gemv_f64": (i32, [vp, i32, i32, vp, vp, vp]),
+    "tf_gemv_f32": (i32, [vp, i32, i32, vp, vp, vp]),
+    "tf_gemm_f64": (i32, [vp, i32, i32, i32, vp, vp, vp]),
+    "tf_gemm_f32": (i32, [vp, i32, i32, i32, vp, vp, vp]),
+    "tf_csr_add_f64": (i32, [vp, i32, i32, i64, vp, vp, vp, i64, vp, vp, vp, f64, vp, C.POINTER(vp), C.POINTER(vp),
+                             I64P]),
+    "tf_csr_transpose_f64": (i32, [vp, i32, i32, i64, vp, vp, vp, vp, C.POINTER(vp), C.POINTER(vp)]),
+    "tf_csr_multiply_f64": (i32, [vp, i32, i32, i32, i64, vp, vp, vp, i64, vp, vp, vp, vp, C.POINTER(vp),
+                                  C.POINTER(vp), I64P]),
     "tf_jacobi_update_f64": (i32, [vp, i32, f64, vp, vp, vp, vp]),
     "tf_jacobi_update_f32": (i32, [vp, i32, f32, vp, vp, vp, vp]),
     "tf_jacobi_fused_f64": (i32, [vp, i32, i64, vp, vp, vp, vp, f64, vp, vp, vp]),

@@ -7,14 +7,14 @@
 // Backend.cu:256-281).
 //
 // Launch shape: G lanes per row (G = k rounded up to a power of two, at most
-// 64 for FP64 and 32 for FP32 blocks; lane = column inside a chunk of G
-// columns, blockIdx.y = chunk) and kBlock / G rows per block step.  At
-// G = 64 a row is one wave: its column
-// indices and values are wave-uniform scalar loads and each gathered block
-// row is one coalesced 64-lane load.  Row-parallel launches are XCD-major:
-// XCD x takes the x-th contiguous slab of rows and its blocks walk the slab
-// together, so the block rows that neighbouring rows gather stay in that
-// XCD's L2.
+// 64; lane = column inside a chunk of G columns, blockIdx.y = chunk) and
+// kBlock / G rows per block step.  At G = 64 a row is one wave: the row's
+// column indices and values are staged by one coalesced vector load (64
+// entries per load, lane l holding entry l) and handed to every lane with
+// readlane, and each gathered block row is one coalesced 64-lane load.
+// Row-parallel launches are XCD-major: XCD x takes the x-th contiguous slab
+// of rows and its blocks walk the slab together, so the block rows that
+// neighbouring rows gather stay in that XCD's L2.
 //
 // Every column is computed with the arithmetic of the single-vector kernels
 // (row sums of rounded products added in stored order, unfused epilogues,
@@ -49,16 +49,15 @@ __device__ __forceinline__ void block_groups(int groups, int& first, int& step, 
     end = min(groups, begin + slab);
 }
 
-// Lanes per row = columns per chunk: up to a wave for FP64 blocks (a row's
-// indices and values become wave-uniform scalar loads), up to 32 for FP32
-// blocks, so that one pass over the rows gathers 32-column slices whose
-// reuse window (a grid plane either side of a stencil row) fits an XCD's L2
-// better (measured at k = 64: FP32 129 -> 114 us; FP64 181 -> 307 us with
-// 32-lane chunks, so FP64 keeps whole waves).
-int lanes_for(int k, size_t elem = sizeof(double)) {
-    const int cap = elem == sizeof(float) ? 32 : kWave;
+// Lanes per row = columns per chunk, at most a wave.  Measured at k = 64 on
+// a 15-point P1 tet matrix (274,625 rows, 4.0M nonzeros), FP64 / FP32:
+// whole-wave rows with staged indices 178 / 88 us, with wave-uniform scalar
+// index loads 180 / 133 us, two rows per wave (32-column chunks) with
+// per-lane index loads 314 / 124 us, with scalar loads 355 / 262 us, with
+// staged loads 245 / 158 us -- all bit-identical.
+int lanes_for(int k) {
     int g = 1;
-    while (g < k && g < cap) g <<= 1;
+    while (g < k && g < kWave) g <<= 1;
     return g;
 }
 
@@ -92,6 +91,55 @@ __device__ __forceinline__ T brow_sum(int s, int e, const int* __restrict__ ci, 
             const T p = pv[u] * xv[u];
             const T sum = acc + p;
             acc = j0 + u < e ? sum : acc;
+        }
+    }
+    return acc;
+}
+
+__device__ __forceinline__ int lane_value(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ float lane_value(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double lane_value(double x, int l) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(b), l));
+    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(b >> 32), l));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Row sum of a wave-uniform row [s, e) against column col (lane = column):
+// the row's indices and values are staged 64 at a time by one coalesced
+// load (lane l holds entry l of the window) and broadcast with readlane;
+// products rounded and added one by one in stored order as in brow_sum.
+// Slots past the window re-read its last entry and are not added.
+template <class T>
+__device__ __forceinline__ T brow_sum_staged(int s, int e, const int* __restrict__ ci, const T* __restrict__ v,
+                                             const T* __restrict__ X, int k, int col) {
+#pragma clang fp contract(off)
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    T acc = 0;
+    for (int b = s; b < e; b += kWave) {
+        int cl = 0;
+        T vl = 0;
+        if (b + lane < e) {
+            cl = ci[b + lane];
+            vl = v[b + lane];
+        }
+        const int cnt = min(kWave, e - b);
+        for (int u0 = 0; u0 < cnt; u0 += 8) {
+            T pv[8], xv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int q = min(u0 + u, cnt - 1);
+                pv[u] = lane_value(vl, q);
+                xv[u] = X[size_t(lane_value(cl, q)) * size_t(k) + size_t(col)];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const T p = pv[u] * xv[u];
+                const T sum = acc + p;
+                acc = u0 + u < cnt ? sum : acc;
+            }
         }
     }
     return acc;
@@ -138,15 +186,20 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(int n, int k, const int* __res
     int first, step, end;
     block_groups((n + R - 1) / R, first, step, end);
     for (int g = first; g < end; g += step) {
-        int row = g * R + sub;
-        if constexpr (G == kWave) row = __builtin_amdgcn_readfirstlane(row);
-        if (row >= n) continue;
-        int s = rp[row], e = rp[row + 1];
+        T acc;
+        int row;
         if constexpr (G == kWave) {
-            s = __builtin_amdgcn_readfirstlane(s);
-            e = __builtin_amdgcn_readfirstlane(e);
+            // the wave's row (wave index made uniform): staged index loads
+            row = g * R + __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
+            if (row >= n) continue;
+            acc = brow_sum_staged<T>(__builtin_amdgcn_readfirstlane(rp[row]),
+                                     __builtin_amdgcn_readfirstlane(rp[row + 1]), ci, v, X, k, cc);
         }
-        const T acc = brow_sum<T>(s, e, ci, v, X, k, cc);
+        else {
+            row = g * R + sub;
+            if (row >= n) continue;
+            acc = brow_sum<T>(rp[row], rp[row + 1], ci, v, X, k, cc);
+        }
         const size_t at = size_t(row) * size_t(k) + size_t(cc);
         T y;
         if constexpr (E == kSpmv) {
@@ -323,7 +376,7 @@ template <class T, int E, int D>
 void bspmm(tf_ctx& c, int n, int k, const int* rp, const int* ci, const T* v, const T* X, T* Y, const T* B,
            const T* inv, T w, double* partials, const KCtl* ctl) {
     if (n <= 0 || k <= 0) return;
-    const int G = lanes_for(k, sizeof(T));
+    const int G = lanes_for(k);
     const int R = kBlock / G;
     const dim3 grid(row_grid((n + R - 1) / R, D != kBNone), (k + G - 1) / G);
 #define TF_BS(GG)                                                                                              \
