@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_rows(int n, int nblocks, const i
     if (row >= n) return;  // whole groups leave together
     const int s = rp[row], e = rp[row + 1];
     T acc = 0;
-    for (int j = s + lane; j < e; j += G) acc += ld_stream(v + j) * x[ld_stream(ci + j)];
+    for (int j = s + lane; j < e; j += G) acc += ld_entry(v + j) * x[ld_entry(ci + j)];
     acc = group_sum<G>(acc);
     if (lane == 0) epilogue<T, E>(row, acc, x, y, b, inv, w);
 }
