@@ -108,6 +108,24 @@ def test_coarse_lu_singular():
     assert not ok
 
 
+@pytest.mark.parametrize("n,dup", [(40, (3, 37)), (150, (20, 149))])
+def test_coarse_lu_singular_in_later_panel(n, dup):
+    """A row repeated further down makes the matrix singular; elimination
+    reaches the zero pivot only in a later 16-column panel (the last,
+    partial one for n = 150), where the factorisation must stop with the
+    same verdict as the oracle."""
+    rng = np.random.default_rng(n)
+    dense = rng.standard_normal((n, n)) + np.diag(np.full(n, 4.0))
+    dense[dup[1]] = dense[dup[0]]
+    rows, cols = np.nonzero(dense)
+    rp = np.zeros(n + 1, np.int32)
+    np.cumsum(np.bincount(rows, minlength=n), out=rp[1:])
+    a = (rp, cols.astype(np.int32), dense[rows, cols])
+    ok, *_ = G.lu_factor(a)
+    ok_h, _, _ = O.lu_factor(a)
+    assert not ok_h and not ok
+
+
 def test_coarse_lu_256_matches_oracle():
     """Dense LU at the widened coarse size (n=256) against the oracle's factors and solve."""
     a = P.poisson2d(16)  # 256 rows
