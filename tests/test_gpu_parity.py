@@ -309,6 +309,27 @@ def test_zero_diagonal_throws():
 
 
 @pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("defect", ["zero diagonal", "nan off-diagonal", "inf value"])
+def test_invalid_fine_level_throws(defect, precision):
+    """A defect on a level that is coarsened further (256 rows, coarsest
+    bound 64): its checks come back with the strength graph's size and must
+    still stop the setup."""
+    rp, ci, v = P.poisson2d(16)
+    v = v.copy()
+    row = 77
+    s, e = rp[row], rp[row + 1]
+    cols = ci[s:e]
+    if defect == "zero diagonal":
+        v[s + int(np.nonzero(cols == row)[0][0])] = 0.0
+    elif defect == "nan off-diagonal":
+        v[s + int(np.nonzero(cols != row)[0][0])] = np.nan
+    else:
+        v[s + int(np.nonzero(cols != row)[0][0])] = np.inf
+    with pytest.raises(ValueError):
+        Amg(AmgOptions(coarse_rows=64, level_precision=precision)).setup((rp, ci, v))
+
+
+@pytest.mark.parametrize("precision", [64, 32])
 def test_cg_amg_heat3d_converges(precision):
     """AMG-PCG (V(2,2), omega 2/3) on the log-normal heat problem at 32^3 reaches 1e-8 in
     FP64 recomputed residual; the solution agrees with the oracle's FP64 solve."""

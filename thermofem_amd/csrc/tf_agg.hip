@@ -1137,6 +1137,12 @@ DevCsr merge_csr_rows(tf_ctx& c, int rows, int64_t nnz, const int* rp, const int
     return gather_merge(c, rows, nullptr, nullptr, nullptr, rp, ci, v, false, false);
 }
 
+void validate_rowmax_async(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax,
+                           int* flag) {
+    hipLaunchKernelGGL(k_validate, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, v, flag, rowmax);
+    check(hipGetLastError(), "validate launch");
+}
+
 bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax) {
     c.clear_flags(1);
     hipLaunchKernelGGL(k_validate, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, v, c.dflags, rowmax);
@@ -1148,7 +1154,7 @@ bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const doubl
 double* rowmax_workspace(tf_ctx& c, int n) { return c.wst<double>(kRowmax, size_t(n)); }
 
 DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta,
-                      int symmetric, bool prevalidated, int* symmetricOut, const double* learned) {
+                      int symmetric, bool prevalidated, int* symmetricOut, const double* learned, int deferred) {
     require(std::isfinite(theta) && theta >= 0 && theta <= 1, "Invalid strength threshold");
     if (nnz > int64_t(0x3fffffff)) throw Error(TF_E_CAPACITY, "Native AMG: strength graph exceeds 32-bit capacity");
     double* rowmax = rowmax_workspace(c, n);
@@ -1188,7 +1194,7 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
         g.n = n;
         g.rp = dev_alloc<int>(c, size_t(n) + 1);
         scan_excl(c, n + 1, cnt, g.rp);
-        g.nnz = c.read_int(g.rp + n);
+        g.nnz = deferred > 0 ? c.read_int_with_flags(g.rp + n, deferred) : c.read_int(g.rp + n);
         g.ci = dev_alloc<int>(c, size_t(g.nnz));
         g.v = dev_alloc<double>(c, size_t(g.nnz));
 #define TF_COMPACT(G)                                                                                            \
@@ -1209,7 +1215,7 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
     check(hipGetLastError(), "strength count launch");
     int* start = c.wst<int>(kStart, size_t(n) + 1);
     scan_excl(c, n + 1, cnt, start);
-    const int total = c.read_int(start + n);
+    const int total = deferred > 0 ? c.read_int_with_flags(start + n, deferred) : c.read_int(start + n);
     if (total == 0) {
         DevCsr g;
         g.n = n;
@@ -1266,7 +1272,7 @@ DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const doub
     return gather_merge(c, nc, moff, mem, map, rp, ci, w, true, true);
 }
 
-DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb) {
+DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb, int* hostHist) {
     require(rounds >= 1 && rounds <= 8 && g.n >= 1, "Invalid matching configuration");
     const int n = g.n;
     DevAgg t;
@@ -1317,10 +1323,19 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb) {
             scan_excl(c, coarse + 1, flags, prefix);
             hipLaunchKernelGGL(k_compose, grid1(n), dim3(kBlock), 0, c.stream, n, t.map, prefix);
             check(hipGetLastError(), "compaction launch");
-            coarse = c.read_int(prefix + coarse);
-            build_members(c, n, coarse, t.map, &sizes, &slots);
+            // The compacted count comes back with the capacity flag; the
+            // members are gathered over the old count, a bound (aggregate
+            // ids past the new count stay empty).
+            const int bound = coarse;
+            check(hipMemcpyAsync(c.dflags + 1, prefix + bound, sizeof(int), hipMemcpyDeviceToDevice, c.stream),
+                  "aggregate count stage");
+            build_members(c, n, bound, t.map, &sizes, &slots);
+            c.read_flags(2);
+            coarse = c.hflags[1];
         }
-        c.read_flags(1);
+        else {
+            c.read_flags(1);
+        }
         if (c.hflags[0]) throw Error(TF_E_INVALID, "Aggregate capacity exceeded");
         t.nc = coarse;
         t.offsets = dev_alloc<int>(c, size_t(coarse) + 1);
@@ -1331,9 +1346,16 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb) {
         hipLaunchKernelGGL(k_pack_members, grid1(coarse), dim3(kBlock), 0, c.stream, coarse, sizes, slots, t.offsets,
                            t.members, hist);
         check(hipGetLastError(), "pack members launch");
-        check(hipMemcpyAsync(c.hflags, hist, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream), "histogram download");
-        c.sync();
-        for (int k = 0; k < 4; ++k) t.hist[k] = c.hflags[k];
+        if (hostHist) {
+            check(hipMemcpyAsync(hostHist, hist, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream),
+                  "histogram download");
+        }
+        else {
+            check(hipMemcpyAsync(c.hflags, hist, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream),
+                  "histogram download");
+            c.sync();
+            for (int k = 0; k < 4; ++k) t.hist[k] = c.hflags[k];
+        }
     }
     catch (...) {
         free_agg(c, t);

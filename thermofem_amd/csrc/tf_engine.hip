@@ -390,6 +390,7 @@ AmgEngine::AmgEngine(tf_ctx* ctx, const AmgOptionsC& o) : opt_(o), c_(ctx) {}
 AmgEngine::~AmgEngine() {
     delete block_;
     release();
+    if (histHost_) (void)hipHostFree(histHost_);
 }
 
 BlockAmg& AmgEngine::block() {
@@ -553,6 +554,8 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
         dev_free(*c_, piv);
         dev_free(*c_, inv);
     };
+    if (!histHost_)
+        check(hipHostMalloc(&histHost_, sizeof(int) * 4 * kMaxHistLevels, hipHostMallocDefault), "hipHostMalloc");
     try {
         DevCsr native = canonicalInput ? copy_csr(*c_, n, nnz, rp, ci, v) : canonicalize(*c_, n, nnz, rp, ci, v);
         int symmetric = -1;
@@ -561,9 +564,18 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
             AmgLevel& L = next.back();
             L.native = native;
             const int ln = native.n;
-            if (!validate_rowmax(*c_, ln, native.rp, native.ci, native.v, rowmax_workspace(*c_, ln)))
-                throw Error(TF_E_NUMERIC,
-                            "Native AMG: missing/zero diagonal, non-finite value or reciprocal in hierarchy");
+            // The level's checks raise flag slots 1 (row validity) and 2
+            // (diagonal); they come back with the strength graph's size, or
+            // in a read of their own on the coarsest level.
+            c_->clear_flags(3);
+            validate_rowmax_async(*c_, ln, native.rp, native.ci, native.v, rowmax_workspace(*c_, ln),
+                                  c_->dflags + 1);
+            auto level_checks = [&]() {
+                if (c_->hflags[1])
+                    throw Error(TF_E_NUMERIC,
+                                "Native AMG: missing/zero diagonal, non-finite value or reciprocal in hierarchy");
+                if (c_->hflags[2]) throw Error(TF_E_NUMERIC, "Missing or invalid diagonal");
+            };
             if (f32) {
                 L.vals = dev_alloc<float>(*c_, size_t(native.nnz));
                 convert<double, float>(*c_, size_t(native.nnz), native.v, static_cast<float*>(L.vals));
@@ -576,21 +588,16 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
             L.x = dev_alloc<char>(*c_, size_t(ln) * es);
             L.x2 = dev_alloc<char>(*c_, size_t(ln) * es);
             L.r = dev_alloc<char>(*c_, size_t(ln) * es);
-            c_->clear_flags(1);
             if (f32)
                 inverse_diagonal<float>(*c_, ln, native.rp, native.ci, static_cast<const float*>(L.vals),
-                                        static_cast<float*>(L.inv), c_->dflags);
+                                        static_cast<float*>(L.inv), c_->dflags + 2);
             else
                 inverse_diagonal<double>(*c_, ln, native.rp, native.ci, native.v, static_cast<double*>(L.inv),
-                                         c_->dflags);
-            c_->read_flags(1);
-            if (c_->hflags[0]) throw Error(TF_E_NUMERIC, "Missing or invalid diagonal");
-            if (ln <= opt_.coarseRows) {
-                stop = "coarse size";
-                break;
-            }
-            if (depth + 1 == opt_.maxLevels) {
-                stop = "level limit";
+                                         c_->dflags + 2);
+            if (ln <= opt_.coarseRows || depth + 1 == opt_.maxLevels) {
+                c_->read_flags(3);
+                level_checks();
+                stop = ln <= opt_.coarseRows ? "coarse size" : "level limit";
                 break;
             }
             // Galerkin products of a pattern-symmetric matrix are pattern-symmetric, so
@@ -606,9 +613,11 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
             DevCsr g;
             try {
                 g = strength_graph(*c_, ln, native.nnz, native.rp, native.ci, native.v, opt_.theta, symmetric, true,
-                                   &symmetric, lw);
+                                   &symmetric, lw, 3);
+                level_checks();
             }
             catch (...) {
+                free_csr(*c_, g);
                 dev_free(*c_, lw);
                 throw;
             }
@@ -623,7 +632,8 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
             }
             DevAgg t;
             try {
-                t = aggregate(*c_, g, opt_.rounds, opt_.absorb);
+                t = aggregate(*c_, g, opt_.rounds, opt_.absorb,
+                              depth < kMaxHistLevels ? histHost_ + 4 * depth : nullptr);
             }
             catch (...) {
                 free_csr(*c_, g);
@@ -636,7 +646,7 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
                 break;
             }
             L.transfer = t;
-            for (int k = 0; k < 4; ++k) L.hist[k] = t.hist[k];
+            for (int k = 0; k < 4; ++k) L.hist[k] = t.hist[k];  // (filled below when copied asynchronously)
             native = galerkin(*c_, L.native, t.map, t.nc, t.offsets, t.members);
         }
         const DevCsr& last = next.back().native;
@@ -671,6 +681,8 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
             }
         }
         c_->sync();
+        for (size_t d = 0; d + 1 < next.size() && d < size_t(kMaxHistLevels); ++d)
+            for (int k = 0; k < 4; ++k) next[d].hist[k] = histHost_[4 * d + k];
     }
     catch (...) {
         cleanup();

@@ -71,6 +71,10 @@ void tail_run(tf_ctx& c, const TailLevel* levels, int top, int count, const void
 
 class BlockAmg;
 
+// Levels whose aggregate histograms setup downloads asynchronously (the
+// option's level limit).
+constexpr int kMaxHistLevels = 20;
+
 class AmgEngine {
 public:
     AmgEngine(tf_ctx* ctx, const AmgOptionsC& o);
@@ -133,6 +137,7 @@ private:
     TailProgram tailPrograms_[12];
 
     tf_ctx* c_;
+    int* histHost_ = nullptr;  // pinned: every level's aggregate-size histogram (4 ints a level)
     bool factored_ = false, singular_ = false;
     double* lu_ = nullptr;
     int* piv_ = nullptr;

@@ -170,6 +170,15 @@ struct tf_ctx {
         sync();
         return hflags[0];
     }
+    // Reads one device int together with device flags [0, slot): the int is
+    // copied into flag slot `slot` on the device, then slots [0, slot] come
+    // back in one download (flags raised by earlier kernels are checked
+    // without a synchronisation of their own).
+    int read_int_with_flags(const int* p, int slot) {
+        tf::check(hipMemcpyAsync(dflags + slot, p, sizeof(int), hipMemcpyDeviceToDevice, stream), "int stage");
+        read_flags(slot + 1);
+        return hflags[slot];
+    }
     // Reads n ints from device flags after the queued work completes.
     void read_flags(int n) {
         tf::check(hipMemcpyAsync(hflags, dflags, size_t(n) * sizeof(int), hipMemcpyDeviceToHost, stream),
@@ -337,17 +346,27 @@ DevCsr csr_multiply(tf_ctx& c, int rows, const int* ra, const int* ca, const dou
                     const double* vb);
 DevCsr copy_csr(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v);
 bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax);
+// The same checks without reading them back: *flag (zeroed by the caller)
+// becomes nonzero when a row fails.
+void validate_rowmax_async(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax,
+                           int* flag);
 // symmetric: -1 unknown (checked here), 0/1 known pattern symmetry.
 // prevalidated: the caller ran validate_rowmax with rowmax_workspace(c, n).
+// With deferred = k > 0 the graph's size comes back through
+// read_int_with_flags(..., k): flag slots 1..k-1 (slot 0 is the graph's own
+// symmetry check) are in c.hflags when it returns.
 DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta,
                       int symmetric = -1, bool prevalidated = false, int* symmetricOut = nullptr,
-                      const double* learned = nullptr);
+                      const double* learned = nullptr, int deferred = 0);
 double* rowmax_workspace(tf_ctx& c, int n);
 void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal);
 // nnz < 0: read from rp[n].
 int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map, int64_t nnz = -1);
 DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc);
-DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb);
+// With hostHist (pinned, 4 ints) the aggregate-size histogram is copied
+// there asynchronously and DevAgg::hist is left zero: it is valid once the
+// stream has drained.
+DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb, int* hostHist = nullptr);
 DevCsr galerkin(tf_ctx& c, const DevCsr& a, const int* map, int nc, const int* offsets, const int* members);
 
 }  // namespace tf

@@ -129,7 +129,8 @@ __global__ __launch_bounds__(kBlock) void k_sell(SellView<C> m, const T* __restr
 namespace {
 // Slice offsets from the widths (exclusive scan); false if the total would
 // pass the int range or the padding limit.  Sets s.base and s.stored.
-bool sell_offsets(tf_ctx& c, int slices, int64_t nnz, const int* width64, double maxPadding, DevSell& s) {
+bool sell_offsets(tf_ctx& c, int slices, int64_t nnz, const int* width64, double maxPadding, DevSell& s,
+                  int flags = 0) {
     if (int64_t(slices) * kSliceRows * 255 > int64_t(INT32_MAX)) {
         // The offsets are scanned in int: check the exact total first when it could overflow.
         std::vector<int> w(static_cast<size_t>(slices));
@@ -138,11 +139,14 @@ bool sell_offsets(tf_ctx& c, int slices, int64_t nnz, const int* width64, double
         c.sync();
         int64_t total = 0;
         for (int x : w) total += x;
-        if (total > int64_t(INT32_MAX)) return false;
+        if (total > int64_t(INT32_MAX)) {
+            if (flags > 0) c.read_flags(flags);
+            return false;
+        }
     }
     int* base = dev_alloc<int>(c, size_t(slices) + 1);
     scan_excl(c, slices + 1, width64, base);
-    const int64_t stored = c.read_int(base + slices);
+    const int64_t stored = flags > 0 ? c.read_int_with_flags(base + slices, flags) : c.read_int(base + slices);
     if (double(stored) > double(nnz) * (1.0 + maxPadding)) {
         dev_free(c, base);
         return false;
@@ -162,10 +166,15 @@ bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, Dev
     hipLaunchKernelGGL(k_sell_width, dim3((slices + kSlicesPerBlock - 1) / kSlicesPerBlock), dim3(kBlock), 0,
                        c.stream, n, slices, rp, ci, width64, c.dflags);
     check(hipGetLastError(), "sell width launch");
-    c.read_flags(2);
-    if (c.hflags[0]) return false;  // a row over 255 entries
-    const bool wide = c.hflags[1] != 0;
-    if (!sell_offsets(c, slices, nnz, width64, maxPadding, s)) {
+    // The width flags (a row over 255 entries, a column offset past 16 bits)
+    // come back with the first offsets scan's total.
+    const bool fits = sell_offsets(c, slices, nnz, width64, maxPadding, s, 2);
+    const bool tooLong = c.hflags[0] != 0, wide = c.hflags[1] != 0;
+    if (tooLong) {
+        free_sell(c, s);
+        return false;
+    }
+    if (!fits) {
         // Rows of one slice differ too much in length: order rows by length in 256-row windows.
         int* perm = dev_alloc<int>(c, size_t(n));
         hipLaunchKernelGGL(k_sell_sort, dim3((n + kSortWindow - 1) / kSortWindow), dim3(kSortWindow), 0, c.stream, n,
