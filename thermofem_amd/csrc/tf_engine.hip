@@ -1359,8 +1359,24 @@ tf_solve_result SolverEngine::apply(const double* b, double* x) {
         if (kind_ == 1) convert<double, double>(*c_, size_t(n_), r_, sh_);
         bool done = false;
         KCtl* h = static_cast<KCtl*>(hctl_);
-        for (int k = 0; k < o_.maxit; ++k) {
-            capture_or_launch(0);
+        // Iterations are launched in batches between reads of the control
+        // block: the batch is one less than the iterations the mean residual
+        // reduction so far says remain (at most 8, one near the tolerance),
+        // so the read rarely comes late.  An iteration launched after the
+        // solve has stopped changes nothing: its Krylov kernels return on the
+        // stopped status, only the preconditioner runs once more.
+        const double res0 = res;
+        int k = 0;
+        while (k < o_.maxit) {
+            int batch = 1;
+            if (k >= 2 && h->res > 0 && h->res < res0) {
+                const double rate = std::pow(h->res / res0, 1.0 / k);
+                const double remain = std::log(tol_ / h->res) / std::log(rate);
+                if (std::isfinite(remain) && remain > 2) batch = std::min(8, int(remain) - 1);
+            }
+            batch = std::min(batch, o_.maxit - k);
+            for (int q = 0; q < batch; ++q) capture_or_launch(0);
+            k += batch;
             check(hipMemcpyAsync(hctl_, ctl_, sizeof(KCtl), hipMemcpyDeviceToHost, c_->stream), "control download");
             c_->sync();
             if (h->status != kRunning) {
