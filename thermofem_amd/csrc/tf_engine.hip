@@ -279,6 +279,25 @@ __global__ __launch_bounds__(1024) void k_kfinal(const double* __restrict__ part
     if (threadIdx.x == 0) finalize(mode, a, ctl);
 }
 
+// The preconditioner's output step fused with the dot product that follows
+// it: z = FP64(level-0 iterate) (or 0), sum(r .* z) per block with the
+// thread order and block reduction of k_kdot, so the partials are the ones
+// k_kdot would form from z.
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_out_dot(int n, const L* __restrict__ xl, int zero,
+                                                    const double* __restrict__ r, double* __restrict__ z,
+                                                    double* partials, const KCtl* ctl) {
+    if (stopped(ctl)) return;
+    double s = 0;
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
+        const double zi = zero ? 0.0 : double(xl[i]);
+        z[i] = zi;
+        s += r[i] * zi;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 // CG direction: p = z (fresh) or p = z + beta p.
 __global__ void k_cg_p(int n, const double* __restrict__ z, double* __restrict__ p, const KCtl* ctl) {
     if (stopped(ctl)) return;
@@ -1065,7 +1084,8 @@ void AmgEngine::cycle(int kind, size_t d, bool zero) {
 }
 
 template <class L>
-void AmgEngine::apply_typed(const double* rhs, double* x) {
+void AmgEngine::apply_typed(const double* rhs, double* x, const double* with, double* partials, int blocks,
+                            const KCtl* ctl) {
     AmgLevel& l0 = levels_[0];
     const int n = l0.native.n;
     const bool first = opt_.cycle != 2 && levels_.size() > 1 && opt_.pre >= 1;
@@ -1075,7 +1095,12 @@ void AmgEngine::apply_typed(const double* rhs, double* x) {
     l0.preapplied = first;
     std::fill(visits_.begin(), visits_.end(), 0);
     cycle<L>(opt_.cycle, 0, true);
-    if (l0.zero) {
+    if (with) {
+        hipLaunchKernelGGL(k_out_dot<L>, dim3(blocks), dim3(kBlock), 0, c_->stream, n,
+                           static_cast<const L*>(l0.x), l0.zero ? 1 : 0, with, x, partials, ctl);
+        check(hipGetLastError(), "output dot launch");
+    }
+    else if (l0.zero) {
         fill_zero(*c_, x, size_t(n) * sizeof(double));
     }
     else {
@@ -1089,6 +1114,15 @@ void AmgEngine::apply(const double* rhs, double* x) {
         apply_typed<float>(rhs, x);
     else
         apply_typed<double>(rhs, x);
+}
+
+void AmgEngine::apply_dot(const double* rhs, double* x, const double* with, double* partials, int blocks,
+                          const KCtl* ctl) {
+    require(ready(), "AMG requires setup", TF_E_INVALID);
+    if (opt_.precision == 32)
+        apply_typed<float>(rhs, x, with, partials, blocks, ctl);
+    else
+        apply_typed<double>(rhs, x, with, partials, blocks, ctl);
 }
 
 // --------------------------------------------------------------- Krylov
@@ -1239,9 +1273,12 @@ void SolverEngine::launch_iteration(const R& rows, int nb) {
         hipLaunchKernelGGL(k_kfinal, dim3(1), dim3(1024), 0, c_->stream, pslot(slot), parts, ctl, mode, onEarly);
     };
     if (kind_ == 0) {
-        precondition(r_, z_);
-        hipLaunchKernelGGL(k_kdot, vb, dim3(kBlock), 0, c_->stream, n_, r_, z_, pslot(kSlotA), T + kSlotA, ctl,
-                           int(kFinRhoCg));
+        // z = M r with rho = r.z formed in the same pass
+        if (amg_)
+            amg_->apply_dot(r_, z_, r_, pslot(kSlotA), nv, ctl);
+        else
+            hipLaunchKernelGGL(k_out_dot<double>, vb, dim3(kBlock), 0, c_->stream, n_, r_, 0, r_, z_,
+                               pslot(kSlotA), ctl);
         fin(kSlotA, nv, kFinRhoCg, 0);
         hipLaunchKernelGGL(k_cg_p, vb, dim3(kBlock), 0, c_->stream, n_, z_, p_, ctl);
         TF_G(k_kspmv_dot<R>, rows, p_, q_, p_, pslot(kSlotB), T + kSlotB, ctl, int(kFinCurvCg));

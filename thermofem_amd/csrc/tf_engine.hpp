@@ -70,6 +70,7 @@ void tail_run(tf_ctx& c, const TailLevel* levels, int top, int count, const void
               const TailOp* ops, int nops, const double* inverse, L w);
 
 class BlockAmg;
+struct KCtl;
 
 // Levels whose aggregate histograms setup downloads asynchronously (the
 // option's level limit).
@@ -84,6 +85,11 @@ public:
     void setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v, bool canonicalInput = false);
     // z = M^-1 r (FP64 device vectors; may alias).  Enqueued on the context stream.
     void apply(const double* rhs, double* x);
+    // apply(rhs, x) whose last step (the level-0 iterate written out as
+    // FP64 x) also forms sum(with .* x) as block partials partials[b],
+    // b < blocks, in the Krylov reductions' layout; that step is skipped once
+    // the control block has stopped.
+    void apply_dot(const double* rhs, double* x, const double* with, double* partials, int blocks, const KCtl* ctl);
     bool ready() const { return !levels_.empty(); }
     // Explicit inverse of the coarsest operator (nullptr: coarse Jacobi).
     const double* coarse_inverse() const { return factored_ ? inverse_ : nullptr; }
@@ -113,7 +119,8 @@ private:
     template <class L>
     void coarse(size_t d);
     template <class L>
-    void apply_typed(const double* rhs, double* x);
+    void apply_typed(const double* rhs, double* x, const double* with = nullptr, double* partials = nullptr,
+                     int blocks = 0, const KCtl* ctl = nullptr);
     template <class L>
     void run_tail(int kind, bool zero);
     void build_tail();
