@@ -54,6 +54,7 @@ struct TuneKey {
     }
 };
 std::map<TuneKey, bool> g_tuned;
+std::map<TuneKey, int> g_tail_start;  // chosen first level of the coarse tail (-1: none)
 std::mutex g_tune_mutex;
 
 // Times the two candidates alternately (one untimed launch each first) and
@@ -81,6 +82,26 @@ bool first_is_faster(tf_ctx& c, A&& first, B&& second, int samples = 5) {
     }
     for (auto& ev : e) (void)hipEventDestroy(ev);
     return best[0] <= best[1];
+}
+
+// Least time of f over `samples` runs (after one untimed run), in ms.
+template <class F>
+float best_time(tf_ctx& c, F&& f, int samples = 3) {
+    hipEvent_t e[2];
+    for (auto& ev : e) check(hipEventCreate(&ev), "hipEventCreate");
+    f();
+    float best = 1e30f;
+    for (int i = 0; i < samples; ++i) {
+        check(hipEventRecord(e[0], c.stream), "event record");
+        f();
+        check(hipEventRecord(e[1], c.stream), "event record");
+        check(hipEventSynchronize(e[1]), "event sync");
+        float t = 0;
+        check(hipEventElapsedTime(&t, e[0], e[1]), "event time");
+        best = std::min(best, t);
+    }
+    for (auto& ev : e) (void)hipEventDestroy(ev);
+    return best;
 }
 
 // Keeps the sliced-ELL copy of a level only where it beats the CSR sweep.
@@ -437,18 +458,26 @@ void AmgEngine::release() {
     if (c_) (void)hipStreamSynchronize(c_->stream);
     for (auto& l : levels_) free_level(*c_, l, opt_.precision == 32);
     levels_.clear();
-    for (void* p : {static_cast<void*>(lu_), static_cast<void*>(piv_), static_cast<void*>(inverse_),
-                    static_cast<void*>(tailLevels_), static_cast<void*>(tailOps_), tailImage_})
+    for (void* p : {static_cast<void*>(lu_), static_cast<void*>(piv_), static_cast<void*>(inverse_)})
         dev_free(*c_, p);
-    tailImage_ = nullptr;
-    tailImageBytes_ = 0;
+    free_tail();
     lu_ = nullptr;
     piv_ = nullptr;
     inverse_ = nullptr;
+    factored_ = singular_ = false;
+}
+
+void AmgEngine::free_tail() {
+    if (tailLevels_ || tailOps_ || tailImage_) {
+        if (c_) (void)hipStreamSynchronize(c_->stream);
+        for (void* p : {static_cast<void*>(tailLevels_), static_cast<void*>(tailOps_), tailImage_})
+            dev_free(*c_, p);
+    }
+    tailImage_ = nullptr;
+    tailImageBytes_ = 0;
     tailLevels_ = nullptr;
     tailOps_ = nullptr;
     tail_ = -1;
-    factored_ = singular_ = false;
 }
 
 namespace {
@@ -726,44 +755,54 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
 void AmgEngine::tune_tail() {
     if (tail_ < 0 || opt_.tail == 1) return;
     if (opt_.tail == 2) {
-        tail_ = -1;
+        free_tail();
         return;
     }
     int64_t sig = 0;
     for (const auto& l : levels_) sig = sig * 1000003 + l.native.nnz;
     const TuneKey key{2, levels_[0].native.n, sig, int64_t(opt_.cycle * 100 + opt_.pre * 10 + opt_.post),
                       opt_.precision};
-    bool useTail;
+    const int first = tail_;
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
-        auto it = g_tuned.find(key);
-        useTail = it != g_tuned.end() ? it->second : false;
-        if (it != g_tuned.end()) {
-            if (!useTail) tail_ = -1;
+        auto it = g_tail_start.find(key);
+        if (it != g_tail_start.end()) {
+            if (it->second < 0)
+                free_tail();
+            else if (it->second != first)
+                build_tail(it->second);
             return;
         }
     }
+    // Candidates: no tail, and the tail from the first level that fits its
+    // LDS budget or from one of the next two levels (the levels above the
+    // start then run as per-level kernels); the fastest cycle wins.
     const size_t n = size_t(levels_[0].native.n);
     double* tmp = dev_alloc<double>(*c_, 2 * n);
     fill_zero(*c_, tmp, 2 * n * sizeof(double));
-    const int t = tail_;
-    useTail = first_is_faster(
-        *c_,
-        [&] {
-            tail_ = t;
-            apply(tmp, tmp + n);
-        },
-        [&] {
-            tail_ = -1;
-            apply(tmp, tmp + n);
-        },
-        3);
+    auto time_cycle = [&] { return best_time(*c_, [&] { apply(tmp, tmp + n); }); };
+    tail_ = -1;
+    float best = time_cycle();
+    int chosen = -1;
+    tail_ = first;
+    for (int s = first; s < int(levels_.size()) && s <= first + 2; ++s) {
+        if (s != tail_) build_tail(s);
+        if (tail_ != s) break;
+        const float t = time_cycle();
+        if (t < best) {
+            best = t;
+            chosen = s;
+        }
+    }
     c_->sync();
     dev_free(*c_, tmp);
-    tail_ = useTail ? t : -1;
+    if (chosen < 0)
+        free_tail();
+    else if (chosen != tail_)
+        build_tail(chosen);
     std::fill(visits_.begin(), visits_.end(), 0);
     std::lock_guard<std::mutex> lock(g_tune_mutex);
-    g_tuned[key] = useTail;
+    g_tail_start[key] = chosen;
 }
 
 namespace {
@@ -848,8 +887,8 @@ struct TailGen {
 // Chooses the tail's first level and records its programs for every entry
 // state (cycle kind, zero start, first sweep already applied by the parent's
 // restriction).
-void AmgEngine::build_tail() {
-    tail_ = -1;
+void AmgEngine::build_tail(int from) {
+    free_tail();
     if (levels_.size() < 2 || (factored_ && levels_.back().native.n > kCoarseMax)) return;
     const size_t es = opt_.precision == 32 ? sizeof(float) : sizeof(double);
     auto al = [](size_t b) { return (b + 15) / 16 * 16; };
@@ -863,7 +902,7 @@ void AmgEngine::build_tail() {
         need[d] = need[d + 1] + b;
     }
     int t = -1;
-    for (size_t d = 1; d < levels_.size(); ++d)
+    for (size_t d = size_t(std::max(1, from)); d < levels_.size(); ++d)
         if (need[d] <= size_t(kTailLds) && levels_[d].native.n <= kTailMaxRows) {
             t = int(d);
             break;

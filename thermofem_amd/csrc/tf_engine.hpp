@@ -123,8 +123,11 @@ private:
                      int blocks = 0, const KCtl* ctl = nullptr);
     template <class L>
     void run_tail(int kind, bool zero);
-    void build_tail();
+    // Builds the tail from the first level at or below `from` that fits its
+    // LDS budget (tail_ = that level, or -1 when none fits).
+    void build_tail(int from = 1);
     void tune_tail();
+    void free_tail();
     void release();
     void setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci, const double* v, bool canonicalInput,
                          bool learned);
