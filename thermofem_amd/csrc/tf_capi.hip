@@ -52,9 +52,7 @@ tf_ctx* make_ctx(int device) {
     auto c = std::make_unique<tf_ctx>();
     c->device = device;
     check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-    check(hipMalloc(&c->partials, size_t(kMaxReduceBlocks) * kTickets * sizeof(double)), "hipMalloc partials");
-    check(hipMalloc(&c->tickets, kTickets * sizeof(unsigned)), "hipMalloc tickets");
-    check(hipMemset(c->tickets, 0, kTickets * sizeof(unsigned)), "tickets clear");
+    check(hipMalloc(&c->partials, size_t(kMaxReduceBlocks) * kReduceSlots * sizeof(double)), "hipMalloc partials");
     check(hipMalloc(&c->dscalars, 64 * sizeof(double)), "hipMalloc scalars");
     check(hipHostMalloc(&c->hscalars, 64 * sizeof(double), hipHostMallocDefault), "hipHostMalloc scalars");
     check(hipMalloc(&c->dflags, 64 * sizeof(int)), "hipMalloc flags");
@@ -72,7 +70,6 @@ void destroy_ctx(tf_ctx* c) {
         if (c->ws_ptr[i]) (void)hipFree(c->ws_ptr[i]);
     if (c->scan_tmp) (void)hipFree(c->scan_tmp);
     if (c->partials) (void)hipFree(c->partials);
-    if (c->tickets) (void)hipFree(c->tickets);
     if (c->dscalars) (void)hipFree(c->dscalars);
     if (c->hscalars) (void)hipHostFree(c->hscalars);
     if (c->dflags) (void)hipFree(c->dflags);

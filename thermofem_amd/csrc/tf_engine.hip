@@ -203,42 +203,39 @@ struct SellRows {
 };
 
 // Block partial of a reduction whose finaliser runs in k_kfinal.
-__device__ __forceinline__ void reduce_fin(double contrib, double* partials, unsigned* ticket, KCtl* ctl, int mode) {
-    (void)ticket;
-    (void)ctl;
-    (void)mode;
+__device__ __forceinline__ void block_partial(double contrib, double* partials) {
     contrib = block_sum(contrib);
     if (threadIdx.x == 0) partials[blockIdx.x] = contrib;
 }
 
 // dot(x, y) -> finalizer.
 __global__ __launch_bounds__(kBlock) void k_kdot(int n, const double* __restrict__ x, const double* __restrict__ y,
-                                                 double* partials, unsigned* ticket, KCtl* ctl, int mode) {
+                                                 double* partials, const KCtl* ctl) {
     if (stopped(ctl)) return;
     double s = 0;
     for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) s += x[i] * y[i];
-    reduce_fin(s, partials, ticket, ctl, mode);
+    block_partial(s, partials);
 }
 
 // y = A x fused with sum(w .* y) -> finalizer.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_kspmv_dot(R rows, const double* __restrict__ x, double* __restrict__ y,
                                                       const double* __restrict__ w, double* partials,
-                                                      unsigned* ticket, KCtl* ctl, int mode) {
+                                                      const KCtl* ctl) {
     if (stopped(ctl)) return;
     double contrib = 0;
     rows.each(x, [&](int row, double acc) {
         y[row] = acc;
         contrib += w[row] * acc;
     });
-    reduce_fin(contrib, partials, ticket, ctl, mode);
+    block_partial(contrib, partials);
 }
 
 // t = A x fused with (t.t, t.s): BiCgStab.cuh:214-226 (omega).
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_kspmv_omega(R rows, const double* __restrict__ x, double* __restrict__ t,
                                                         const double* __restrict__ s, double* partials,
-                                                        unsigned* tickets, KCtl* ctl) {
+                                                        const KCtl* ctl) {
     if (stopped(ctl) || ctl->early) return;
     double tt = 0, ts = 0;
     rows.each(x, [&](int row, double acc) {
@@ -259,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_kspmv_omega(R rows, const double* __
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_ktrue(R rows, const double* __restrict__ b, const double* __restrict__ x,
                                                   double* __restrict__ r, double* __restrict__ shadow,
-                                                  double* partials, unsigned* ticket, KCtl* ctl, int onEarly) {
+                                                  double* partials, const KCtl* ctl, int onEarly) {
     if (stopped(ctl)) return;
     if (onEarly ? !ctl->early : (!ctl->check || ctl->early)) return;
     double contrib = 0;
@@ -269,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_ktrue(R rows, const double* __restri
         if (shadow) shadow[row] = res;
         contrib += res * res;
     });
-    reduce_fin(contrib, partials, ticket, ctl, kFinTrue);
+    block_partial(contrib, partials);
 }
 
 // Finalisation of one reduction: adds the producer's block partials in
@@ -331,8 +328,8 @@ __global__ void k_cg_p(int n, const double* __restrict__ z, double* __restrict__
 // x += alpha p; r -= alpha q; sum(r.^2).
 __global__ __launch_bounds__(kBlock) void k_cg_update(int n, const double* __restrict__ p,
                                                       const double* __restrict__ q, double* __restrict__ x,
-                                                      double* __restrict__ r, double* partials, unsigned* ticket,
-                                                      KCtl* ctl) {
+                                                      double* __restrict__ r, double* partials,
+                                                      const KCtl* ctl) {
     if (stopped(ctl)) return;
     const double alpha = ctl->alpha;
     double s = 0;
@@ -342,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int n, const double* __res
         r[i] = ri;
         s += ri * ri;
     }
-    reduce_fin(s, partials, ticket, ctl, kFinUpdCg);
+    block_partial(s, partials);
 }
 
 // BiCGStab direction: p = r (fresh) or p = r + beta (p - omega v).
@@ -358,7 +355,7 @@ __global__ void k_bi_p(int n, const double* __restrict__ r, const double* __rest
 // s = r - alpha v; x += alpha ph; sum(s.^2).
 __global__ __launch_bounds__(kBlock) void k_bi_s(int n, const double* __restrict__ r, const double* __restrict__ v,
                                                  const double* __restrict__ ph, double* __restrict__ s,
-                                                 double* __restrict__ x, double* partials, unsigned* ticket,
+                                                 double* __restrict__ x, double* partials,
                                                  KCtl* ctl) {
     if (stopped(ctl)) return;
     const double alpha = ctl->alpha;
@@ -369,14 +366,14 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(int n, const double* __restrict
         x[i] = alpha * ph[i] + x[i];
         acc += si * si;
     }
-    reduce_fin(acc, partials, ticket, ctl, kFinSBi);
+    block_partial(acc, partials);
 }
 
 // x += omega shat; r = s - omega t; sum(r.^2).
 __global__ __launch_bounds__(kBlock) void k_bi_update(int n, const double* __restrict__ shat,
                                                       const double* __restrict__ s, const double* __restrict__ t,
                                                       double* __restrict__ x, double* __restrict__ r,
-                                                      double* partials, unsigned* ticket, KCtl* ctl) {
+                                                      double* partials, const KCtl* ctl) {
     if (stopped(ctl) || ctl->early) return;
     const double omega = ctl->omega;
     double acc = 0;
@@ -386,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(int n, const double* __res
         r[i] = ri;
         acc += ri * ri;
     }
-    reduce_fin(acc, partials, ticket, ctl, kFinUpdBi);
+    block_partial(acc, partials);
 }
 
 int vec_blocks(int n) {
@@ -1304,7 +1301,6 @@ void SolverEngine::launch_iteration(const R& rows, int nb) {
     KCtl* ctl = static_cast<KCtl*>(ctl_);
     const dim3 grid(std::min(round_xcd(nb), kMaxReduceBlocks)), vb(vec_blocks(n_));
     double* P = c_->partials;
-    unsigned* T = c_->tickets;
     auto pslot = [&](int s) { return P + size_t(s) * kMaxReduceBlocks; };
 #define TF_G(NAME, ...) hipLaunchKernelGGL(NAME, grid, dim3(kBlock), 0, c_->stream, __VA_ARGS__)
     const int ng = int(grid.x), nv = int(vb.x);
@@ -1320,35 +1316,33 @@ void SolverEngine::launch_iteration(const R& rows, int nb) {
                                pslot(kSlotA), ctl);
         fin(kSlotA, nv, kFinRhoCg, 0);
         hipLaunchKernelGGL(k_cg_p, vb, dim3(kBlock), 0, c_->stream, n_, z_, p_, ctl);
-        TF_G(k_kspmv_dot<R>, rows, p_, q_, p_, pslot(kSlotB), T + kSlotB, ctl, int(kFinCurvCg));
+        TF_G(k_kspmv_dot<R>, rows, p_, q_, p_, pslot(kSlotB), ctl);
         fin(kSlotB, ng, kFinCurvCg, 0);
-        hipLaunchKernelGGL(k_cg_update, vb, dim3(kBlock), 0, c_->stream, n_, p_, q_, x_, r_, pslot(kSlotC),
-                           T + kSlotC, ctl);
+        hipLaunchKernelGGL(k_cg_update, vb, dim3(kBlock), 0, c_->stream, n_, p_, q_, x_, r_, pslot(kSlotC), ctl);
         fin(kSlotC, nv, kFinUpdCg, 0);
-        TF_G(k_ktrue<R>, rows, rhs_, x_, r_, static_cast<double*>(nullptr), pslot(kSlotD), T + kSlotD, ctl, 0);
+        TF_G(k_ktrue<R>, rows, rhs_, x_, r_, static_cast<double*>(nullptr), pslot(kSlotD), ctl, 0);
         fin(kSlotD, ng, kFinTrue, 0);
     }
     else {
         double* v = q_;
-        hipLaunchKernelGGL(k_kdot, vb, dim3(kBlock), 0, c_->stream, n_, sh_, r_, pslot(kSlotA), T + kSlotA, ctl,
-                           int(kFinRhoBi));
+        hipLaunchKernelGGL(k_kdot, vb, dim3(kBlock), 0, c_->stream, n_, sh_, r_, pslot(kSlotA), ctl);
         fin(kSlotA, nv, kFinRhoBi, 0);
         hipLaunchKernelGGL(k_bi_p, vb, dim3(kBlock), 0, c_->stream, n_, r_, v, p_, ctl);
         precondition(p_, ph_);
-        TF_G(k_kspmv_dot<R>, rows, ph_, v, sh_, pslot(kSlotB), T + kSlotB, ctl, int(kFinDenBi));
+        TF_G(k_kspmv_dot<R>, rows, ph_, v, sh_, pslot(kSlotB), ctl);
         fin(kSlotB, ng, kFinDenBi, 0);
-        hipLaunchKernelGGL(k_bi_s, vb, dim3(kBlock), 0, c_->stream, n_, r_, v, ph_, s_, x_, pslot(kSlotC), T + kSlotC,
+        hipLaunchKernelGGL(k_bi_s, vb, dim3(kBlock), 0, c_->stream, n_, r_, v, ph_, s_, x_, pslot(kSlotC),
                            ctl);
         fin(kSlotC, nv, kFinSBi, 0);
-        TF_G(k_ktrue<R>, rows, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 1);
+        TF_G(k_ktrue<R>, rows, rhs_, x_, r_, sh_, pslot(kSlotD), ctl, 1);
         fin(kSlotD, ng, kFinTrue, 1);
         precondition(s_, shat_);
-        TF_G(k_kspmv_omega<R>, rows, shat_, t_, s_, pslot(kSlotE), T + kSlotE, ctl);
+        TF_G(k_kspmv_omega<R>, rows, shat_, t_, s_, pslot(kSlotE), ctl);
         fin(kSlotE, ng, kFinOmega, 0);
         hipLaunchKernelGGL(k_bi_update, vb, dim3(kBlock), 0, c_->stream, n_, shat_, s_, t_, x_, r_, pslot(kSlotA),
-                           T + kSlotA, ctl);
+                           ctl);
         fin(kSlotA, nv, kFinUpdBi, 0);
-        TF_G(k_ktrue<R>, rows, rhs_, x_, r_, sh_, pslot(kSlotD), T + kSlotD, ctl, 0);
+        TF_G(k_ktrue<R>, rows, rhs_, x_, r_, sh_, pslot(kSlotD), ctl, 0);
         fin(kSlotD, ng, kFinTrue, 0);
     }
 #undef TF_G

@@ -51,7 +51,7 @@ inline void require(bool ok, const char* msg, int code = TF_E_INVALID) {
 }
 
 constexpr int kMaxReduceBlocks = 16384;
-constexpr int kTickets = 64;
+constexpr int kReduceSlots = 64;
 
 // Scoped current-device switch that restores the caller's device
 // (OperationScope, src/LinearAlgebra/Backend.cu:116-126).
@@ -117,8 +117,7 @@ struct tf_ctx {
     int device = 0;
     DevicePool pool;
     hipStream_t stream = nullptr;
-    double* partials = nullptr;   // per-block partial sums (kMaxReduceBlocks x kTickets slots)
-    unsigned* tickets = nullptr;  // last-block tickets, zero between launches
+    double* partials = nullptr;   // per-block partial sums (kMaxReduceBlocks x kReduceSlots slots)
     double* dscalars = nullptr;   // device result scalars
     double* hscalars = nullptr;   // pinned host mirror
     int* dflags = nullptr;        // device integer flags/counters
@@ -243,7 +242,7 @@ template <class T>
 void axpy(tf_ctx& c, int n, T alpha, const T* x, T* y);
 template <class T>
 void scal(tf_ctx& c, int n, T alpha, T* x);
-// Sum of x*y into the device scalar dst (slot selects partial/ticket storage).
+// Sum of x*y into the device scalar dst (slot selects the partial-sum storage).
 template <class T>
 void dot_device(tf_ctx& c, int n, const T* x, const T* y, double* dst, int slot);
 template <class A, class B>

@@ -314,7 +314,7 @@ void scal(tf_ctx& c, int n, T alpha, T* x) {
 }
 template <class T>
 void dot_device(tf_ctx& c, int n, const T* x, const T* y, double* dst, int slot) {
-    require(slot >= 0 && slot < kTickets, "reduction slot out of range");
+    require(slot >= 0 && slot < kReduceSlots, "reduction slot out of range");
     const int blocks = blocks_for(size_t(n > 0 ? n : 1), 8192);
     double* part = c.partials + size_t(slot) * kMaxReduceBlocks;
     hipLaunchKernelGGL(k_dot<T>, dim3(blocks), dim3(kBlock), 0, c.stream, n, x, y, part);
