@@ -196,14 +196,15 @@ int tf_prolong_f64(tf_ctx* ctx, int n, const int* map, const double* coarse, dou
 int tf_prolong_f32(tf_ctx* ctx, int n, const int* map, const float* coarse, float* fine);
 
 /* ---- coarse dense LU: factor (:827-893) and triangular (:904-939), widened to n <= 1024 ----
- * lu (n*n row-major, device), pivots (n, device), inverse (n*n row-major, device, may be NULL).
+ * lu (n*n row-major, device), pivots (n, device), inverse (n*n column-major: inverse[j*n + i] = (A^-1)_ij;
+ * device, may be NULL).
  * *host_singular = 1 when a pivot is <= 64*eps*max|A| or a factor is non-finite. */
 int tf_coarse_lu_factor_f64(tf_ctx* ctx, int n, const int* rp, const int* ci, const double* v, double* lu,
                             int* pivots, double* inverse, int* host_singular);
 /* x = A^-1 b by permuted forward/backward substitution in FP64 (rhs and x may alias). */
 int tf_coarse_lu_solve_f64(tf_ctx* ctx, int n, const double* lu, const int* pivots, const double* b, double* x);
 int tf_coarse_lu_solve_f32(tf_ctx* ctx, int n, const double* lu, const int* pivots, const float* b, float* x);
-/* x = inverse * b in FP64 (the per-cycle coarse solve; rhs and x may alias). */
+/* x = A^-1 b in FP64 from the column-major inverse above (the per-cycle coarse solve; rhs and x may alias). */
 int tf_coarse_inv_apply_f64(tf_ctx* ctx, int n, const double* inverse, const double* b, double* x);
 int tf_coarse_inv_apply_f32(tf_ctx* ctx, int n, const double* inverse, const float* b, float* x);
 

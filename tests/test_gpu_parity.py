@@ -429,3 +429,65 @@ def test_cycles_c1_fp32_bit_exact(cycle, tail):
     assert np.array_equal(dev.apply(b32), expected)
     assert np.array_equal(dev.apply(b32), expected)
     assert list(dev.last_cycle_visits()) == list(host.visits())
+
+
+def with_hub(a, b, stride, eps=1e-3):
+    """Adds unknown n coupled by -eps to every stride-th unknown (their
+    diagonals raised by eps; the new diagonal is eps * couplings + 1), so the
+    matrix stays symmetric positive definite and row n is longer than 255
+    entries on every level of the hierarchy."""
+    rp, ci, v = (np.asarray(t) for t in a)
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    hubs = np.arange(0, n, stride)
+    is_hub = np.zeros(n, bool)
+    is_hub[hubs] = True
+    v = v.copy()
+    v[(ci == rows) & is_hub[rows]] += eps
+    rp2 = np.zeros(n + 2, np.int64)
+    rp2[1:n + 1] = np.cumsum(np.diff(rp) + is_hub)
+    rp2[n + 1] = rp2[n] + len(hubs) + 1
+    ci2 = np.empty(rp2[-1], np.int32)
+    v2 = np.empty(rp2[-1])
+    pos = rp2[rows] + (np.arange(len(ci)) - rp[rows])
+    ci2[pos] = ci
+    v2[pos] = v
+    ci2[rp2[hubs + 1] - 1] = n
+    v2[rp2[hubs + 1] - 1] = -eps
+    ci2[rp2[n]:] = np.append(hubs, n)
+    v2[rp2[n]:] = np.append(np.full(len(hubs), -eps), eps * len(hubs) + 1.0)
+    return (rp2.astype(np.int32), ci2, v2), np.append(b, 0.5)
+
+
+@pytest.mark.parametrize("precision,n,hub", [(32, 44, 0), (64, 60, 0), (32, 44, 7), (64, 60, 5)])
+def test_tail_row_sums_bit_exact(precision, n, hub):
+    """heat3d log-normal 44^3 (FP32 levels) and 60^3 (FP64 levels), as is and
+    with an unknown coupled to every 7th / 5th cell.  As is, the coarse levels
+    keep sliced-ELL copies and the one-workgroup coarse program sums their
+    rows from them.  With the coupled unknown every level has a row of more
+    than 255 entries, so no level has a sliced-ELL copy and the program sums
+    rows from CSR: its first level (1,990 rows / 35,514 entries in FP32,
+    1,430 / 26,516 in FP64) has more entries than one load pass covers and
+    than its LDS product area holds, so every row sum is carried across
+    passes and chunks.  Both ways the program repeats the per-level kernels'
+    arithmetic, so the cycle is bit-identical with the level-by-level cycle
+    (and, with FP32 levels, with the oracle's)."""
+    a, b = P.heat3d_lognormal(n, seed=3)
+    if hub:
+        a, b = with_hub(a, b, hub)
+    kw = dict(coarse_rows=256, level_precision=precision, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0)
+    whole = Amg(AmgOptions(coarse_tail=1, **kw)).setup(a)
+    split = Amg(AmgOptions(coarse_tail=2, **kw)).setup(a)
+    if precision == 32:
+        b = b.astype(np.float32).astype(np.float64)
+    x = whole.apply(b)
+    log(f"tail row sums n={n} hub={hub} precision={precision} levels {whole.level_count()}")
+    assert np.array_equal(x, split.apply(b))
+    assert np.array_equal(whole.apply(b), x)
+    host = O.Amg(coarse_rows=256, level_precision=precision, pre_sweeps=2, post_sweeps=2,
+                 jacobi_weight=2.0 / 3.0).setup(a)
+    expected = host.apply(b)
+    if precision == 32:
+        assert np.array_equal(x, expected)
+    else:
+        assert np.allclose(x, expected, rtol=0, atol=1e-10 * np.abs(expected).max())

@@ -351,20 +351,27 @@ __global__ void k_bconvert(size_t count, const S* __restrict__ s, D* __restrict_
         d[i] = D(s[i]);
 }
 
-// x = inv * b on the dense coarsest level, one wave per (row, column): lane
-// l adds inv[r][l + 64u] * b[l + 64u] for u = 0, 1, ... in FP64, then the
-// wave reduction -- the operations coarse_inverse_apply performs for one
-// vector.
+// x = A^-1 b on the dense coarsest level from the column-major inverse, one
+// wave per (row, column): lane s < coarse_segments(n) forms the row's
+// partial over column run s (fused multiply-adds, columns ascending) and
+// lane 0 adds the partials in run order -- the operations
+// coarse_inverse_apply performs for one vector.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_bcoarse(int n, int k, const double* __restrict__ inv,
+__global__ __launch_bounds__(kBlock) void k_bcoarse(int n, int k, const double* __restrict__ it,
                                                     const T* __restrict__ b, T* __restrict__ x) {
     const int lane = int(threadIdx.x) & (kWave - 1);
     const int item = int(blockIdx.x) * (kBlock / kWave) + int(threadIdx.x) / kWave;
     if (item >= n * k) return;
     const int r = item / k, j = item % k;
-    double acc = 0;
-    for (int c = lane; c < n; c += kWave) acc += inv[size_t(r) * n + c] * double(b[size_t(c) * size_t(k) + j]);
-    acc = wave_sum(acc);
+    const int segs = coarse_segments(n), run = coarse_run(n);
+    double p = 0;
+    if (lane < segs) {
+        const int c1 = min(n, (lane + 1) * run);
+        for (int c = lane * run; c < c1; ++c)
+            p = fma(it[size_t(c) * size_t(n) + size_t(r)], double(b[size_t(c) * size_t(k) + j]), p);
+    }
+    double acc = __shfl(p, 0);
+    for (int s = 1; s < segs; ++s) acc += __shfl(p, s);
     if (lane == 0) x[size_t(r) * size_t(k) + j] = T(acc);
 }
 

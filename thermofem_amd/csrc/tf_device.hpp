@@ -215,51 +215,48 @@ __device__ __forceinline__ T sell_row_sum(int row, int len, const T* __restrict_
     return acc;
 }
 
-// x = inv * b for the dense coarsest level (order n <= kCoarseMax), one
-// 1024-thread block: b is staged in LDS (bs) as FP64 first, so rhs and x may
-// alias; wave w forms rows w, w+16, ... from lane-strided partial sums (lane
-// l adds entries l, l+64, ... in that order) and a wave reduction.  Up to
-// order 256 the loads of four rows are issued together.  All threads of the
-// block must call it.
+// x = A^-1 b for the dense coarsest level (order n <= kCoarseMax) from the
+// column-major inverse (it[j * n + r] = (A^-1)_rj), one 1024-thread block.
+// b is staged in LDS (bs) as FP64 first, so rhs and x may alias.  The
+// columns are cut into coarse_segments(n) runs of coarse_run(n) consecutive
+// columns; thread s * n + r forms row r's partial over run s (fused
+// multiply-adds, columns ascending, starting from 0; the loads, coalesced
+// over r, are issued 8 at a time before their first use), the partials meet
+// in part (segments * n <= 1024 doubles) and thread r adds them in run
+// order.  All threads of the block must call it.
 constexpr int kCoarseMax = 1024;
+__host__ __device__ constexpr int coarse_segments(int n) {
+    return n >= 1024 ? 1 : (1024 / n < 16 ? 1024 / n : 16);
+}
+__host__ __device__ constexpr int coarse_run(int n) {
+    return (n + coarse_segments(n) - 1) / coarse_segments(n);
+}
 template <class T>
-__device__ __forceinline__ void coarse_inverse_apply(int n, const double* __restrict__ inv, const T* b, T* x,
-                                                     double* bs) {
-    for (int i = int(threadIdx.x); i < n; i += 1024) bs[i] = double(b[i]);
+__device__ __forceinline__ void coarse_inverse_apply(int n, const double* __restrict__ it, const T* b, T* x,
+                                                     double* bs, double* part) {
+    const int tid = int(threadIdx.x);
+    for (int i = tid; i < n; i += 1024) bs[i] = double(b[i]);
     __syncthreads();
-    const int lane = int(threadIdx.x) & (kWave - 1), w = int(threadIdx.x) / kWave;
-    constexpr int RB = 4, UB = 4;
-    if (n <= UB * kWave) {
-        for (int r0 = w; r0 < n; r0 += 16 * RB) {
-            double a[RB][UB];
+    const int segs = coarse_segments(n), run = coarse_run(n);
+    if (tid < segs * n) {
+        const int s = tid / n, r = tid - s * n;
+        const int j0 = s * run, j1 = min(n, j0 + run);
+        double p = 0;
+        for (int jb = j0; jb < j1; jb += 8) {
+            double a[8];
 #pragma unroll
-            for (int q = 0; q < RB; ++q) {
-                const int r = min(r0 + 16 * q, n - 1);
+            for (int u = 0; u < 8; ++u) a[u] = it[size_t(min(jb + u, j1 - 1)) * size_t(n) + size_t(r)];
 #pragma unroll
-                for (int u = 0; u < UB; ++u) a[q][u] = inv[size_t(r) * n + min(lane + kWave * u, n - 1)];
-            }
-#pragma unroll
-            for (int q = 0; q < RB; ++q) {
-                double acc = 0;
-#pragma unroll
-                for (int u = 0; u < UB; ++u) {
-                    const int j = lane + kWave * u;
-                    if (j < n) acc += a[q][u] * bs[j];
-                }
-                const int r = r0 + 16 * q;
-                if (r < n) {
-                    acc = wave_sum(acc);
-                    if (lane == 0) x[r] = T(acc);
-                }
-            }
+            for (int u = 0; u < 8; ++u)
+                if (jb + u < j1) p = fma(a[u], bs[jb + u], p);
         }
-        return;
+        part[tid] = p;
     }
-    for (int r = w; r < n; r += 1024 / kWave) {
-        double acc = 0;
-        for (int j = lane; j < n; j += kWave) acc += inv[size_t(r) * n + j] * bs[j];
-        acc = wave_sum(acc);
-        if (lane == 0) x[r] = T(acc);
+    __syncthreads();
+    for (int r = tid; r < n; r += 1024) {
+        double acc = part[r];
+        for (int s = 1; s < segs; ++s) acc += part[s * n + r];
+        x[r] = T(acc);
     }
 }
 

@@ -465,13 +465,15 @@ void AmgEngine::release() {
 }
 
 void AmgEngine::free_tail() {
-    if (tailLevels_ || tailOps_ || tailImage_) {
+    if (tailLevels_ || tailOps_ || tailImage_ || tailPack_) {
         if (c_) (void)hipStreamSynchronize(c_->stream);
-        for (void* p : {static_cast<void*>(tailLevels_), static_cast<void*>(tailOps_), tailImage_})
+        for (void* p : {static_cast<void*>(tailLevels_), static_cast<void*>(tailOps_), tailImage_, tailPack_})
             dev_free(*c_, p);
     }
     tailImage_ = nullptr;
+    tailPack_ = nullptr;
     tailImageBytes_ = 0;
+    tailLdsBytes_ = 0;
     tailLevels_ = nullptr;
     tailOps_ = nullptr;
     tail_ = -1;
@@ -528,7 +530,9 @@ void AmgEngine::swap_hierarchy(AmgEngine& o) {
     std::swap(tailLevels_, o.tailLevels_);
     std::swap(tailOps_, o.tailOps_);
     std::swap(tailImage_, o.tailImage_);
+    std::swap(tailPack_, o.tailPack_);
     std::swap(tailImageBytes_, o.tailImageBytes_);
+    std::swap(tailLdsBytes_, o.tailLdsBytes_);
     std::swap(tailPrograms_, o.tailPrograms_);
     std::swap(factored_, o.factored_);
     std::swap(singular_, o.singular_);
@@ -889,12 +893,14 @@ void AmgEngine::build_tail(int from) {
     if (levels_.size() < 2 || (factored_ && levels_.back().native.n > kCoarseMax)) return;
     const size_t es = opt_.precision == 32 ? sizeof(float) : sizeof(double);
     auto al = [](size_t b) { return (b + 15) / 16 * 16; };
-    // LDS bytes of levels d..coarsest: five vectors, row offsets and the transfer.
+    // LDS bytes of levels d..coarsest: five vectors, row offsets, the transfer
+    // and the packed slices' offsets.
+    auto slices_of = [](int n) { return (n + kWave - 1) / kWave; };
     std::vector<size_t> need(levels_.size() + 1, 0);
     for (size_t d = levels_.size(); d-- > 0;) {
         const AmgLevel& l = levels_[d];
         const size_t n = size_t(l.native.n);
-        size_t b = 5 * al(n * es) + al((n + 1) * 4);
+        size_t b = 5 * al(n * es) + al((n + 1) * 4) + al((size_t(slices_of(l.native.n)) + 1) * 4);
         if (l.transfer.nc > 0) b += 2 * al(n * 4) + al((size_t(l.transfer.nc) + 1) * 4);
         need[d] = need[d + 1] + b;
     }
@@ -927,10 +933,6 @@ void AmgEngine::build_tail(int from) {
         e.map = l.transfer.map;
         e.off = l.transfer.offsets;
         e.mem = l.transfer.members;
-        e.lb = place(n * es);
-        e.lx0 = place(n * es);
-        e.lx1 = place(n * es);
-        e.lr = place(n * es);
         e.linv = place(n * es);
         e.lrp = place((n + 1) * 4);
         if (e.nc > 0) {
@@ -938,6 +940,21 @@ void AmgEngine::build_tail(int from) {
             e.lmem = place(n * 4);
             e.loff = place((size_t(e.nc) + 1) * 4);
         }
+        e.pv = nullptr;
+        e.pcol = nullptr;
+        e.slices = 0;
+        e.lsbase = place((size_t(slices_of(e.n)) + 1) * 4);
+    }
+    // The constant arrays above form the image staged at each launch; the
+    // vectors go after them.
+    const size_t imageBytes = al(at);
+    for (size_t d = size_t(t); d < levels_.size(); ++d) {
+        TailLevel& e = table[d];
+        const size_t n = size_t(e.n);
+        e.lb = place(n * es);
+        e.lx0 = place(n * es);
+        e.lx1 = place(n * es);
+        e.lr = place(n * es);
     }
     std::vector<TailOp> all;
     for (int key = 0; key < 12; ++key) {
@@ -955,10 +972,45 @@ void AmgEngine::build_tail(int from) {
         tailPrograms_[key].visits = g.visits;
         all.insert(all.end(), g.ops.begin(), g.ops.end());
     }
-    // LDS image: each level's inverse diagonal, row offsets and transfer at
-    // their LDS offsets (the vector slots in between are left undefined).
-    tailImageBytes_ = int(al(at));
-    tailImage_ = dev_alloc<char>(*c_, size_t(tailImageBytes_));
+    // Packed copies of the levels whose rows the program sums (all but a
+    // factored coarsest): slice widths from the row lengths, rounded up to
+    // groups of 4; a level whose padding would pass 2x its entries (a few
+    // very long rows) keeps CSR row sums.
+    const size_t last = factored_ ? levels_.size() - 1 : levels_.size();
+    std::vector<std::vector<int>> rows(levels_.size()), bases(levels_.size());
+    for (size_t d = size_t(t); d < last; ++d) {
+        rows[d].resize(size_t(levels_[d].native.n) + 1);
+        check(hipMemcpyAsync(rows[d].data(), levels_[d].native.rp, rows[d].size() * sizeof(int),
+                             hipMemcpyDeviceToHost, c_->stream),
+              "tail row offsets");
+    }
+    c_->sync();
+    size_t packBytes = 0;
+    std::vector<size_t> packAt(levels_.size(), 0);
+    for (size_t d = size_t(t); d < last; ++d) {
+        const int n = levels_[d].native.n, slices = slices_of(n);
+        std::vector<int>& base = bases[d];
+        base.assign(size_t(slices) + 1, 0);
+        int64_t total = 0;
+        for (int s = 0; s < slices; ++s) {
+            int w = 0;
+            for (int i = s * kWave; i < std::min(n, (s + 1) * kWave); ++i) w = std::max(w, rows[d][i + 1] - rows[d][i]);
+            total += int64_t((w + 3) / 4) * 4 * kWave;
+            base[size_t(s) + 1] = int(std::min<int64_t>(total, INT32_MAX));
+        }
+        if (total > 2 * int64_t(levels_[d].native.nnz) + 4 * kWave * int64_t(slices)) {
+            base.clear();
+            continue;
+        }
+        packAt[d] = packBytes;
+        packBytes += al(size_t(total) * es) + al(size_t(total) * 2);
+    }
+    if (packBytes > 0) tailPack_ = dev_alloc<char>(*c_, packBytes);
+    // LDS image: each level's inverse diagonal, row offsets, transfer and
+    // packed slices' offsets at their LDS offsets.
+    tailImageBytes_ = int(imageBytes);
+    tailLdsBytes_ = int(al(at));
+    tailImage_ = dev_alloc<char>(*c_, imageBytes);
     char* img = static_cast<char*>(tailImage_);
     auto put = [&](int off, const void* src, size_t bytes) {
         if (bytes) check(hipMemcpyAsync(img + off, src, bytes, hipMemcpyDeviceToDevice, c_->stream), "tail image");
@@ -973,6 +1025,30 @@ void AmgEngine::build_tail(int from) {
             put(e.lmem, e.mem, n * 4);
             put(e.loff, e.off, (size_t(e.nc) + 1) * 4);
         }
+    }
+    // packed levels: slice offsets into the image, matrices into tailPack_
+    for (size_t d = size_t(t); d < last; ++d) {
+        const std::vector<int>& base = bases[d];
+        if (base.empty()) continue;
+        TailLevel& e = table[d];
+        const AmgLevel& l = levels_[d];
+        const int slices = int(base.size()) - 1;
+        const size_t total = size_t(base.back());
+        char* at0 = static_cast<char*>(tailPack_) + packAt[d];
+        unsigned short* pcol = reinterpret_cast<unsigned short*>(at0 + al(total * es));
+        check(hipMemcpyAsync(img + e.lsbase, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice,
+                             c_->stream),
+              "tail slice offsets");
+        const int* dbase = reinterpret_cast<const int*>(img + e.lsbase);
+        if (opt_.precision == 32)
+            tail_pack<float>(*c_, e.n, slices, l.native.rp, l.native.ci, static_cast<const float*>(l.vals), dbase,
+                             reinterpret_cast<float*>(at0), pcol);
+        else
+            tail_pack<double>(*c_, e.n, slices, l.native.rp, l.native.ci, static_cast<const double*>(l.vals), dbase,
+                              reinterpret_cast<double*>(at0), pcol);
+        e.pv = at0;
+        e.pcol = pcol;
+        e.slices = slices;
     }
     tailLevels_ = dev_alloc<TailLevel>(*c_, table.size());
     tailOps_ = dev_alloc<TailOp>(*c_, all.size());
@@ -989,7 +1065,7 @@ template <class L>
 void AmgEngine::run_tail(int kind, bool zero) {
     AmgLevel& l = levels_[size_t(tail_)];
     const TailProgram& p = tailPrograms_[kind * 4 + (zero ? 2 : 0) + (l.preapplied ? 1 : 0)];
-    tail_run<L>(*c_, tailLevels_, tail_, int(levels_.size()) - tail_, tailImage_, tailImageBytes_,
+    tail_run<L>(*c_, tailLevels_, tail_, int(levels_.size()) - tail_, tailImage_, tailImageBytes_, tailLdsBytes_,
                 tailOps_ + p.offset, p.count, inverse_, L(opt_.omega));
     for (size_t d = 0; d < visits_.size(); ++d) visits_[d] += p.visits[d];
     l.preapplied = false;

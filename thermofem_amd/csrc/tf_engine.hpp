@@ -57,17 +57,33 @@ struct TailLevel {
     const int* map;
     const int* off;
     const int* mem;
-    // byte offsets of this level's arrays in the workgroup's LDS
-    int lb, lx0, lx1, lr, linv, lrp, lmap, loff, lmem;
+    // the level's packed copy for the program's row sums (slices of 64 rows,
+    // each lane's entries in groups of 4 contiguous ones), when it has one
+    const void* pv;
+    const unsigned short* pcol;
+    int slices;     // 0: row sums from the CSR arrays
+    // byte offsets of this level's arrays in the workgroup's LDS (lsbase:
+    // the packed slices' entry offsets)
+    int lb, lx0, lx1, lr, linv, lrp, lmap, loff, lmem, lsbase;
 };
-// LDS bytes for the tail's vectors and index arrays.
+// LDS bytes for the tail's vectors and index arrays, and for those plus the
+// product area of the CSR row sums, which gets what they leave (at least
+// 40 KB).
 constexpr int kTailLds = 104 * 1024;
+constexpr int kTailLdsAll = 143 * 1024;
 constexpr int kTailMaxRows = 4096;  // rows of a tail level (4 per thread)
 constexpr int kMaxTailLevels = 20;
 constexpr int kMaxTailOps = 256;    // operations of one program
 template <class L>
 void tail_run(tf_ctx& c, const TailLevel* levels, int top, int count, const void* image, int imageBytes,
-              const TailOp* ops, int nops, const double* inverse, L w);
+              int ldsBytes, const TailOp* ops, int nops, const double* inverse, L w);
+// Packed copy of a tail level (n <= kTailMaxRows, so columns fit 16 bits):
+// entry k of row 64 s + l at base[s] + 256 (k / 4) + 4 l + k % 4; slots past
+// a row's end hold value 0 and column 0.  base (device, slices + 1 entry
+// offsets, multiples of 256) gives each slice's width.
+template <class L>
+void tail_pack(tf_ctx& c, int n, int slices, const int* rp, const int* ci, const L* v, const int* base, L* pv,
+               unsigned short* pcol);
 
 class BlockAmg;
 struct KCtl;
@@ -143,7 +159,9 @@ private:
     TailLevel* tailLevels_ = nullptr;
     TailOp* tailOps_ = nullptr;
     void* tailImage_ = nullptr;  // the levels' constant arrays in their LDS layout
+    void* tailPack_ = nullptr;   // the levels' packed matrices
     int tailImageBytes_ = 0;
+    int tailLdsBytes_ = 0;      // image plus the levels' vectors
     TailProgram tailPrograms_[12];
 
     tf_ctx* c_;

@@ -444,9 +444,10 @@ __global__ __launch_bounds__(kInvThreads) void k_lu_inverse(int n, const double*
         }
         __syncthreads();
     }
+    // column-major output: column j0 + t is contiguous
     for (int i = tid; i < n * kNb; i += kInvThreads) {
-        const int r = i / kNb, t = i % kNb;
-        if (t < live) inv[size_t(r) * n + j0 + t] = X[i];
+        const int t = i / n, r = i - t * n;
+        if (t < live) inv[size_t(j0 + t) * n + r] = X[r * kNb + t];
     }
 }
 

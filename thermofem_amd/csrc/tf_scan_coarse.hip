@@ -86,11 +86,13 @@ void scan_rec(tf_ctx& c, int n, const int* in, int* out, int* scratch) {
 
 constexpr int kLuThreads = 1024;
 
-// x = inv * b in FP64; b is staged in LDS first so rhs and x may alias.
+// x = A^-1 b in FP64 from the column-major inverse; b is staged in LDS first
+// so rhs and x may alias.
 template <class T>
 __global__ __launch_bounds__(kLuThreads) void k_inv_apply(int n, const double* __restrict__ inv, const T* b, T* x) {
     __shared__ double bs[kCoarseMax];
-    coarse_inverse_apply<T>(n, inv, b, x, bs);
+    __shared__ double part[kLuThreads];
+    coarse_inverse_apply<T>(n, inv, b, x, bs, part);
 }
 
 // Triangular solve of GpuAggregation.cu:904-939 (pivots, forward, backward)
