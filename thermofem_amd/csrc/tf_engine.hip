@@ -221,6 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_kdot(int n, const double* __restrict
                                                  double* partials, const KCtl* ctl) {
     if (stopped(ctl)) return;
     double s = 0;
+#pragma unroll 4
     for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) s += x[i] * y[i];
     block_partial(s, partials);
 }
@@ -315,6 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_out_dot(int n, const L* __restrict__
                                                     double* partials, const KCtl* ctl) {
     if (stopped(ctl)) return;
     double s = 0;
+#pragma unroll 4
     for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
         const double zi = zero ? 0.0 : double(xl[i]);
         z[i] = zi;
@@ -329,6 +331,7 @@ __global__ void k_cg_p(int n, const double* __restrict__ z, double* __restrict__
     if (stopped(ctl)) return;
     const int fresh = ctl->fresh;
     const double beta = ctl->beta;
+#pragma unroll 4
     for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock)
         p[i] = fresh ? z[i] : z[i] + beta * p[i];
 }
@@ -341,6 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int n, const double* __res
     if (stopped(ctl)) return;
     const double alpha = ctl->alpha;
     double s = 0;
+#pragma unroll 4
     for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
         x[i] = alpha * p[i] + x[i];
         const double ri = -alpha * q[i] + r[i];
@@ -356,6 +360,7 @@ __global__ void k_bi_p(int n, const double* __restrict__ r, const double* __rest
     if (stopped(ctl)) return;
     const int fresh = ctl->fresh;
     const double beta = ctl->beta, omega = ctl->omega;
+#pragma unroll 4
     for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock)
         p[i] = fresh ? r[i] : 1.0 * r[i] + beta * (-omega * v[i] + p[i]);
 }
@@ -368,6 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(int n, const double* __restrict
     if (stopped(ctl)) return;
     const double alpha = ctl->alpha;
     double acc = 0;
+#pragma unroll 4
     for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
         const double si = -alpha * v[i] + r[i];
         s[i] = si;
@@ -385,6 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(int n, const double* __res
     if (stopped(ctl) || ctl->early) return;
     const double omega = ctl->omega;
     double acc = 0;
+#pragma unroll 4
     for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
         x[i] = omega * shat[i] + x[i];
         const double ri = -omega * t[i] + s[i];
@@ -394,9 +401,12 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(int n, const double* __res
     block_partial(acc, partials);
 }
 
+// Element-wise Krylov kernels: at most 2048 blocks (8 per CU, all resident
+// at once), each thread walking its elements with the loop unrolled by 4 so
+// that several elements' loads are in flight.
 int vec_blocks(int n) {
     int b = (n + kBlock - 1) / kBlock;
-    if (b > 8192) b = 8192;
+    if (b > 2048) b = 2048;
     return b < 1 ? 1 : b;
 }
 
