@@ -227,6 +227,7 @@ template <class T>
 __global__ void k_enter(int n, const double* __restrict__ rhs, T* __restrict__ b, const T* __restrict__ inv, T w,
                         T* __restrict__ x) {
 #pragma clang fp contract(off)
+#pragma unroll 4
     for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
         const T bi = T(rhs[i]);
         b[i] = bi;
@@ -240,8 +241,9 @@ __global__ void k_enter(int n, const double* __restrict__ rhs, T* __restrict__ b
 // fine += P coarse, or fine = P coarse when the fine iterate is known zero.
 template <class T>
 __global__ void k_prolong(int n, const int* __restrict__ map, const T* __restrict__ cvec, T* __restrict__ f, int set) {
-    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i < n) f[i] = set ? cvec[map[i]] : f[i] + cvec[map[i]];
+#pragma unroll 4
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock)
+        f[i] = set ? cvec[map[i]] : f[i] + cvec[map[i]];
 }
 
 // Inverse diagonal: duplicate diagonal entries summed in stored order.
@@ -357,15 +359,16 @@ void prolong_jacobi(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci,
 template <class T>
 void enter_level(tf_ctx& c, int n, const double* rhs, T* b, const T* inv, T w, T* x) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_enter<T>, dim3(blocks_for(size_t(n), 8192)), dim3(kBlock), 0, c.stream, n, rhs, b, inv, w,
+    // at most 8 resident blocks a CU, each thread walking several elements
+    hipLaunchKernelGGL(k_enter<T>, dim3(blocks_for(size_t(n), 2048)), dim3(kBlock), 0, c.stream, n, rhs, b, inv, w,
                        x);
     check(hipGetLastError(), "level entry launch");
 }
 template <class T>
 void prolong_agg(tf_ctx& c, int n, const int* map, const T* coarse, T* fine, bool set) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_prolong<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, n, map, coarse, fine,
-                       int(set));
+    hipLaunchKernelGGL(k_prolong<T>, dim3(blocks_for(size_t(n), 2048)), dim3(kBlock), 0, c.stream, n, map, coarse,
+                       fine, int(set));
     check(hipGetLastError(), "prolong launch");
 }
 template <class T>
