@@ -149,8 +149,25 @@ __global__ void k_restrict(int nc, const int* __restrict__ off, const int* __res
 #pragma clang fp contract(off)
     const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (c >= nc) return;
+    const int k0 = off[c], k1 = off[c + 1];
     T s = 0;
-    for (int k = off[c]; k < off[c + 1]; ++k) s += f[mem[k]];
+    if (k1 > k0) {
+        // the first four members' loads and gathers are issued together
+        // (slots past the aggregate re-read its last member and are not
+        // added), then added in member order
+        int mm[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) mm[u] = mem[min(k0 + u, k1 - 1)];
+        T fv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) fv[u] = f[mm[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const T t = s + fv[u];
+            s = k0 + u < k1 ? t : s;
+        }
+        for (int k = k0 + 4; k < k1; ++k) s += f[mem[k]];
+    }
     cvec[c] = s;
     if (xc) {
         const T wd = w * inv[c];
