@@ -18,8 +18,8 @@
 //
 // Every column is computed with the arithmetic of the single-vector kernels
 // (row sums of rounded products added in stored order, unfused epilogues,
-// sequential restriction sums, the coarse inverse's lane partials and wave
-// reduction), so column j of a block cycle equals the single-vector cycle
+// sequential restriction sums, the coarse inverse's column-run partials
+// added in run order), so column j of a block cycle equals the single-vector cycle
 // of column j bit for bit.  Krylov dot products are per column but reduced
 // in a different order than the single-vector solver's.
 #include <algorithm>
@@ -328,8 +328,20 @@ __global__ void k_brestrict(int nc, int k, const int* __restrict__ off, const in
     const size_t nk = size_t(nc) * size_t(k);
     for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
         const int c = int(i / size_t(k)), j = int(i % size_t(k));
+        const int q0 = off[c], q1 = off[c + 1];
         T s = 0;
-        for (int q = off[c]; q < off[c + 1]; ++q) s += f[size_t(mem[q]) * size_t(k) + size_t(j)];
+        if (q1 > q0) {
+            // the first four members' loads issued together, added in order
+            T fv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) fv[u] = f[size_t(mem[min(q0 + u, q1 - 1)]) * size_t(k) + size_t(j)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const T t = s + fv[u];
+                s = q0 + u < q1 ? t : s;
+            }
+            for (int q = q0 + 4; q < q1; ++q) s += f[size_t(mem[q]) * size_t(k) + size_t(j)];
+        }
         coarse[i] = s;
     }
 }
