@@ -37,6 +37,7 @@ __device__ __forceinline__ T block_sum(T v) {
     __syncthreads();
     T r = 0;
     if (threadIdx.x == 0) {
+#pragma unroll 16
         for (int i = 0; i < int(blockDim.x) / kWave; ++i) r += partial[i];
     }
     __syncthreads();
@@ -49,6 +50,7 @@ __device__ __forceinline__ T block_sum(T v) {
 // shared counter are used: across XCDs those serialise at the memory side.
 __device__ __forceinline__ double sum_partials(const double* __restrict__ partials, int nparts) {
     double s = 0;
+#pragma unroll 8
     for (int i = int(threadIdx.x); i < nparts; i += int(blockDim.x)) s += partials[i];
     return block_sum(s);
 }
