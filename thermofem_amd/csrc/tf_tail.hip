@@ -4,7 +4,9 @@
 // by its bytes; here the whole sub-cycle (sweeps, residuals, restrictions,
 // prolongations, the dense coarse solve) runs between workgroup barriers in
 // a single launch, with every vector and index array of those levels held in
-// LDS, so an operation costs one round of independent matrix loads.  Each operation repeats the arithmetic of the
+// LDS and each level matrix read from a packed copy (each lane's row in
+// contiguous groups of 4 entries), so an operation costs about one round of
+// independent matrix loads.  Each operation repeats the arithmetic of the
 // corresponding per-level kernel exactly: sequential row sums of rounded
 // products, the same epilogue order, restriction over members in ascending
 // order and the same dense inverse apply, so results do not depend on where
