@@ -491,3 +491,28 @@ def test_tail_row_sums_bit_exact(precision, n, hub):
         assert np.array_equal(x, expected)
     else:
         assert np.allclose(x, expected, rtol=0, atol=1e-10 * np.abs(expected).max())
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_shared_amg_set_up_again_recaptures(precision):
+    """Two CG solvers share one AMG hierarchy; setting up the second on another
+    matrix rebuilds the hierarchy behind the first one's captured iteration.
+    The first solver must re-capture (its graph named the freed level
+    buffers) and still solve its own system: with the hierarchy of 2 A the
+    preconditioner is M / 2, to which CG is invariant."""
+    a, b = P.heat3d_lognormal(24, seed=3)
+    opts = AmgOptions(coarse_rows=64, pre_sweeps=2, post_sweeps=2, jacobi_weight=2 / 3, level_precision=precision)
+    amg = Amg(opts)
+    s1 = Cg.with_amg(amg, KrylovOptions(relative_tolerance=1e-10)).setup(a)
+    x1, r1 = s1.apply(b)
+    assert r1.status == SolveStatus.Converged
+    s2 = Cg.with_amg(amg, KrylovOptions(relative_tolerance=1e-10)).setup((a[0], a[1], 2.0 * a[2]))
+    x2, r2 = s2.apply(b)
+    assert r2.status == SolveStatus.Converged
+    assert np.linalg.norm(2.0 * x2 - x1) <= 1e-7 * np.linalg.norm(x1)
+    x1b, r1b = s1.apply(b)
+    log(f"shared AMG prec={precision}: it {r1.iterations} / {r2.iterations} / {r1b.iterations}")
+    assert r1b.status == SolveStatus.Converged
+    assert np.linalg.norm(b - O.spmv(a, x1b)) <= 1.0001e-10 * np.linalg.norm(b)
+    assert abs(r1b.iterations - r1.iterations) <= 1
+    assert np.linalg.norm(x1b - x1) <= 1e-8 * np.linalg.norm(x1)

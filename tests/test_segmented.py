@@ -115,3 +115,30 @@ def test_segments_edge_cases():
         Cg(KrylovOptions(), None).setup(coupled).apply_segmented(off, b)
     with pytest.raises(ValueError):
         BiCgStab(KrylovOptions(), None).setup(a).apply_segmented(off, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [64, 32])
+def test_non_finite_segment_does_not_spread(precision):
+    """One segment's right-hand side holds a NaN (and another's an Inf): those
+    segments stop NonFinite at once, every other segment converges as if
+    alone (the shared cycle only ever sees zeros in the stopped segments,
+    and the coarse inverse skips its zero couplings)."""
+    AmgOptions, BiCgStab, Cg, KrylovOptions, SolveStatus = _solver()
+    a, b, off, edges = P.heat3d_batch(count=5, lo=6, hi=12)
+    b_bad = b.copy()
+    b_bad[off[1] + 3] = np.nan
+    b_bad[off[3]] = np.inf
+    amg = AmgOptions(coarse_rows=256, pre_sweeps=2, post_sweeps=2, jacobi_weight=2 / 3, level_precision=precision)
+    s = Cg(KrylovOptions(relative_tolerance=1e-8), amg).setup(a)
+    x_ok, res_ok = s.apply_segmented(off, b)
+    x, res = s.apply_segmented(off, b_bad)
+    for k in range(len(edges)):
+        seg = slice(off[k], off[k + 1])
+        print(f"segment {k}: {res[k].status.name} it {res[k].iterations}", flush=True)
+        if k in (1, 3):
+            assert res[k].status == SolveStatus.NonFinite
+            continue
+        assert res[k].status == SolveStatus.Converged
+        assert res[k].iterations == res_ok[k].iterations
+        assert np.array_equal(x[seg], x_ok[seg])

@@ -25,6 +25,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "tf_device.hpp"
 #include "tf_engine.hpp"
@@ -181,7 +184,11 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(int n, int k, const int* __res
     const bool live = col < k;
     const int cc = live ? col : k - 1;
     bool active = live;
-    if constexpr (D == kBTrue) active = live && !stopped(ctl + cc) && ctl[cc].check && !ctl[cc].early;
+    if constexpr (D == kBTrue) {
+        // launched every iteration; only columns due for a refresh compute
+        active = live && !stopped(ctl + cc) && ctl[cc].check && !ctl[cc].early;
+        if (!__syncthreads_or(active ? 1 : 0)) return;  // (their partials are never read)
+    }
     double contrib = 0;
     int first, step, end;
     block_groups((n + R - 1) / R, first, step, end);
@@ -379,8 +386,10 @@ __global__ __launch_bounds__(kBlock) void k_bcoarse(int n, int k, const double* 
     double p = 0;
     if (lane < segs) {
         const int c1 = min(n, (lane + 1) * run);
-        for (int c = lane * run; c < c1; ++c)
-            p = fma(it[size_t(c) * size_t(n) + size_t(r)], double(b[size_t(c) * size_t(k) + j]), p);
+        for (int c = lane * run; c < c1; ++c) {
+            const double a = it[size_t(c) * size_t(n) + size_t(r)];
+            if (a != 0.0) p = fma(a, double(b[size_t(c) * size_t(k) + j]), p);
+        }
     }
     double acc = __shfl(p, 0);
     for (int s = 1; s < segs; ++s) acc += __shfl(p, s);
@@ -391,11 +400,12 @@ unsigned flat_blocks(size_t count) {
     return unsigned(std::max<size_t>(1, std::min<size_t>((count + kBlock - 1) / kBlock, 8192)));
 }
 
+// G: lanes per row = columns per pass (0: lanes_for(k)).
 template <class T, int E, int D>
 void bspmm(tf_ctx& c, int n, int k, const int* rp, const int* ci, const T* v, const T* X, T* Y, const T* B,
-           const T* inv, T w, double* partials, const KCtl* ctl) {
+           const T* inv, T w, double* partials, const KCtl* ctl, int G = 0) {
     if (n <= 0 || k <= 0) return;
-    const int G = lanes_for(k);
+    if (G <= 0) G = lanes_for(k);
     const int R = kBlock / G;
     const dim3 grid(row_grid((n + R - 1) / R, D != kBNone), (k + G - 1) / G);
 #define TF_BS(GG)                                                                                              \
@@ -409,8 +419,8 @@ void bspmm(tf_ctx& c, int n, int k, const int* rp, const int* ci, const T* v, co
 }
 
 // Number of partials a row-parallel launch over n rows writes per column.
-int row_parts(int n, int k) {
-    const int R = kBlock / lanes_for(k);
+int row_parts(int n, int k, int G = 0) {
+    const int R = kBlock / (G > 0 ? G : lanes_for(k));
     return row_grid((n + R - 1) / R);
 }
 
@@ -445,6 +455,61 @@ void bcg_update(tf_ctx& c, int n, int k, const double* p, const double* q, doubl
 void bfinal(tf_ctx& c, int k, const double* partials, int nparts, KCtl* ctl, int mode, double* out) {
     hipLaunchKernelGGL(k_bfinal, dim3(1), dim3(1024), 0, c.stream, k, partials, nparts, ctl, mode, out);
     check(hipGetLastError(), "block finaliser launch");
+}
+
+// Autotuned right-hand-side tile width: the columns one pass of a block
+// SpMM covers (= lanes per row).  Narrower tiles read the matrix once per
+// tile but keep a smaller window of gathered block rows in the XCD's L2;
+// which wins depends on the operator and k, so each candidate (the widest,
+// lanes_for(k), then halves down to 8 columns) is timed on the operator
+// itself -- best of 3 runs of two launches each -- and the fastest kept, per
+// operator shape, for the process.  Every width gives the same bits.
+struct LaneKey {
+    int op, n;
+    int64_t nnz;
+    int k, bytes;
+    bool operator<(const LaneKey& o) const {
+        return std::tie(op, n, nnz, k, bytes) < std::tie(o.op, o.n, o.nnz, o.k, o.bytes);
+    }
+};
+std::map<LaneKey, int> g_lanes;
+std::mutex g_lanes_mutex;
+
+template <class F>
+int tuned_lanes(tf_ctx& c, const LaneKey& key, F&& run) {
+    const int widest = lanes_for(key.k);
+    if (key.k < 16 || key.nnz < 100000) return widest;
+    {
+        std::lock_guard<std::mutex> lock(g_lanes_mutex);
+        auto it = g_lanes.find(key);
+        if (it != g_lanes.end()) return it->second;
+    }
+    hipEvent_t e[2];
+    for (auto& ev : e) check(hipEventCreate(&ev), "hipEventCreate");
+    int best = widest;
+    float bestT = 1e30f;
+    for (int G = widest; G >= 8; G /= 2) {
+        run(G);
+        float t = 1e30f;
+        for (int i = 0; i < 3; ++i) {
+            check(hipEventRecord(e[0], c.stream), "event record");
+            run(G);
+            run(G);
+            check(hipEventRecord(e[1], c.stream), "event record");
+            check(hipEventSynchronize(e[1]), "event sync");
+            float ms = 0;
+            check(hipEventElapsedTime(&ms, e[0], e[1]), "event time");
+            t = std::min(t, ms);
+        }
+        if (t < bestT) {
+            bestT = t;
+            best = G;
+        }
+    }
+    for (auto& ev : e) (void)hipEventDestroy(ev);
+    std::lock_guard<std::mutex> lock(g_lanes_mutex);
+    g_lanes[key] = best;
+    return best;
 }
 
 }  // namespace
@@ -506,6 +571,31 @@ void BlockAmg::reserve(int k) {
     k_ = k;
     shape_ = shape;
     visits.assign(lv_.size(), 0);
+    if (amg_->opt_.precision == 32)
+        tune_lanes<float>();
+    else
+        tune_lanes<double>();
+}
+
+// Tile width of every level's sweeps and residuals (timed on the level's
+// Jacobi sweep from zero blocks).
+template <class L>
+void BlockAmg::tune_lanes() {
+    lanes_.assign(lv_.size(), 0);
+    const L w = L(amg_->opt_.omega);
+    for (size_t d = 0; d < lv_.size(); ++d) {
+        const AmgLevel& a = amg_->levels_[d];
+        const DevCsr& A = a.native;
+        const size_t bytes = size_t(A.n) * size_t(k_) * sizeof(L);
+        fill_zero(*c_, lv_[d].b, bytes);
+        fill_zero(*c_, lv_[d].x, bytes);
+        lanes_[d] = tuned_lanes(*c_, LaneKey{1, A.n, A.nnz, k_, int(sizeof(L))}, [&](int G) {
+            bspmm<L, kJacobi, kBNone>(*c_, A.n, k_, A.rp, A.ci, static_cast<const L*>(a.vals),
+                                      static_cast<const L*>(lv_[d].x), static_cast<L*>(lv_[d].x2),
+                                      static_cast<const L*>(lv_[d].b), static_cast<const L*>(a.inv), w, nullptr,
+                                      nullptr, G);
+        });
+    }
 }
 
 template <class L>
@@ -525,7 +615,7 @@ void BlockAmg::smooth(size_t d, int sweeps) {
         else {
             bspmm<L, kJacobi, kBNone>(*c_, A.n, k_, A.rp, A.ci, static_cast<const L*>(a.vals),
                                       static_cast<const L*>(l.x), static_cast<L*>(l.x2), static_cast<const L*>(l.b),
-                                      static_cast<const L*>(a.inv), w, nullptr, nullptr);
+                                      static_cast<const L*>(a.inv), w, nullptr, nullptr, lanes_[d]);
             std::swap(l.x, l.x2);
         }
     }
@@ -540,7 +630,7 @@ void BlockAmg::restrict_to(size_t d, bool ofResidual) {
     if (ofResidual && !l.zero) {
         bspmm<L, kResidual, kBNone>(*c_, a.native.n, k_, a.native.rp, a.native.ci, static_cast<const L*>(a.vals),
                                     static_cast<const L*>(l.x), static_cast<L*>(l.r), static_cast<const L*>(l.b),
-                                    nullptr, L(0), nullptr, nullptr);
+                                    nullptr, L(0), nullptr, nullptr, lanes_[d]);
         f = static_cast<const L*>(l.r);
     }
     const size_t nk = size_t(t.nc) * size_t(k_);
@@ -663,6 +753,8 @@ void SolverEngine::release_block() {
     hbctl_ = nullptr;
     hbsums_ = nullptr;
     bk_ = 0;
+    bLanes_ = 0;
+    bgraph_.reset();
 }
 
 // Column sums of a row-parallel reduction -> host.
@@ -673,14 +765,11 @@ static void column_sums(tf_ctx& c, int k, const double* partials, int nparts, do
     c.sync();
 }
 
-// CG (Cg.cuh:123-189) on k right-hand sides at once: every column follows
-// the single-vector recurrence on its own control block and stops on its
-// own; the preconditioner and the products run on the whole block.
-void SolverEngine::apply_block(int k, const double* B, double* X, tf_solve_result* out) {
-    require(n_ > 0, "CG requires setup", TF_E_INVALID);
-    require(kind_ == 0, "the multi-right-hand-side solve runs CG", TF_E_INVALID);
-    require(k >= 1, "block solve needs at least one right-hand side");
-    reserve_block(k);
+// One block CG iteration (Cg.cuh:138-187 per column) as a fixed launch
+// sequence: preconditioner, rho, direction, A p with p.Ap, the update with
+// |r|^2, and the true-residual refresh, which only columns due for one
+// compute.  Stopped columns are left as they are.
+void SolverEngine::launch_block_iteration(int k) {
     const size_t nk = size_t(n_) * size_t(k);
     double* bB = bvec_;
     double* bX = bvec_ + nk;
@@ -689,9 +778,54 @@ void SolverEngine::apply_block(int k, const double* B, double* X, tf_solve_resul
     double* bP = bvec_ + 4 * nk;
     double* bQ = bvec_ + 5 * nk;
     KCtl* ctl = static_cast<KCtl*>(bctl_);
+    const int np = row_parts(n_, k), npA = row_parts(n_, k, bLanes_);
+    if (amg_)
+        amg_->block().apply(k, bR, bZ);
+    else
+        hipLaunchKernelGGL((k_bconvert<double, double>), dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk, bR,
+                           bZ);
+    bdot(*c_, n_, k, bR, bZ, bparts_);
+    bfinal(*c_, k, bparts_, np, ctl, kFinRhoCg, nullptr);
+    hipLaunchKernelGGL(k_bcg_p, dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk, k, bZ, bP, ctl);
+    bspmm<double, kSpmv, kBDotX>(*c_, n_, k, A_.rp, A_.ci, A_.v, bP, bQ, nullptr, nullptr, 0.0, bparts_, ctl,
+                                 bLanes_);
+    bfinal(*c_, k, bparts_, npA, ctl, kFinCurvCg, nullptr);
+    bcg_update(*c_, n_, k, bP, bQ, bX, bR, bparts_, ctl);
+    bfinal(*c_, k, bparts_, np, ctl, kFinUpdCg, nullptr);
+    bspmm<double, kResidual, kBTrue>(*c_, n_, k, A_.rp, A_.ci, A_.v, bX, bR, bB, nullptr, 0.0, bparts_, ctl,
+                                     bLanes_);
+    bfinal(*c_, k, bparts_, npA, ctl, kFinTrue, nullptr);
+    check(hipGetLastError(), "block iteration launch");
+}
+
+// CG (Cg.cuh:123-189) on k right-hand sides at once: every column follows
+// the single-vector recurrence on its own control block and stops on its
+// own; the preconditioner and the products run on the whole block.  The
+// iteration is captured once as a hipGraph and launched in batches between
+// reads of the k control blocks (krylov_batch).
+void SolverEngine::apply_block(int k, const double* B, double* X, tf_solve_result* out) {
+    require(n_ > 0, "CG requires setup", TF_E_INVALID);
+    require(kind_ == 0, "the multi-right-hand-side solve runs CG", TF_E_INVALID);
+    require(k >= 1, "block solve needs at least one right-hand side");
+    reserve_block(k);
+    if (amg_) amg_->block().reserve(k);
+    const size_t nk = size_t(n_) * size_t(k);
+    double* bB = bvec_;
+    double* bX = bvec_ + nk;
+    double* bR = bvec_ + 2 * nk;
+    double* bP = bvec_ + 4 * nk;
+    double* bQ = bvec_ + 5 * nk;
+    KCtl* ctl = static_cast<KCtl*>(bctl_);
     KCtl* h = static_cast<KCtl*>(hbctl_);
     double* hs = static_cast<double*>(hbsums_);
-    const int np = row_parts(n_, k);
+    if (bLanes_ == 0) {
+        fill_zero(*c_, bP, nk * sizeof(double));
+        bLanes_ = tuned_lanes(*c_, LaneKey{0, n_, A_.nnz, k, 8}, [&](int G) {
+            bspmm<double, kSpmv, kBNone>(*c_, n_, k, A_.rp, A_.ci, A_.v, bP, bQ, nullptr, nullptr, 0.0, nullptr,
+                                         nullptr, G);
+        });
+    }
+    const int np = row_parts(n_, k), npA = row_parts(n_, k, bLanes_);
     const unsigned fb = flat_blocks(nk);
     hipLaunchKernelGGL((k_bconvert<double, double>), dim3(fb), dim3(kBlock), 0, c_->stream, nk, B, bB);
     hipLaunchKernelGGL((k_bconvert<double, double>), dim3(fb), dim3(kBlock), 0, c_->stream, nk, X, bX);
@@ -699,8 +833,8 @@ void SolverEngine::apply_block(int k, const double* B, double* X, tf_solve_resul
 
     auto residual_norms = [&](std::vector<double>& res) {
         bspmm<double, kResidual, kBSquares>(*c_, n_, k, A_.rp, A_.ci, A_.v, bX, bR, bB, nullptr, 0.0, bparts_,
-                                            nullptr);
-        column_sums(*c_, k, bparts_, np, bsums_, hs);
+                                            nullptr, bLanes_);
+        column_sums(*c_, k, bparts_, npA, bsums_, hs);
         res.resize(size_t(k));
         for (int j = 0; j < k; ++j) res[size_t(j)] = std::sqrt(hs[j]);
     };
@@ -735,31 +869,20 @@ void SolverEngine::apply_block(int k, const double* B, double* X, tf_solve_resul
         }
     }
     check(hipMemcpyAsync(ctl, h, sizeof(KCtl) * size_t(k), hipMemcpyHostToDevice, c_->stream), "block control upload");
-    auto download = [&]() {
+    const std::vector<double> res0 = res;
+    // the graph names the block work buffers (made for k columns) and the
+    // hierarchy's block buffers (made for its shape): both follow the
+    // preconditioner's generation and k
+    const uint64_t key = (amg_ ? amg_->generation() : 0) * 65536 + uint64_t(k);
+    int done = 0;
+    while (done < o_.maxit && running > 0) {
+        const int batch = std::min(krylov_batch(h, res0.data(), k, done), o_.maxit - done);
+        for (int q = 0; q < batch; ++q)
+            launch_captured(*c_, bgraph_, o_.graph, key, [&] { launch_block_iteration(k); });
+        done += batch;
         check(hipMemcpyAsync(h, ctl, sizeof(KCtl) * size_t(k), hipMemcpyDeviceToHost, c_->stream),
               "block control download");
         c_->sync();
-    };
-    for (int it = 0; it < o_.maxit && running > 0; ++it) {
-        if (amg_)
-            amg_->block().apply(k, bR, bZ);
-        else
-            hipLaunchKernelGGL((k_bconvert<double, double>), dim3(fb), dim3(kBlock), 0, c_->stream, nk, bR, bZ);
-        bdot(*c_, n_, k, bR, bZ, bparts_);
-        bfinal(*c_, k, bparts_, np, ctl, kFinRhoCg, nullptr);
-        hipLaunchKernelGGL(k_bcg_p, dim3(fb), dim3(kBlock), 0, c_->stream, nk, k, bZ, bP, ctl);
-        bspmm<double, kSpmv, kBDotX>(*c_, n_, k, A_.rp, A_.ci, A_.v, bP, bQ, nullptr, nullptr, 0.0, bparts_, ctl);
-        bfinal(*c_, k, bparts_, np, ctl, kFinCurvCg, nullptr);
-        bcg_update(*c_, n_, k, bP, bQ, bX, bR, bparts_, ctl);
-        bfinal(*c_, k, bparts_, np, ctl, kFinUpdCg, nullptr);
-        download();
-        bool refresh = false;
-        for (int j = 0; j < k; ++j) refresh |= h[j].status == kRunning && h[j].check && !h[j].early;
-        if (refresh) {
-            bspmm<double, kResidual, kBTrue>(*c_, n_, k, A_.rp, A_.ci, A_.v, bX, bR, bB, nullptr, 0.0, bparts_, ctl);
-            bfinal(*c_, k, bparts_, np, ctl, kFinTrue, nullptr);
-            download();
-        }
         running = 0;
         for (int j = 0; j < k; ++j) running += h[j].status == kRunning;
     }

@@ -225,7 +225,10 @@ __device__ __forceinline__ T sell_row_sum(int row, int len, const T* __restrict_
 // multiply-adds, columns ascending, starting from 0; the loads, coalesced
 // over r, are issued 8 at a time before their first use), the partials meet
 // in part (segments * n <= 1024 doubles) and thread r adds them in run
-// order.  All threads of the block must call it.
+// order.  Zero entries of the inverse are skipped, so a non-finite value in
+// b reaches only the rows the inverse couples to it (the coarse inverse of a
+// block-diagonal operator is block-diagonal: independent systems stay
+// independent).  All threads of the block must call it.
 constexpr int kCoarseMax = 1024;
 __host__ __device__ constexpr int coarse_segments(int n) {
     return n >= 1024 ? 1 : (1024 / n < 16 ? 1024 / n : 16);
@@ -250,7 +253,7 @@ __device__ __forceinline__ void coarse_inverse_apply(int n, const double* __rest
             for (int u = 0; u < 8; ++u) a[u] = it[size_t(min(jb + u, j1 - 1)) * size_t(n) + size_t(r)];
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-                if (jb + u < j1) p = fma(a[u], bs[jb + u], p);
+                if (jb + u < j1 && a[u] != 0.0) p = fma(a[u], bs[jb + u], p);
         }
         part[tid] = p;
     }

@@ -474,6 +474,7 @@ void free_level(tf_ctx& c, AmgLevel& l, bool f32) {
 }  // namespace
 
 void AmgEngine::release() {
+    ++generation_;
     if (c_) (void)hipStreamSynchronize(c_->stream);
     for (auto& l : levels_) free_level(*c_, l, opt_.precision == 32);
     levels_.clear();
@@ -487,6 +488,7 @@ void AmgEngine::release() {
 }
 
 void AmgEngine::free_tail() {
+    ++generation_;
     if (tailLevels_ || tailOps_ || tailImage_ || tailPack_) {
         if (c_) (void)hipStreamSynchronize(c_->stream);
         for (void* p : {static_cast<void*>(tailLevels_), static_cast<void*>(tailOps_), tailImage_, tailPack_})
@@ -543,6 +545,8 @@ void AmgEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const do
 }
 
 void AmgEngine::swap_hierarchy(AmgEngine& o) {
+    ++generation_;
+    ++o.generation_;
     (void)hipStreamSynchronize(c_->stream);
     std::swap(levels_, o.levels_);
     std::swap(visits_, o.visits_);
@@ -1081,6 +1085,7 @@ void AmgEngine::build_tail(int from) {
           "tail program upload");
     c_->sync();
     tail_ = t;
+    ++generation_;
 }
 
 template <class L>
@@ -1272,11 +1277,9 @@ SolverEngine::SolverEngine(tf_ctx* ctx, int kind, const KrylovOptionsC& o, AmgEn
 }
 
 void SolverEngine::release_graphs() {
-    for (auto& g : graph_)
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
+    graph_.reset();
+    bgraph_.reset();
+    sgraph_.reset();
 }
 
 SolverEngine::~SolverEngine() {
@@ -1377,9 +1380,8 @@ double SolverEngine::norm_host(const double* x) {
     return std::sqrt(c_->hscalars[0]);
 }
 
-// One iteration of the solver as a fixed launch sequence (part 0).
-void SolverEngine::launch_part(int part) {
-    (void)part;
+// One iteration of the solver as a fixed launch sequence.
+void SolverEngine::launch_part() {
     if (!As_) {
         const int rpb = stream_rows_per_block<double>(A_.n, A_.nnz);
         const int nb = (A_.n + rpb - 1) / rpb;
@@ -1448,30 +1450,6 @@ void SolverEngine::launch_iteration(const R& rows, int nb) {
     check(hipGetLastError(), "Krylov iteration launch");
 }
 
-void SolverEngine::capture_or_launch(int part) {
-    if (!o_.graph) {
-        launch_part(part);
-        return;
-    }
-    if (!graph_[part]) {
-        hipGraph_t graph = nullptr;
-        check(hipStreamBeginCapture(c_->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-        try {
-            launch_part(part);
-        }
-        catch (...) {
-            (void)hipStreamEndCapture(c_->stream, &graph);
-            if (graph) (void)hipGraphDestroy(graph);
-            throw;
-        }
-        check(hipStreamEndCapture(c_->stream, &graph), "end capture");
-        const hipError_t e = hipGraphInstantiate(&graph_[part], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        check(e, "graph instantiate");
-    }
-    check(hipGraphLaunch(graph_[part], c_->stream), "graph launch");
-}
-
 void SolverEngine::true_residual() {
     if (As_)
         sell_residual<double>(*c_, As_, Asv_, rhs_, x_, r_);
@@ -1537,14 +1515,11 @@ tf_solve_result SolverEngine::apply(const double* b, double* x) {
         const double res0 = res;
         int k = 0;
         while (k < o_.maxit) {
-            int batch = 1;
-            if (k >= 2 && h->res > 0 && h->res < res0) {
-                const double rate = std::pow(h->res / res0, 1.0 / k);
-                const double remain = std::log(tol_ / h->res) / std::log(rate);
-                if (std::isfinite(remain) && remain > 2) batch = std::min(8, int(remain) - 1);
-            }
-            batch = std::min(batch, o_.maxit - k);
-            for (int q = 0; q < batch; ++q) capture_or_launch(0);
+            const int batch = std::min(krylov_batch(h, &res0, 1, k), o_.maxit - k);
+            // the captured launches name the preconditioner's level buffers:
+            // a shared AMG set up again since the capture is re-captured
+            const uint64_t key = amg_ ? amg_->generation() : 0;
+            for (int q = 0; q < batch; ++q) launch_captured(*c_, graph_, o_.graph, key, [&] { launch_part(); });
             k += batch;
             check(hipMemcpyAsync(hctl_, ctl_, sizeof(KCtl), hipMemcpyDeviceToHost, c_->stream), "control download");
             c_->sync();

@@ -88,6 +88,47 @@ void tail_pack(tf_ctx& c, int n, int slices, const int* rp, const int* ci, const
 class BlockAmg;
 struct KCtl;
 
+// A launch sequence captured once as a hipGraph and replayed.  It names
+// device buffers by address, so it is valid only while they live: `key`
+// identifies the buffers it was captured for (owners change it when they
+// free or replace one), and a different key re-captures.
+struct Captured {
+    hipGraphExec_t exec = nullptr;
+    uint64_t key = 0;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+};
+// Runs launch() through `g` (captured on first use or when key changed), or
+// directly when use is false.
+template <class F>
+void launch_captured(tf_ctx& c, Captured& g, bool use, uint64_t key, F&& launch) {
+    if (!use) {
+        launch();
+        return;
+    }
+    if (g.exec && g.key != key) g.reset();
+    if (!g.exec) {
+        hipGraph_t graph = nullptr;
+        check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            launch();
+        }
+        catch (...) {
+            (void)hipStreamEndCapture(c.stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        check(hipStreamEndCapture(c.stream, &graph), "end capture");
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        check(e, "graph instantiate");
+        g.key = key;
+    }
+    check(hipGraphLaunch(g.exec, c.stream), "graph launch");
+}
+
 // Levels whose aggregate histograms setup downloads asynchronously (the
 // option's level limit).
 constexpr int kMaxHistLevels = 20;
@@ -119,6 +160,11 @@ public:
     double rho_plain() const { return rhoPlain_; }
     double convergence_estimate();
     int n() const { return levels_.empty() ? 0 : levels_[0].native.n; }
+    // Changes whenever a device buffer the cycle's launches name may have been
+    // freed or replaced (setup, release, tail rebuild, hierarchy swap): a
+    // captured graph of the cycle is valid only for the generation it was
+    // captured at.
+    uint64_t generation() const { return generation_; }
 
     std::vector<AmgLevel> levels_;
     std::vector<int64_t> visits_;
@@ -165,6 +211,7 @@ private:
     TailProgram tailPrograms_[12];
 
     tf_ctx* c_;
+    uint64_t generation_ = 0;
     int* histHost_ = nullptr;  // pinned: every level's aggregate-size histogram (4 ints a level)
     bool factored_ = false, singular_ = false;
     double* lu_ = nullptr;
@@ -184,6 +231,9 @@ public:
     BlockAmg(tf_ctx* ctx, AmgEngine* amg);
     ~BlockAmg();
     void apply(int k, const double* R, double* Z);  // Z = M^-1 R per column; R, Z distinct
+    // Makes the buffers (and tunes the launch widths) for k columns, so that
+    // apply(k, ...) allocates nothing and can be captured.
+    void reserve(int k);
     std::vector<int64_t> visits;
 
 private:
@@ -203,12 +253,14 @@ private:
     void prolong(size_t d);
     template <class L>
     void coarse(size_t d);
-    void reserve(int k);
     void release();
+    template <class L>
+    void tune_lanes();
 
     tf_ctx* c_;
     AmgEngine* amg_;
     int k_ = 0;
+    std::vector<int> lanes_;  // columns per pass of each level's SpMM (tuned)
     std::vector<int> shape_;  // level sizes and element size the buffers were made for
     std::vector<Lvl> lv_;
     void* mem_ = nullptr;
@@ -239,8 +291,10 @@ public:
 
 private:
     void precondition(const double* in, double* out);
-    void capture_or_launch(int part);
-    void launch_part(int part);
+    void launch_part();
+    void launch_block_iteration(int k);
+    struct Segments;
+    void launch_segment_iteration();
     template <class R>
     void launch_iteration(const R& rows, int nb);
     tf_solve_result finish(int status, int iterations);
@@ -265,7 +319,7 @@ private:
     double *sh_ = nullptr, *s_ = nullptr, *t_ = nullptr, *ph_ = nullptr, *shat_ = nullptr;
     void* ctl_ = nullptr;      // device control block
     void* hctl_ = nullptr;     // pinned host mirror
-    hipGraphExec_t graph_[2] = {nullptr, nullptr};
+    Captured graph_;  // one iteration (key: the preconditioner's generation)
     double normb_ = 0, tol_ = 0;
     // multi-right-hand-side work (allocated by the first block solve)
     int bk_ = 0;
@@ -275,10 +329,11 @@ private:
     double* bparts_ = nullptr;
     double* bsums_ = nullptr;
     void* hbsums_ = nullptr;
+    int bLanes_ = 0;          // columns per pass of the operator's SpMM (tuned)
+    Captured bgraph_;         // one block iteration
     // segmented batch work (tile table, control blocks, partials, vectors)
-    void* sbuf_ = nullptr;
-    void* hsbuf_ = nullptr;
-    int sSegments_ = 0, sTiles_ = 0;
+    Segments* seg_ = nullptr;
+    Captured sgraph_;         // one segmented iteration
 };
 
 }  // namespace tf

@@ -5,6 +5,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+
 namespace tf {
 
 struct KCtl {
@@ -113,6 +116,26 @@ __device__ inline void finalize(int mode, double total, KCtl* c) {
 }
 
 __device__ __forceinline__ bool stopped(const KCtl* c) { return c->status != kRunning; }
+
+// Iterations to launch before the host next reads the control blocks of
+// `count` systems that have done `done` iterations from initial residuals
+// res0: one less than the most iterations any running system's mean residual
+// reduction so far says it still needs, at least 1, at most 8.  An iteration
+// launched after a system has stopped leaves it unchanged (its Krylov
+// kernels return on the stopped status), so the batch never changes results.
+inline int krylov_batch(const KCtl* h, const double* res0, int count, int done) {
+    if (done < 2) return 1;
+    double most = 0;
+    for (int s = 0; s < count; ++s) {
+        if (h[s].status != kRunning) continue;
+        if (!(h[s].res > 0 && h[s].res < res0[s])) return 1;
+        const double rate = std::pow(h[s].res / res0[s], 1.0 / done);
+        const double remain = std::log(h[s].tol / h[s].res) / std::log(rate);
+        if (!std::isfinite(remain)) return 1;
+        most = std::max(most, remain);
+    }
+    return most > 2 ? std::min(8, int(most) - 1) : 1;
+}
 
 
 }  // namespace tf

@@ -174,107 +174,171 @@ __global__ __launch_bounds__(1024) void k_sfinal(int segments, const int* __rest
     }
 }
 
+// Preconditioner input: w = r on running segments, 0 on stopped ones, so a
+// segment that stopped on a non-finite value feeds nothing into the shared
+// cycle.
+__global__ __launch_bounds__(kBlock) void k_smask(const STile* __restrict__ tiles, int ntiles,
+                                                  const double* __restrict__ r, double* __restrict__ w,
+                                                  const KCtl* __restrict__ ctl) {
+    for (int t = int(blockIdx.x); t < ntiles; t += int(gridDim.x)) {
+        const STile tl = tiles[t];
+        if (int(threadIdx.x) >= tl.rows) continue;
+        const int row = tl.row0 + int(threadIdx.x);
+        w[row] = stopped(ctl + tl.seg) ? 0.0 : r[row];
+    }
+}
+
 }  // namespace
 
+// Device and pinned work of a segment layout, kept while the offsets stay
+// the same: tiles, tile offsets per segment, segment offsets, control
+// blocks, sums, per-tile partials and the vectors b, x, r, z, p, q, w.
+struct SolverEngine::Segments {
+    std::vector<int> off;
+    int segments = 0, tiles = 0;
+    void* dev = nullptr;
+    void* host = nullptr;
+    STile* dTiles = nullptr;
+    int* dTileOff = nullptr;
+    int* dOff = nullptr;
+    KCtl* ctl = nullptr;
+    double* sums = nullptr;
+    double* parts = nullptr;
+    double* v[7] = {};
+    KCtl* hctl = nullptr;
+    double* hsums = nullptr;
+};
+
 void SolverEngine::release_segments() {
-    if (!sbuf_) return;
+    sgraph_.reset();
+    if (!seg_) return;
     (void)hipStreamSynchronize(c_->stream);
-    dev_free(*c_, sbuf_);
-    (void)hipHostFree(hsbuf_);
-    sbuf_ = nullptr;
-    hsbuf_ = nullptr;
-    sSegments_ = 0;
-    sTiles_ = 0;
+    dev_free(*c_, seg_->dev);
+    if (seg_->host) (void)hipHostFree(seg_->host);
+    delete seg_;
+    seg_ = nullptr;
+}
+
+// One segmented CG iteration (Cg.cuh:138-187 per segment) as a fixed launch
+// sequence over every segment's tiles; stopped segments' tiles are skipped.
+void SolverEngine::launch_segment_iteration() {
+    Segments& S = *seg_;
+    const int T = S.tiles, segments = S.segments;
+    const dim3 grid(unsigned(std::max(1, std::min(T, kSegGrid))));
+    double *sB = S.v[0], *sX = S.v[1], *sR = S.v[2], *sZ = S.v[3], *sP = S.v[4], *sQ = S.v[5], *sW = S.v[6];
+    const KCtl* cctl = S.ctl;
+    auto fin = [&](int mode) {
+        hipLaunchKernelGGL(k_sfinal, dim3(1), dim3(1024), 0, c_->stream, segments, S.dTileOff, S.parts, S.ctl, mode,
+                           static_cast<double*>(nullptr));
+    };
+    hipLaunchKernelGGL(k_smask, grid, dim3(kBlock), 0, c_->stream, S.dTiles, T, sR, sW, cctl);
+    precondition(sW, sZ);
+    hipLaunchKernelGGL(k_sdot, grid, dim3(kBlock), 0, c_->stream, S.dTiles, T, sR, sZ, S.parts, cctl);
+    fin(kFinRhoCg);
+    hipLaunchKernelGGL(k_scg_p, grid, dim3(kBlock), 0, c_->stream, S.dTiles, T, sZ, sP, cctl);
+    hipLaunchKernelGGL(k_sspmv_dot, grid, dim3(kBlock), 0, c_->stream, S.dTiles, T, A_.rp, A_.ci, A_.v, sP, sQ,
+                       S.parts, cctl);
+    fin(kFinCurvCg);
+    hipLaunchKernelGGL(k_scg_update, grid, dim3(kBlock), 0, c_->stream, S.dTiles, T, sP, sQ, sX, sR, S.parts, cctl);
+    fin(kFinUpdCg);
+    hipLaunchKernelGGL(k_strue, grid, dim3(kBlock), 0, c_->stream, S.dTiles, T, A_.rp, A_.ci, A_.v, sB, sX, sR,
+                       S.parts, cctl, 0);
+    fin(kFinTrue);
+    check(hipGetLastError(), "segmented iteration launch");
 }
 
 // CG on every segment of the block-diagonal operator at once.  off: host
 // row offsets of the segments (segments + 1 entries, off[0] = 0,
-// off[segments] = n, nondecreasing).
+// off[segments] = n, nondecreasing).  The iteration is captured once per
+// segment layout as a hipGraph and launched in batches between reads of
+// the control blocks (krylov_batch); every segment stops on its own test.
 void SolverEngine::apply_segmented(int segments, const int* off, const double* b, double* x,
                                    tf_solve_result* out) {
     require(n_ > 0, "CG requires setup", TF_E_INVALID);
     require(kind_ == 0, "the segmented batch solve runs CG", TF_E_INVALID);
     require(segments >= 1 && off != nullptr, "segmented solve needs at least one segment");
     require(off[0] == 0 && off[segments] == n_, "segment offsets must start at 0 and end at the row count");
-    std::vector<STile> tiles;
-    std::vector<int> tileOff(size_t(segments) + 1, 0);
-    for (int s = 0; s < segments; ++s) {
-        require(off[s + 1] >= off[s], "segment offsets must be nondecreasing");
-        tileOff[size_t(s)] = int(tiles.size());
-        for (int r = off[s]; r < off[s + 1]; r += kTileRows) tiles.push_back({r, std::min(kTileRows, off[s + 1] - r), s, 0});
+    for (int s = 0; s < segments; ++s) require(off[s + 1] >= off[s], "segment offsets must be nondecreasing");
+    if (!seg_ || seg_->segments != segments || !std::equal(off, off + segments + 1, seg_->off.begin())) {
+        release_segments();
+        std::vector<STile> tiles;
+        std::vector<int> tileOff(size_t(segments) + 1, 0);
+        for (int s = 0; s < segments; ++s) {
+            tileOff[size_t(s)] = int(tiles.size());
+            for (int r = off[s]; r < off[s + 1]; r += kTileRows)
+                tiles.push_back({r, std::min(kTileRows, off[s + 1] - r), s, 0});
+        }
+        tileOff[size_t(segments)] = int(tiles.size());
+        const int T = int(tiles.size());
+        auto rnd = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t bTiles = rnd(sizeof(STile) * size_t(std::max(T, 1)));
+        const size_t bTOff = rnd(sizeof(int) * (size_t(segments) + 1));
+        const size_t bCtl = rnd(sizeof(KCtl) * size_t(segments));
+        const size_t bSums = rnd(sizeof(double) * size_t(segments));
+        const size_t bParts = rnd(sizeof(double) * size_t(std::max(T, 1)));
+        const size_t bVec = rnd(sizeof(double) * size_t(n_));
+        seg_ = new Segments();
+        Segments& S = *seg_;
+        S.off.assign(off, off + segments + 1);
+        S.segments = segments;
+        S.tiles = T;
+        S.dev = dev_alloc<char>(*c_, bTiles + 2 * bTOff + bCtl + bSums + bParts + 7 * bVec);
+        check(hipHostMalloc(&S.host, bCtl + bSums, hipHostMallocDefault), "hipHostMalloc segment control");
+        char* p = static_cast<char*>(S.dev);
+        auto take = [&](size_t bytes) {
+            char* q = p;
+            p += bytes;
+            return q;
+        };
+        S.dTiles = reinterpret_cast<STile*>(take(bTiles));
+        S.dTileOff = reinterpret_cast<int*>(take(bTOff));
+        S.dOff = reinterpret_cast<int*>(take(bTOff));
+        S.ctl = reinterpret_cast<KCtl*>(take(bCtl));
+        S.sums = reinterpret_cast<double*>(take(bSums));
+        S.parts = reinterpret_cast<double*>(take(bParts));
+        for (auto& q : S.v) q = reinterpret_cast<double*>(take(bVec));
+        S.hctl = static_cast<KCtl*>(S.host);
+        S.hsums = reinterpret_cast<double*>(static_cast<char*>(S.host) + bCtl);
+        check(hipMemcpyAsync(S.dTiles, tiles.data(), sizeof(STile) * size_t(T), hipMemcpyHostToDevice, c_->stream),
+              "tile upload");
+        check(hipMemcpyAsync(S.dTileOff, tileOff.data(), sizeof(int) * tileOff.size(), hipMemcpyHostToDevice,
+                             c_->stream),
+              "tile offset upload");
+        check(hipMemcpyAsync(S.dOff, off, sizeof(int) * (size_t(segments) + 1), hipMemcpyHostToDevice, c_->stream),
+              "segment offset upload");
+        // no entry may couple two segments
+        c_->clear_flags(1);
+        hipLaunchKernelGGL(k_seg_check, dim3(unsigned(std::max(1, std::min(T, kSegGrid)))), dim3(kBlock), 0,
+                           c_->stream, S.dTiles, T, A_.rp, A_.ci, S.dOff, c_->dflags);
+        check(hipGetLastError(), "segment check launch");
+        c_->read_flags(1);
+        if (c_->hflags[0] != 0) {
+            release_segments();
+            throw Error(TF_E_INVALID, "the operator couples two segments (it must be block diagonal)");
+        }
     }
-    tileOff[size_t(segments)] = int(tiles.size());
-    const int T = int(tiles.size());
-    // workspace: tiles, tile offsets, segment offsets, control blocks, sums,
-    // partials, then the vectors b, x, r, z, p, q
-    release_segments();
-    auto rnd = [](size_t b) { return (b + 255) / 256 * 256; };
-    const size_t bTiles = rnd(sizeof(STile) * size_t(std::max(T, 1)));
-    const size_t bTOff = rnd(sizeof(int) * (size_t(segments) + 1));
-    const size_t bOff = bTOff;
-    const size_t bCtl = rnd(sizeof(KCtl) * size_t(segments));
-    const size_t bSums = rnd(sizeof(double) * size_t(segments));
-    const size_t bParts = rnd(sizeof(double) * size_t(std::max(T, 1)));
-    const size_t bVec = rnd(sizeof(double) * size_t(n_));
-    const size_t total = bTiles + bTOff + bOff + bCtl + bSums + bParts + 6 * bVec;
-    sbuf_ = dev_alloc<char>(*c_, total);
-    check(hipHostMalloc(&hsbuf_, bCtl + bSums, hipHostMallocDefault), "hipHostMalloc segment control");
-    sSegments_ = segments;
-    sTiles_ = T;
-    char* p = static_cast<char*>(sbuf_);
-    STile* dTiles = reinterpret_cast<STile*>(p);
-    p += bTiles;
-    int* dTileOff = reinterpret_cast<int*>(p);
-    p += bTOff;
-    int* dOff = reinterpret_cast<int*>(p);
-    p += bOff;
-    KCtl* ctl = reinterpret_cast<KCtl*>(p);
-    p += bCtl;
-    double* dSums = reinterpret_cast<double*>(p);
-    p += bSums;
-    double* parts = reinterpret_cast<double*>(p);
-    p += bParts;
-    double* v[6];
-    for (auto& q : v) {
-        q = reinterpret_cast<double*>(p);
-        p += bVec;
-    }
-    double *sB = v[0], *sX = v[1], *sR = v[2], *sZ = v[3], *sP = v[4], *sQ = v[5];
-    KCtl* h = static_cast<KCtl*>(hsbuf_);
-    double* hs = reinterpret_cast<double*>(static_cast<char*>(hsbuf_) + bCtl);
-    check(hipMemcpyAsync(dTiles, tiles.data(), sizeof(STile) * size_t(T), hipMemcpyHostToDevice, c_->stream),
-          "tile upload");
-    check(hipMemcpyAsync(dTileOff, tileOff.data(), sizeof(int) * tileOff.size(), hipMemcpyHostToDevice, c_->stream),
-          "tile offset upload");
-    check(hipMemcpyAsync(dOff, off, sizeof(int) * (size_t(segments) + 1), hipMemcpyHostToDevice, c_->stream),
-          "segment offset upload");
+    Segments& S = *seg_;
+    const int T = S.tiles;
     const dim3 grid(unsigned(std::max(1, std::min(T, kSegGrid))));
-    // no entry may couple two segments
-    c_->clear_flags(1);
-    hipLaunchKernelGGL(k_seg_check, grid, dim3(kBlock), 0, c_->stream, dTiles, T, A_.rp, A_.ci, dOff, c_->dflags);
-    check(hipGetLastError(), "segment check launch");
-    c_->read_flags(1);
-    require(c_->hflags[0] == 0, "the operator couples two segments (it must be block diagonal)");
+    double *sB = S.v[0], *sX = S.v[1], *sR = S.v[2];
+    KCtl* h = S.hctl;
     convert<double, double>(*c_, size_t(n_), b, sB);
     convert<double, double>(*c_, size_t(n_), x, sX);
 
     auto sums = [&](double* dst) {
-        hipLaunchKernelGGL(k_sfinal, dim3(1), dim3(1024), 0, c_->stream, segments, dTileOff, parts, ctl, -1, dSums);
-        check(hipMemcpyAsync(hs, dSums, sizeof(double) * size_t(segments), hipMemcpyDeviceToHost, c_->stream),
+        hipLaunchKernelGGL(k_sfinal, dim3(1), dim3(1024), 0, c_->stream, segments, S.dTileOff, S.parts, S.ctl, -1,
+                           S.sums);
+        check(hipMemcpyAsync(S.hsums, S.sums, sizeof(double) * size_t(segments), hipMemcpyDeviceToHost, c_->stream),
               "segment sums download");
         c_->sync();
-        for (int s = 0; s < segments; ++s) dst[s] = hs[s];
-    };
-    auto fin = [&](int mode) {
-        hipLaunchKernelGGL(k_sfinal, dim3(1), dim3(1024), 0, c_->stream, segments, dTileOff, parts, ctl, mode,
-                           static_cast<double*>(nullptr));
+        for (int s = 0; s < segments; ++s) dst[s] = S.hsums[s];
     };
     std::vector<double> normb(static_cast<size_t>(segments)), res(static_cast<size_t>(segments));
-    hipLaunchKernelGGL(k_sdot, grid, dim3(kBlock), 0, c_->stream, dTiles, T, sB, sB, parts,
+    hipLaunchKernelGGL(k_sdot, grid, dim3(kBlock), 0, c_->stream, S.dTiles, T, sB, sB, S.parts,
                        static_cast<const KCtl*>(nullptr));
     sums(normb.data());
-    hipLaunchKernelGGL(k_strue, grid, dim3(kBlock), 0, c_->stream, dTiles, T, A_.rp, A_.ci, A_.v, sB, sX, sR, parts,
-                       static_cast<const KCtl*>(nullptr), 1);
+    hipLaunchKernelGGL(k_strue, grid, dim3(kBlock), 0, c_->stream, S.dTiles, T, A_.rp, A_.ci, A_.v, sB, sX, sR,
+                       S.parts, static_cast<const KCtl*>(nullptr), 1);
     sums(res.data());
     std::vector<double> tol(static_cast<size_t>(segments));
     int running = 0;
@@ -300,41 +364,26 @@ void SolverEngine::apply_segmented(int segments, const int* off, const double* b
             ++running;
         }
     }
-    check(hipMemcpyAsync(ctl, h, sizeof(KCtl) * size_t(segments), hipMemcpyHostToDevice, c_->stream),
+    check(hipMemcpyAsync(S.ctl, h, sizeof(KCtl) * size_t(segments), hipMemcpyHostToDevice, c_->stream),
           "segment control upload");
-    auto download = [&]() {
-        check(hipMemcpyAsync(h, ctl, sizeof(KCtl) * size_t(segments), hipMemcpyDeviceToHost, c_->stream),
+    const std::vector<double> res0 = res;
+    // the graph names the layout's buffers (released with it) and the
+    // preconditioner's level buffers
+    const uint64_t key = amg_ ? amg_->generation() : 0;
+    int done = 0;
+    while (done < o_.maxit && running > 0) {
+        const int batch = std::min(krylov_batch(h, res0.data(), segments, done), o_.maxit - done);
+        for (int q = 0; q < batch; ++q)
+            launch_captured(*c_, sgraph_, o_.graph, key, [&] { launch_segment_iteration(); });
+        done += batch;
+        check(hipMemcpyAsync(h, S.ctl, sizeof(KCtl) * size_t(segments), hipMemcpyDeviceToHost, c_->stream),
               "segment control download");
         c_->sync();
-    };
-    for (int it = 0; it < o_.maxit && running > 0; ++it) {
-        precondition(sR, sZ);
-        hipLaunchKernelGGL(k_sdot, grid, dim3(kBlock), 0, c_->stream, dTiles, T, sR, sZ, parts,
-                           static_cast<const KCtl*>(ctl));
-        fin(kFinRhoCg);
-        hipLaunchKernelGGL(k_scg_p, grid, dim3(kBlock), 0, c_->stream, dTiles, T, sZ, sP,
-                           static_cast<const KCtl*>(ctl));
-        hipLaunchKernelGGL(k_sspmv_dot, grid, dim3(kBlock), 0, c_->stream, dTiles, T, A_.rp, A_.ci, A_.v, sP, sQ,
-                           parts, static_cast<const KCtl*>(ctl));
-        fin(kFinCurvCg);
-        hipLaunchKernelGGL(k_scg_update, grid, dim3(kBlock), 0, c_->stream, dTiles, T, sP, sQ, sX, sR, parts,
-                           static_cast<const KCtl*>(ctl));
-        fin(kFinUpdCg);
-        check(hipGetLastError(), "segmented iteration launch");
-        download();
-        bool refresh = false;
-        for (int s = 0; s < segments; ++s) refresh |= h[s].status == kRunning && h[s].check && !h[s].early;
-        if (refresh) {
-            hipLaunchKernelGGL(k_strue, grid, dim3(kBlock), 0, c_->stream, dTiles, T, A_.rp, A_.ci, A_.v, sB, sX, sR,
-                               parts, static_cast<const KCtl*>(ctl), 0);
-            fin(kFinTrue);
-            download();
-        }
         running = 0;
         for (int s = 0; s < segments; ++s) running += h[s].status == kRunning;
     }
-    hipLaunchKernelGGL(k_strue, grid, dim3(kBlock), 0, c_->stream, dTiles, T, A_.rp, A_.ci, A_.v, sB, sX, sR, parts,
-                       static_cast<const KCtl*>(nullptr), 1);
+    hipLaunchKernelGGL(k_strue, grid, dim3(kBlock), 0, c_->stream, S.dTiles, T, A_.rp, A_.ci, A_.v, sB, sX, sR,
+                       S.parts, static_cast<const KCtl*>(nullptr), 1);
     sums(res.data());
     for (int s = 0; s < segments; ++s) {
         const double r = std::sqrt(res[size_t(s)]);

@@ -227,6 +227,24 @@ class _Krylov:
         self.handle = h
         self.device = device
 
+    @classmethod
+    def with_amg(cls, amg, options=None):
+        """A solver preconditioned by an existing Amg (tf_solver_create_with_amg):
+        the hierarchy is shared, so setting up either re-builds it for both."""
+        self = cls.__new__(cls)
+        self.options = options or KrylovOptions()
+        self.amg_options = amg.options
+        self._amg = amg  # the solver borrows the hierarchy: keep it alive
+        h = C.c_void_p()
+        o = self.options.c()
+        try:
+            check(lib().tf_solver_create_with_amg(self.kind, C.byref(o), amg.handle, C.byref(h)))
+        except TfError as e:
+            _raise(e)
+        self.handle = h
+        self.device = 0
+        return self
+
     def __del__(self):
         if getattr(self, "handle", None):
             lib().tf_solver_destroy(self.handle)
