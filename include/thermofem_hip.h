@@ -154,6 +154,12 @@ int tf_copy_f64(tf_ctx* ctx, size_t n, const double* src, double* dst);
 
 /* ---- scan: cub::DeviceScan::ExclusiveSum of GpuAggregation.cu:116-121 ---- */
 int tf_scan_excl_i32(tf_ctx* ctx, int n, const int* in, int* out); /* out has n entries */
+/* The same exclusive scan, allocating nothing: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops).  Its block totals live in a fixed
+ * 516 KB device array per GPU, so n <= 268435456 (2^28; larger n returns
+ * TF_E_INVALID); calls that overlap on one GPU share that array and race. */
+int tf_scan_excl_i32_noalloc(tf_ctx* ctx, int n, const int* in, int* out);
 
 /* ---- aggregation setup: GpuAggregation.cu (multigrid::GpuAggregation, GpuAggregation.cuh:34-69) ---- */
 /* checkCanonical (:283-304): *host_flags bit0 = unsorted or duplicate columns, bit1 = invalid. */

@@ -108,6 +108,7 @@ _SIGS = {
     "tf_convert_i32_i64": (i32, [vp, sz, vp, vp]),
     "tf_copy_f64": (i32, [vp, sz, vp, vp]),
     "tf_scan_excl_i32": (i32, [vp, i32, vp, vp]),
+    "tf_scan_excl_i32_noalloc": (i32, [vp, i32, vp, vp]),
     "tf_csr_check": (i32, [vp, i32, i64, vp, vp, vp, I32P]),
     "tf_csr_validate_structure": (i32, [vp, i32, i32, i64, vp, vp, I32P]),
     "tf_csr_canonicalize": (i32, [vp, i32, i64, vp, vp, vp, vp, C.POINTER(vp), C.POINTER(vp), I64P]),

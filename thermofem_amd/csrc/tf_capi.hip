@@ -414,6 +414,10 @@ int tf_copy_f64(tf_ctx* ctx, size_t n, const double* src, double* dst) {
 
 int tf_scan_excl_i32(tf_ctx* ctx, int n, const int* in, int* out) { TF_CTX_GUARD(ctx, scan_excl(*ctx, n, in, out)); }
 
+int tf_scan_excl_i32_noalloc(tf_ctx* ctx, int n, const int* in, int* out) {
+    TF_CTX_GUARD(ctx, require(n >= 0, "negative length"); scan_excl_noalloc(*ctx, n, in, out));
+}
+
 int tf_csr_check(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v, int* host_flags) {
     TF_CTX_GUARD(ctx, *host_flags = csr_check(*ctx, n, nnz, rp, ci, v));
 }

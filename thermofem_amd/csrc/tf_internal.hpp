@@ -264,6 +264,9 @@ template <class T>
 void inverse_diagonal(tf_ctx& c, int n, const int* rp, const int* ci, const T* v, T* inv, int* dbad);
 void fill_zero(tf_ctx& c, void* dst, size_t bytes);
 void scan_excl(tf_ctx& c, int n, const int* in, int* out);
+// The same scan with its block totals in a fixed device array: no allocation,
+// n <= 2^28 (tf_scan_coarse.hip).
+void scan_excl_noalloc(tf_ctx& c, int n, const int* in, int* out);
 template <class T>
 void coarse_inv_apply(tf_ctx& c, int n, const double* inverse, const T* b, T* x);
 template <class T>

@@ -2,6 +2,8 @@
 // per 256-thread block) and the per-cycle coarse solves: the FP64 dense
 // product with the explicit inverse and the pivoted triangular solve (the
 // factorisation and the inverse are built in tf_lu.hip).
+#include <mutex>
+
 #include "tf_device.hpp"
 #include "tf_internal.hpp"
 
@@ -127,15 +129,57 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_solve(int n, const double* __
 
 }  // namespace
 
-void scan_excl(tf_ctx& c, int n, const int* in, int* out) {
-    if (n <= 0) return;
+// Block totals every level of scan_rec stores for n items.
+size_t scan_scratch_ints(int n) {
     size_t need = 0;
     for (int m = n; m > kScanTile;) {
         m = (m + kScanTile - 1) / kScanTile;
         need += size_t(m);
     }
-    int* scratch = c.scan_scratch(need + 16);
+    return need;
+}
+
+void scan_excl(tf_ctx& c, int n, const int* in, int* out) {
+    if (n <= 0) return;
+    int* scratch = c.scan_scratch(scan_scratch_ints(n) + 16);
     scan_rec(c, n, in, out, scratch);
+}
+
+// ---- the scan without allocation ----
+// Its block totals live in a fixed device array of kScanNoAllocInts ints
+// (516 KB, held on every GPU the library runs on until the process ends),
+// enough for n <= kScanNoAllocMaxN.  The array's host-side name is not a
+// device address: each device's copy is looked up once, with that device
+// current, and only that address is handed to kernels.  The array keeps the
+// last call's totals; scan_rec writes every total before reading it, so
+// nothing needs clearing.  Calls that overlap on one GPU (two streams) share
+// the array and race.
+constexpr int kScanNoAllocInts = 132096;
+constexpr int kScanNoAllocMaxN = 1 << 28;
+static __device__ __align__(256) int scratch[kScanNoAllocInts];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+int* g_scratch_ptr[kMaxDevices] = {};
+
+int* device_scratch(int device) {
+    require(device >= 0 && device < kMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        int* p = nullptr;
+        check(hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch)), "scan scratch address");
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
+void scan_excl_noalloc(tf_ctx& c, int n, const int* in, int* out) {
+    require(n <= kScanNoAllocMaxN && scan_scratch_ints(n) <= size_t(kScanNoAllocInts),
+            "tf_scan_excl_i32_noalloc: n exceeds 268435456, the fixed scratch's limit");
+    if (n <= 0) return;
+    scan_rec(c, n, in, out, device_scratch(c.device));
 }
 
 template <class T>
