@@ -388,47 +388,64 @@ __global__ __launch_bounds__(kBlock) void k_max_final(int nparts, const int* __r
     }
 }
 
-// Group-per-output-row merge (G lanes, at most G*8 staged entries per row).
-// The group gathers its row's member entries into LDS in encounter order;
-// each lane then takes entries, keeps those that are the first occurrence
-// of their column, merges every later duplicate into it sequentially in
-// encounter order (sum or max: the order of the stable merge) and places
-// the result at its rank among the distinct columns, i.e. sorted by column.
-// Results go to a staging area at gstart[I]; ucnt[I] = distinct columns.
-// Groups never use block barriers: the wave-level fence orders the LDS
-// writes of a group before its reads.
-constexpr int kDedupCap = 512;
+// ---- per-output-row merges (canonical form, contraction, Galerkin) ----
+// Output row I gathers its raw entries -- the rows of its members in member
+// order, each in stored order; graph mode drops intra-aggregate and
+// non-positive entries (mapGraph, GpuAggregation.cu:534-553) -- into LDS as
+// keys (column << 32 | encounter index) and values, sorts the keys with a
+// bitonic network, so that equal columns become adjacent in encounter order,
+// and reduces each run of equal columns sequentially in encounter order (sum,
+// or max for the graph: the order of the stable sort and reduce-by-key of
+// GpuAggregation.cu:178-221, which the host oracle fixes).  A run's rank
+// among the row's runs is its place in the sorted row.  Results go to
+// staging at gstart[I]; ucnt[I] = distinct columns.  Rows of up to 32 raw
+// entries are merged by 32-lane groups; longer rows (up to kMergeCap) are
+// queued for a wave each; longer still raise the overflow flag and the whole
+// product takes the global sort path.
+constexpr int kNoCol = 0x7fffffff;  // dropped entries and padding sort last
+constexpr int kMergeSmall = 32;
+constexpr int kMergeCap = 512;
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int G, bool Max>
-__global__ __launch_bounds__(kBlock) void k_dedup_rows(int nc, const int* __restrict__ offsets,
-                                                       const int* __restrict__ members, const int* __restrict__ map,
-                                                       const int* __restrict__ rp, const int* __restrict__ ci,
-                                                       const double* __restrict__ v, int graph,
-                                                       const int* __restrict__ gstart, int* __restrict__ scol,
-                                                       double* __restrict__ sval, int* __restrict__ ucnt, int* bad) {
-    constexpr int CAP = G * 8;
-    constexpr int NG = kBlock / G;
-    __shared__ int lcol[NG][CAP];
-    __shared__ double lval[NG][CAP];
-    __shared__ unsigned char lfirst[NG][CAP];
-    __shared__ int mstart[NG][G], moff[NG][G];
-    const int g = int(threadIdx.x) / G, lane = int(threadIdx.x) % G;
-    const int I = int(blockIdx.x) * NG + g;
-    const bool active = I < nc;
-    int* C = lcol[g];
-    double* V = lval[g];
-    unsigned char* F = lfirst[g];
+// raw[I] = entries output row I gathers (before filtering); raw[nc] = 0.
+__global__ void k_raw_count(int nc, const int* __restrict__ offsets, const int* __restrict__ members,
+                            const int* __restrict__ rp, int* __restrict__ raw) {
+    const int I = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (I == nc) raw[nc] = 0;
+    if (I >= nc) return;
+    int mb, me;
+    member_range(I, offsets, mb, me);
+    int total = 0;
+    // four member rows per step, loads clamped into range and issued together
+    for (int k = mb; k < me; k += 4) {
+        int rows[4], len[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = min(k + u, me - 1);
+            rows[u] = offsets ? members[q] : q;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) len[u] = rp[rows[u] + 1] - rp[rows[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) total += k + u < me ? len[u] : 0;
+    }
+    raw[I] = total;
+}
+
+// Gathers output row I's raw entries into K / V with the group's G lanes
+// (up to G member rows at a time: lane k looks up member k's row range, a
+// group prefix sum places the rows one after another in member order, then
+// all lanes gather the entries of those rows together); returns the count.
+template <int G>
+__device__ int merge_gather(int I, int mb, int me, const int* __restrict__ offsets, const int* __restrict__ members,
+                            const int* __restrict__ map, const int* __restrict__ rp, const int* __restrict__ ci,
+                            const double* __restrict__ v, int graph, u64* K, double* V, int* mstart, int* moff,
+                            int lane) {
     int m = 0;
-    int mb = 0, me = 0;
-    if (active) member_range(I, offsets, mb, me);
-    // Up to G member rows at a time: lane k looks up member k's row range,
-    // a group prefix sum places the rows one after another (member order),
-    // then all lanes gather the entries of all those rows together.
     for (int k0 = mb; k0 < me; k0 += G) {
         const int k = k0 + lane;
         int s = 0, len = 0;
@@ -444,90 +461,175 @@ __global__ __launch_bounds__(kBlock) void k_dedup_rows(int nc, const int* __rest
             if (lane >= o) incl += t;
         }
         const int total = __shfl(incl, G - 1, G);
-        mstart[g][lane] = s;
-        moff[g][lane] = incl - len;
+        mstart[lane] = s;
+        moff[lane] = incl - len;
         wave_lds_sync();
         const int nm = min(G, me - k0);
         for (int t = lane; t < total; t += G) {
             int q = 0;
             for (int b = 1; b < nm; ++b)
-                if (moff[g][b] <= t) q = b;
-            const int at = mstart[g][q] + (t - moff[g][q]);
+                if (moff[b] <= t) q = b;
+            const int at = mstart[q] + (t - moff[q]);
             const int J = map ? map[ci[at]] : ci[at];
             const double x = v[at];
             const bool keep = !graph || (J != I && x > 0);
-            C[m + t] = keep ? J : -1;
+            K[m + t] = pack_key(keep ? J : kNoCol, m + t);
             V[m + t] = x;
         }
         m += total;
         wave_lds_sync();
     }
-    wave_lds_sync();
-    // The scans below read 4 LDS entries per step (indices clamped, extra
-    // slots masked) so that their loads overlap.
-    for (int e = lane; e < m; e += G) {
-        const int J = C[e];
-        bool first = J >= 0;
-        for (int q0 = 0; q0 < e && first; q0 += 4) {
-            int cq[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) cq[u] = C[min(q0 + u, e - 1)];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (q0 + u < e && cq[u] == J) first = false;
+    return m;
+}
+
+// Ascending bitonic sort of K[0, W) (W a power of two) by the group's lanes.
+template <int G>
+__device__ void merge_sort(u64* K, int W, int lane) {
+    for (int k = 2; k <= W; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < W; i += G) {
+                const int p = i ^ j;
+                if (p > i) {
+                    const u64 a = K[i], b = K[p];
+                    if ((a > b) == ((i & k) == 0)) {
+                        K[i] = b;
+                        K[p] = a;
+                    }
+                }
+            }
+            wave_lds_sync();
         }
-        F[e] = first ? 1 : 0;
+}
+
+// Reduces the runs of equal columns of the sorted K[0, W) (W >= G): lane l
+// owns positions [l P, l P + P), P = W / G, and the runs that start there.
+// Returns the row's distinct columns.
+template <int G, bool Max>
+__device__ int merge_runs(const u64* K, const double* V, int W, int lane, int base, int* __restrict__ scol,
+                          double* __restrict__ sval, int* bad) {
+    const int P = W / G, p0 = lane * P;
+    int heads = 0;
+    for (int q = 0; q < P; ++q) {
+        const int p = p0 + q, J = key_col(K[p]);
+        heads += (J != kNoCol && (p == 0 || key_col(K[p - 1]) != J)) ? 1 : 0;
     }
-    wave_lds_sync();
-    int mine = 0;
-    const int base = active ? gstart[I] : 0;
-    for (int e = lane; e < m; e += G) {
-        if (!F[e]) continue;
-        const int J = C[e];
-        double acc = V[e];
-        int rank = 0;
+    int incl = heads;
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        const int t = __shfl_up(incl, o, G);
+        if (lane >= o) incl += t;
+    }
+    int rank = incl - heads;
+    for (int q = 0; q < P; ++q) {
+        const int p = p0 + q, J = key_col(K[p]);
+        if (J == kNoCol || (p > 0 && key_col(K[p - 1]) == J)) continue;
+        double acc = V[unsigned(K[p] & 0xffffffffu)];
         bool merged = false;
-        // duplicates after e merge into acc in encounter order
-        for (int q0 = e + 1; q0 < m; q0 += 4) {
-            int cq[4];
-            double vq[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int q = min(q0 + u, m - 1);
-                cq[u] = C[q];
-                vq[u] = V[q];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (q0 + u >= m || cq[u] != J) continue;
-                const double x = vq[u];
-                acc = Max ? (acc > x ? acc : x) : acc + x;
-                merged = true;
-            }
+        for (int r = p + 1; r < W; ++r) {
+            const u64 kr = K[r];
+            if (key_col(kr) != J) break;
+            const double x = V[unsigned(kr & 0xffffffffu)];
+            acc = Max ? (acc > x ? acc : x) : acc + x;
+            merged = true;
         }
         if (merged && !isfinite(acc)) atomicOr(bad, 1);
-        for (int q0 = 0; q0 < m; q0 += 4) {
-            int cq[4];
-            unsigned char fq[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int q = min(q0 + u, m - 1);
-                cq[u] = C[q];
-                fq[u] = F[q];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) rank += (q0 + u < m && fq[u] && cq[u] < J) ? 1 : 0;
-        }
         scol[base + rank] = J;
         sval[base + rank] = acc;
-        ++mine;
+        ++rank;
     }
-    mine = group_sum<G>(mine);
-    if (active && lane == 0) {
-        ucnt[I] = mine;
-        if (I == nc - 1) ucnt[nc] = 0;
+    return __shfl(incl, G - 1, G);
+}
+
+// Rows of up to kMergeSmall raw entries, 32 lanes each; longer rows are
+// appended to the worklist wl (count in *wlCount) for k_merge_large.
+template <bool Max>
+__global__ __launch_bounds__(kBlock) void k_merge_small(int nc, const int* __restrict__ offsets,
+                                                        const int* __restrict__ members, const int* __restrict__ map,
+                                                        const int* __restrict__ rp, const int* __restrict__ ci,
+                                                        const double* __restrict__ v, int graph,
+                                                        const int* __restrict__ raw, const int* __restrict__ gstart,
+                                                        int* __restrict__ scol, double* __restrict__ sval,
+                                                        int* __restrict__ ucnt, int* __restrict__ wl, int* wlCount,
+                                                        int* bad) {
+    constexpr int G = kMergeSmall, NG = kBlock / G;
+    __shared__ u64 lk[NG][kMergeSmall];
+    __shared__ double lv[NG][kMergeSmall];
+    __shared__ int mstart[NG][G], moff[NG][G];
+    const int g = int(threadIdx.x) / G, lane = int(threadIdx.x) % G;
+    const int I = int(blockIdx.x) * NG + g;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ucnt[nc] = 0;
+    if (I >= nc) return;
+    // the row's length, member range and staging offset loaded together
+    int mb, me;
+    member_range(I, offsets, mb, me);
+    const int r = raw[I], base = gstart[I];
+    if (r > kMergeSmall) {
+        if (lane == 0) wl[atomicAdd(wlCount, 1)] = I;
+        return;
+    }
+    const int m = merge_gather<G>(I, mb, me, offsets, members, map, rp, ci, v, graph, lk[g], lv[g], mstart[g],
+                                  moff[g], lane);
+    // one key a lane, sorted in registers (compare-exchange with lane ^ j)
+    u64 key = lane < m ? lk[g][lane] : pack_key(kNoCol, lane);
+#pragma unroll
+    for (int k = 2; k <= G; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const u64 other = __shfl_xor(key, j, G);
+            const bool low = (lane & j) == 0, up = (lane & k) == 0;
+            key = (low == up) ? (other < key ? other : key) : (other > key ? other : key);
+        }
+    lk[g][lane] = key;
+    wave_lds_sync();
+    const int u = merge_runs<G, Max>(lk[g], lv[g], kMergeSmall, lane, base, scol, sval, bad);
+    if (lane == 0) ucnt[I] = u;
+}
+
+// The worklist's rows, a wave each, sorted at the next power of two >= 64
+// of their raw length; rows past kMergeCap raise *overflow.
+template <bool Max>
+__global__ __launch_bounds__(kBlock) void k_merge_large(int nc, const int* __restrict__ offsets,
+                                                        const int* __restrict__ members, const int* __restrict__ map,
+                                                        const int* __restrict__ rp, const int* __restrict__ ci,
+                                                        const double* __restrict__ v, int graph,
+                                                        const int* __restrict__ raw, const int* __restrict__ gstart,
+                                                        int* __restrict__ scol, double* __restrict__ sval,
+                                                        int* __restrict__ ucnt, const int* __restrict__ wl,
+                                                        const int* wlCount, int* bad, int* overflow) {
+    constexpr int G = kWave, NG = kBlock / G;
+    __shared__ u64 lk[NG][kMergeCap];
+    __shared__ double lv[NG][kMergeCap];
+    __shared__ int mstart[NG][G], moff[NG][G];
+    const int g = int(threadIdx.x) / G, lane = int(threadIdx.x) % G;
+    const int count = wl ? *wlCount : nc;
+    if (!wl && blockIdx.x == 0 && threadIdx.x == 0) ucnt[nc] = 0;
+    for (int q = int(blockIdx.x) * NG + g; q < count; q += int(gridDim.x) * NG) {
+        const int I = wl ? wl[q] : q;
+        const int r = raw[I];
+        if (r > kMergeCap) {
+            if (lane == 0) {
+                atomicOr(overflow, 1);
+                ucnt[I] = 0;
+            }
+            continue;
+        }
+        int W = G;
+        while (W < r) W <<= 1;
+        int mb, me;
+        member_range(I, offsets, mb, me);
+        const int m = merge_gather<G>(I, mb, me, offsets, members, map, rp, ci, v, graph, lk[g], lv[g], mstart[g],
+                                      moff[g], lane);
+        for (int e = m + lane; e < W; e += G) lk[g][e] = pack_key(kNoCol, e);
+        wave_lds_sync();
+        merge_sort<G>(lk[g], W, lane);
+        const int u = merge_runs<G, Max>(lk[g], lv[g], W, lane, gstart[I], scol, sval, bad);
+        if (lane == 0) ucnt[I] = u;
+        wave_lds_sync();
     }
 }
+
+// dst[0] = src[0] (a device scalar into a flag slot, without a copy command).
+__global__ void k_copy_int(const int* __restrict__ src, int* __restrict__ dst) { *dst = *src; }
 
 // Staging -> CSR, 16 lanes per row (coalesced row copies).
 __global__ __launch_bounds__(kBlock) void k_dedup_copy(int nc, const int* __restrict__ gstart,
@@ -985,65 +1087,59 @@ DevCsr merge_rows(tf_ctx& c, int rows, const int* start, const u64* keys, const 
 }
 
 // Gather member rows into per-output-row segments and merge duplicate
-// columns: wave-per-row LDS merge when every row fits kDedupCap entries,
-// otherwise staged keys + per-segment sort + sequential merge.
+// columns: LDS merges (k_merge_small / k_merge_large) when every row fits
+// kMergeCap raw entries, otherwise staged keys + per-segment sort +
+// sequential merge.  nnzIn: entries of the rows gathered (bounds the
+// staging; < 0 reads it back).
 DevCsr gather_merge(tf_ctx& c, int nc, const int* offsets, const int* members, const int* map, const int* rp,
-                    const int* ci, const double* v, bool graph, bool maximum) {
-    int* cnt = c.wst<int>(kCount, size_t(nc) + 1);
-    int* rawLen = c.wst<int>(kRawLen, size_t(nc) + 1);
-    hipLaunchKernelGGL(k_gather_count, grid1(int64_t(nc) + 1), dim3(kBlock), 0, c.stream, nc, offsets, members, map,
-                       rp, ci, v, int(graph), cnt, rawLen);
-    check(hipGetLastError(), "gather count launch");
-    {
-        const int parts = int(std::min<int64_t>((int64_t(nc) + kBlock - 1) / kBlock, 1024));
-        int* part = c.wst<int>(kMaxPart, size_t(parts) + 1);
-        hipLaunchKernelGGL(k_max_partials, dim3(parts), dim3(kBlock), 0, c.stream, nc, rawLen, part);
-        hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kBlock), 0, c.stream, parts, part, c.dflags);
-        check(hipGetLastError(), "max launch");
-    }
+                    const int* ci, const double* v, bool graph, bool maximum, int64_t nnzIn) {
+    int* raw = c.wst<int>(kRawLen, size_t(nc) + 1);
+    hipLaunchKernelGGL(k_raw_count, grid1(int64_t(nc) + 1), dim3(kBlock), 0, c.stream, nc, offsets, members, rp, raw);
+    check(hipGetLastError(), "raw count launch");
     int* start = c.wst<int>(kStart, size_t(nc) + 1);
-    scan_excl(c, nc + 1, cnt, start);
-    check(hipMemcpyAsync(c.dflags + 1, start + nc, sizeof(int), hipMemcpyDeviceToDevice, c.stream), "total copy");
-    c.read_flags(2);
-    const int maxRow = c.hflags[0], total = c.hflags[1];
-    if (maxRow <= kDedupCap) {
-        int* scol = c.wst<int>(kKeys, size_t(total) + 1);
-        double* sval = c.wst<double>(kVals, size_t(total) + 1);
-        int* ucnt = c.wst<int>(kUnique, size_t(nc) + 1);
-        c.clear_flags(1);
-#define TF_DEDUP(GG, MX)                                                                                       \
-    hipLaunchKernelGGL((k_dedup_rows<GG, MX>), dim3((nc + kBlock / GG - 1) / (kBlock / GG)), dim3(kBlock), 0,    \
-                       c.stream, nc, offsets, members, map, rp, ci, v, int(graph), start, scol, sval, ucnt,      \
-                       c.dflags)
-        // 16 lanes per output row while rows are short; a wave per row once
-        // the average row gathers more than 40 entries (the quadratic merge
-        // then dominates and needs the lanes more than the rows do)
-        if (maxRow <= 16 * 8 && total <= 40 * int64_t(nc)) {
-            if (maximum)
-                TF_DEDUP(16, true);
-            else
-                TF_DEDUP(16, false);
+    scan_excl(c, nc + 1, raw, start);
+    if (nnzIn < 0) nnzIn = c.read_int(start + nc);
+    int* scol = c.wst<int>(kKeys, size_t(nnzIn) + 1);
+    double* sval = c.wst<double>(kVals, size_t(nnzIn) + 1);
+    int* ucnt = c.wst<int>(kUnique, size_t(nc) + 1);
+    int* wl = c.wst<int>(kLarge, size_t(nc) + 1);
+    // flags: 0 non-finite merge, 1 worklist length, 2 overflow, 3 output nnz
+    c.clear_flags(3);
+    {
+        // Rows short on average (<= 28 entries): 32-lane groups, the few long
+        // rows queued for waves; otherwise a wave per row throughout.
+        const bool small = nnzIn <= int64_t(28) * nc;
+        const int ng = kBlock / kMergeSmall;
+        const dim3 grid((nc + ng - 1) / ng), large(std::max(1, std::min((nc + 3) / 4, 4096)));
+        int* list = small ? wl : nullptr;
+#define TF_MERGE(MX)                                                                                           \
+    if (small) {                                                                                               \
+        hipLaunchKernelGGL(k_merge_small<MX>, grid, dim3(kBlock), 0, c.stream, nc, offsets, members, map, rp, ci, \
+                           v, int(graph), raw, start, scol, sval, ucnt, wl, c.dflags + 1, c.dflags);             \
+    }                                                                                                          \
+    hipLaunchKernelGGL(k_merge_large<MX>, large, dim3(kBlock), 0, c.stream, nc, offsets, members, map, rp, ci, v, \
+                       int(graph), raw, start, scol, sval, ucnt, list, c.dflags + 1, c.dflags, c.dflags + 2)
+        if (maximum) {
+            TF_MERGE(true);
         }
         else {
-            if (maximum)
-                TF_DEDUP(64, true);
-            else
-                TF_DEDUP(64, false);
+            TF_MERGE(false);
         }
-#undef TF_DEDUP
-        check(hipGetLastError(), "dedup launch");
-        DevCsr out;
-        out.n = nc;
-        out.rp = dev_alloc<int>(c, size_t(nc) + 1);
-        scan_excl(c, nc + 1, ucnt, out.rp);
-        check(hipMemcpyAsync(c.dflags + 1, out.rp + nc, sizeof(int), hipMemcpyDeviceToDevice, c.stream),
-              "nnz copy");
-        c.read_flags(2);
+#undef TF_MERGE
+        check(hipGetLastError(), "merge launch");
+    }
+    DevCsr out;
+    out.n = nc;
+    out.rp = dev_alloc<int>(c, size_t(nc) + 1);
+    scan_excl(c, nc + 1, ucnt, out.rp);
+    hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, c.stream, out.rp + nc, c.dflags + 3);
+    c.read_flags(4);
+    if (!c.hflags[2]) {
         if (c.hflags[0]) {
             dev_free(c, out.rp);
             throw Error(TF_E_NUMERIC, "Nonfinite aggregate reduction");
         }
-        out.nnz = c.hflags[1];
+        out.nnz = c.hflags[3];
         out.ci = dev_alloc<int>(c, size_t(out.nnz));
         out.v = dev_alloc<double>(c, size_t(out.nnz));
         hipLaunchKernelGGL(k_dedup_copy, dim3((nc + kBlock / 16 - 1) / (kBlock / 16)), dim3(kBlock), 0, c.stream, nc,
@@ -1051,6 +1147,14 @@ DevCsr gather_merge(tf_ctx& c, int nc, const int* offsets, const int* members, c
         check(hipGetLastError(), "dedup copy launch");
         return out;
     }
+    // a row longer than kMergeCap: every row through the global sort path
+    dev_free(c, out.rp);
+    int* cnt = c.wst<int>(kCount, size_t(nc) + 1);
+    hipLaunchKernelGGL(k_gather_count, grid1(int64_t(nc) + 1), dim3(kBlock), 0, c.stream, nc, offsets, members, map,
+                       rp, ci, v, int(graph), cnt, static_cast<int*>(nullptr));
+    check(hipGetLastError(), "gather count launch");
+    scan_excl(c, nc + 1, cnt, start);
+    const int total = c.read_int(start + nc);
     u64* keys = c.wst<u64>(kKeys, size_t(total));
     double* vals = c.wst<double>(kVals, size_t(total));
     hipLaunchKernelGGL(k_gather_fill, grid1(nc), dim3(kBlock), 0, c.stream, nc, offsets, members, map, rp, ci, v,
@@ -1129,12 +1233,12 @@ DevCsr canonicalize(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci,
     const int state = csr_check(c, n, nnz, rp, ci, v);
     if (state & 2) throw Error(TF_E_INVALID, "AMG invalid CSR or nonfinite coefficient");
     if (state == 0) return copy_csr(c, n, nnz, rp, ci, v);
-    return gather_merge(c, n, nullptr, nullptr, nullptr, rp, ci, v, false, false);
+    return gather_merge(c, n, nullptr, nullptr, nullptr, rp, ci, v, false, false, nnz);
 }
 
 DevCsr merge_csr_rows(tf_ctx& c, int rows, int64_t nnz, const int* rp, const int* ci, const double* v) {
     require(rows >= 1 && nnz >= 1 && nnz <= 0x3fffffff, "row merge needs rows >= 1 and 1..2^30 entries");
-    return gather_merge(c, rows, nullptr, nullptr, nullptr, rp, ci, v, false, false);
+    return gather_merge(c, rows, nullptr, nullptr, nullptr, rp, ci, v, false, false, nnz);
 }
 
 void validate_rowmax_async(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax,
@@ -1260,7 +1364,8 @@ int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int ro
     return c.read_int(prefix + n);
 }
 
-DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc) {
+DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc,
+                      int64_t nnz) {
     int *sizes, *slots;
     build_members(c, n, nc, map, &sizes, &slots);
     int* moff = c.wst<int>(kMemOffsets, size_t(nc) + 1);
@@ -1269,7 +1374,7 @@ DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const doub
     hipLaunchKernelGGL(k_pack_members, grid1(nc), dim3(kBlock), 0, c.stream, nc, sizes, slots, moff, mem,
                        static_cast<int*>(nullptr));
     check(hipGetLastError(), "pack members launch");
-    return gather_merge(c, nc, moff, mem, map, rp, ci, w, true, true);
+    return gather_merge(c, nc, moff, mem, map, rp, ci, w, true, true, nnz);
 }
 
 DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb, int* hostHist) {
@@ -1280,7 +1385,7 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb, int* hostH
     t.map = dev_alloc<int>(c, size_t(n));
     try {
         const int first = pair(c, n, g.rp, g.ci, g.v, rounds, t.map, g.nnz);
-        DevCsr contracted = contract_graph(c, n, g.rp, g.ci, g.v, t.map, first);
+        DevCsr contracted = contract_graph(c, n, g.rp, g.ci, g.v, t.map, first, g.nnz);
         int* second = c.wst<int>(kSecond, size_t(first));
         int coarse = 0;
         try {
@@ -1366,7 +1471,7 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb, int* hostH
 
 DevCsr galerkin(tf_ctx& c, const DevCsr& a, const int* map, int nc, const int* offsets, const int* members) {
     require(nc >= 1, "AMG transfer dimension mismatch");
-    return gather_merge(c, nc, offsets, members, map, a.rp, a.ci, a.v, false, false);
+    return gather_merge(c, nc, offsets, members, map, a.rp, a.ci, a.v, false, false, a.nnz);
 }
 
 }  // namespace tf

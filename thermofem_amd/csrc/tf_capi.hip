@@ -574,6 +574,7 @@ int tf_galerkin(tf_ctx* ctx, int n, const int* rp, const int* ci, const double* 
         a.rp = const_cast<int*>(rp);
         a.ci = const_cast<int*>(ci);
         a.v = const_cast<double*>(v);
+        a.nnz = -1;  // the members' rows are not known to partition A: staging sized by a read-back
         DevCsr out = galerkin(*ctx, a, map, nc, offsets, members);
         *c_rp = out.rp;
         *c_ci = out.ci;

@@ -364,7 +364,8 @@ double* rowmax_workspace(tf_ctx& c, int n);
 void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal);
 // nnz < 0: read from rp[n].
 int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map, int64_t nnz = -1);
-DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc);
+DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc,
+                      int64_t nnz = -1);
 // With hostHist (pinned, 4 ints) the aggregate-size histogram is copied
 // there asynchronously and DevAgg::hist is left zero: it is valid once the
 // stream has drained.
