@@ -53,7 +53,7 @@ struct TuneKey {
         return std::tie(op, n, nnz, stored, bytes) < std::tie(o.op, o.n, o.nnz, o.stored, o.bytes);
     }
 };
-std::map<TuneKey, bool> g_tuned;
+std::map<TuneKey, int> g_tuned;  // 0: CSR kernels; 1, 2, 4: sliced ELL with that many lanes per row
 std::map<TuneKey, int> g_tail_start;  // chosen first level of the coarse tail (-1: none)
 std::mutex g_tune_mutex;
 
@@ -112,16 +112,20 @@ float best_time(tf_ctx& c, F&& f, int samples = 3) {
     return best;
 }
 
-// Keeps the sliced-ELL copy of a level only where it beats the CSR sweep.
+// Keeps the sliced-ELL copy of a level only where it beats the CSR sweep,
+// with the lanes per row (1, 2 or 4) that sweep it fastest; the three give
+// identical results.
 template <class L>
-bool sell_pays(tf_ctx& c, const AmgLevel& l, double w) {
+bool sell_pays(tf_ctx& c, AmgLevel& l, double w) {
     const DevCsr& A = l.native;
-    if (A.nnz < kTuneMinNnz) return true;
     const TuneKey key{0, A.n, A.nnz, l.sell.stored, int(sizeof(L))};
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         auto it = g_tuned.find(key);
-        if (it != g_tuned.end()) return it->second;
+        if (it != g_tuned.end()) {
+            l.sell.lanes = std::max(1, it->second);
+            return it->second != 0;
+        }
     }
     fill_zero(c, l.b, size_t(A.n) * sizeof(L));
     fill_zero(c, l.x, size_t(A.n) * sizeof(L));
@@ -131,11 +135,25 @@ bool sell_pays(tf_ctx& c, const AmgLevel& l, double w) {
     const L* b = static_cast<const L*>(l.b);
     const L* x = static_cast<const L*>(l.x);
     L* x2 = static_cast<L*>(l.x2);
-    const bool sell = first_is_faster(
-        c, [&] { sell_jacobi<L>(c, l.sell, svals, inv, L(w), b, x, x2); },
-        [&] { jacobi_fused<L>(c, A.n, A.nnz, A.rp, A.ci, vals, inv, L(w), b, x, x2); }, kSellPreference);
+    auto sweep = [&] { sell_jacobi<L>(c, l.sell, svals, inv, L(w), b, x, x2); };
+    int lanes = 1;
+    float best = 1e30f;
+    for (int r : {1, 2, 4}) {
+        l.sell.lanes = r;
+        const float t = best_time(c, [&] {
+            for (int q = 0; q < 4; ++q) sweep();
+        });
+        if (t < best) {
+            best = t;
+            lanes = r;
+        }
+    }
+    l.sell.lanes = lanes;
+    const bool sell = A.nnz < kTuneMinNnz ||
+                      first_is_faster(c, sweep, [&] { jacobi_fused<L>(c, A.n, A.nnz, A.rp, A.ci, vals, inv, L(w), b, x, x2); },
+                                      kSellPreference);
     std::lock_guard<std::mutex> lock(g_tune_mutex);
-    g_tuned[key] = sell;
+    g_tuned[key] = sell ? lanes : 0;
     return sell;
 }
 
@@ -143,10 +161,10 @@ bool sell_pays(tf_ctx& c, const AmgLevel& l, double w) {
 // Loop over LDS-staged row blocks for the fused reduction kernels (XCD
 // contiguous; with a grid below the block count each XCD walks its slab).
 // f(row, sum) runs in the thread owning the row.
-template <class F>
+template <class XF, class F>
 __device__ __forceinline__ void for_stream_blocks(int n, int nblocks, int rpb, const int* __restrict__ rp,
                                                   const int* __restrict__ ci, const double* __restrict__ v,
-                                                  const double* __restrict__ x, F&& f) {
+                                                  XF&& xval, F&& f) {
     __shared__ double prod[StreamCap<double>::value];
     __shared__ int srp[kMaxStreamRows + 1];
     const int per = int(gridDim.x) / kXcds;
@@ -157,7 +175,7 @@ __device__ __forceinline__ void for_stream_blocks(int n, int nblocks, int rpb, c
         const int row0 = lb * rpb;
         const int rows = min(rpb, n - row0);
         double acc[kMaxStreamRows / kBlock];
-        csr_stream_rows<double>(row0, rows, rp, ci, v, [&](int col) { return x[col]; }, prod, srp, acc);
+        csr_stream_rows<double>(row0, rows, rp, ci, v, xval, prod, srp, acc);
 #pragma unroll
         for (int k = 0; k < kMaxStreamRows / kBlock; ++k) {
             const int r = int(threadIdx.x) + k * kBlock;
@@ -168,9 +186,8 @@ __device__ __forceinline__ void for_stream_blocks(int n, int nblocks, int rpb, c
 
 // The same over a sliced-ELL operator: one lane per row, slices of 64 rows,
 // 4 slices per block, blocks XCD-contiguous.
-template <class C, class F>
-__device__ __forceinline__ void for_sell_rows(const SellView<C>& m, const double* __restrict__ v,
-                                              const double* __restrict__ x, F&& f) {
+template <class C, class XF, class F>
+__device__ __forceinline__ void for_sell_rows(const SellView<C>& m, const double* __restrict__ v, XF&& xval, F&& f) {
     constexpr int spb = kBlock / kWave;
     const int nblocks = (m.slices + spb - 1) / spb;
     const int per = int(gridDim.x) / kXcds;
@@ -184,7 +201,7 @@ __device__ __forceinline__ void for_sell_rows(const SellView<C>& m, const double
         if (pos < m.n) {
             const int off = m.base[slice] + lane;
             const int row = m.perm ? m.perm[pos] : pos;
-            f(row, sell_row_sum<double>(row, int(m.len[pos]), v + off, m.col + off, [&](int c) { return x[c]; }));
+            f(row, sell_row_sum<double>(row, int(m.len[pos]), v + off, m.col + off, xval));
         }
     }
 }
@@ -197,7 +214,12 @@ struct CsrRows {
     const double* v;
     template <class F>
     __device__ __forceinline__ void each(const double* x, F&& f) const {
-        for_stream_blocks(n, nblocks, rpb, rp, ci, v, x, f);
+        for_stream_blocks(n, nblocks, rpb, rp, ci, v, [&](int c) { return x[c]; }, f);
+    }
+    // the same with x[c] given by xval(c)
+    template <class XF, class F>
+    __device__ __forceinline__ void each_fn(XF&& xval, F&& f) const {
+        for_stream_blocks(n, nblocks, rpb, rp, ci, v, xval, f);
     }
 };
 template <class C>
@@ -206,7 +228,11 @@ struct SellRows {
     const double* v;
     template <class F>
     __device__ __forceinline__ void each(const double* x, F&& f) const {
-        for_sell_rows(m, v, x, f);
+        for_sell_rows(m, v, [&](int c) { return x[c]; }, f);
+    }
+    template <class XF, class F>
+    __device__ __forceinline__ void each_fn(XF&& xval, F&& f) const {
+        for_sell_rows(m, v, xval, f);
     }
 };
 
@@ -326,34 +352,6 @@ __global__ __launch_bounds__(kBlock) void k_out_dot(int n, const L* __restrict__
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-// CG direction: p = z (fresh) or p = z + beta p.
-__global__ void k_cg_p(int n, const double* __restrict__ z, double* __restrict__ p, const KCtl* ctl) {
-    if (stopped(ctl)) return;
-    const int fresh = ctl->fresh;
-    const double beta = ctl->beta;
-#pragma unroll 4
-    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock)
-        p[i] = fresh ? z[i] : z[i] + beta * p[i];
-}
-
-// x += alpha p; r -= alpha q; sum(r.^2).
-__global__ __launch_bounds__(kBlock) void k_cg_update(int n, const double* __restrict__ p,
-                                                      const double* __restrict__ q, double* __restrict__ x,
-                                                      double* __restrict__ r, double* partials,
-                                                      const KCtl* ctl) {
-    if (stopped(ctl)) return;
-    const double alpha = ctl->alpha;
-    double s = 0;
-#pragma unroll 4
-    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
-        x[i] = alpha * p[i] + x[i];
-        const double ri = -alpha * q[i] + r[i];
-        r[i] = ri;
-        s += ri * ri;
-    }
-    block_partial(s, partials);
-}
-
 // BiCGStab direction: p = r (fresh) or p = r + beta (p - omega v).
 __global__ void k_bi_p(int n, const double* __restrict__ r, const double* __restrict__ v, double* __restrict__ p,
                        const KCtl* ctl) {
@@ -401,6 +399,120 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(int n, const double* __res
     block_partial(acc, partials);
 }
 
+// ---- CG iteration with the scalar recurrences folded into the consumers ----
+// Each kernel below starts by adding its producer's block partials (every
+// block in the same order, so every block reaches the same value), applies
+// the recurrence step to a copy of the control block, and lets block 0 write
+// the fields that step sets.  The fields a kernel reads are never ones it
+// writes, except the status, whose new value every block derives itself,
+// so blocks that start after block 0's write behave identically.  Between
+// two iterations the control block holds the same values as after separate
+// finalisers; "fresh = 0", the curvature step's, is written by the residual
+// step that follows, where no block still reads it.
+
+// Every thread's share of the producer's partials -> thread 0 (block order).
+__device__ __forceinline__ double fold_partials(const double* __restrict__ parts, int n) {
+    return sum_partials(parts, n);
+}
+
+// rho = r.z (finaliser of the preconditioner's output step), then the CG
+// direction p = z (fresh) or z + beta p.
+__global__ __launch_bounds__(kBlock) void k_cg_dir(int n, const double* __restrict__ z, double* __restrict__ p,
+                                                   const double* __restrict__ partsIn, int nIn, KCtl* ctl) {
+    __shared__ double sBeta;
+    __shared__ int sRun, sFresh;
+    if (stopped(ctl)) return;
+    const double rho = fold_partials(partsIn, nIn);
+    if (threadIdx.x == 0) {
+        KCtl c = *ctl;
+        finalize(kFinRhoCg, rho, &c);
+        if (blockIdx.x == 0) {
+            ctl->iter = c.iter;
+            ctl->rho = c.rho;
+            ctl->beta = c.beta;
+            ctl->status = c.status;
+        }
+        sBeta = c.beta;
+        sFresh = c.fresh;
+        sRun = c.status == kRunning;
+    }
+    __syncthreads();
+    if (!sRun) return;
+    const double beta = sBeta;
+    const bool fresh = sFresh != 0;
+#pragma unroll 4
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock)
+        p[i] = fresh ? z[i] : z[i] + beta * p[i];
+}
+
+// alpha = rho / p.q (finaliser of the product A p), then x += alpha p,
+// r -= alpha q, and |r|^2 per block.
+__global__ __launch_bounds__(kBlock) void k_cg_step(int n, const double* __restrict__ p, const double* __restrict__ q,
+                                                    double* __restrict__ x, double* __restrict__ r,
+                                                    const double* __restrict__ partsIn, int nIn,
+                                                    double* __restrict__ partsOut, KCtl* ctl) {
+    __shared__ double sAlpha;
+    __shared__ int sRun;
+    if (stopped(ctl)) return;
+    const double curv = fold_partials(partsIn, nIn);
+    if (threadIdx.x == 0) {
+        KCtl c = *ctl;
+        finalize(kFinCurvCg, curv, &c);
+        if (blockIdx.x == 0) {
+            ctl->alpha = c.alpha;
+            ctl->status = c.status;
+        }
+        sAlpha = c.alpha;
+        sRun = c.status == kRunning;
+    }
+    __syncthreads();
+    if (!sRun) return;
+    const double alpha = sAlpha;
+    double s = 0;
+#pragma unroll 4
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
+        x[i] = alpha * p[i] + x[i];
+        const double ri = -alpha * q[i] + r[i];
+        r[i] = ri;
+        s += ri * ri;
+    }
+    block_partial(s, partsOut);
+}
+
+// |r| and the refresh test (Cg.cuh:173-187), then, where a refresh is due,
+// r = b - A x and |r|^2 per block for the true-residual finaliser.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_cg_check(R rows, const double* __restrict__ b,
+                                                     const double* __restrict__ x, double* __restrict__ r,
+                                                     const double* __restrict__ partsIn, int nIn,
+                                                     double* __restrict__ partsOut, KCtl* ctl) {
+    __shared__ int sTrue;
+    if (stopped(ctl)) return;
+    const double rr = fold_partials(partsIn, nIn);
+    if (threadIdx.x == 0) {
+        KCtl c = *ctl;
+        finalize(kFinUpdCg, rr, &c);
+        c.fresh = 0;
+        if (blockIdx.x == 0) {
+            ctl->res = c.res;
+            ctl->check = c.check;
+            ctl->rho_prev = c.rho_prev;
+            ctl->status = c.status;
+            ctl->fresh = c.fresh;
+        }
+        sTrue = c.status == kRunning && c.check && !c.early;
+    }
+    __syncthreads();
+    if (!sTrue) return;
+    double contrib = 0;
+    rows.each(x, [&](int row, double acc) {
+        const double res = b[row] - acc;
+        r[row] = res;
+        contrib += res * res;
+    });
+    block_partial(contrib, partsOut);
+}
+
 // Element-wise Krylov kernels: at most `cap` blocks, each thread walking its
 // elements with the loop unrolled by 4 so that several elements' loads are
 // in flight.  2048 blocks (8 per CU, all resident at once) suit the kernels
@@ -408,6 +520,9 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(int n, const double* __res
 // streams) ran faster with one element a thread (at 128^3: 20.3 us at 8192
 // blocks against 22.8 us at 2048 for the CG update).
 constexpr int kVecBlocks = 2048, kUpdateBlocks = 8192;
+// Most partials a reduction whose recurrence step is folded into its
+// consumer may leave: every consumer block adds them all.
+constexpr int kFoldParts = 2048;
 int vec_blocks(int n, int cap = kVecBlocks) {
     int b = (n + kBlock - 1) / kBlock;
     if (b > cap) b = cap;
@@ -1354,7 +1469,7 @@ bool SolverEngine::sell_spmv_pays() {
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         auto it = g_tuned.find(key);
-        if (it != g_tuned.end()) return it->second;
+        if (it != g_tuned.end()) return it->second != 0;
     }
     fill_zero(*c_, p_, size_t(n_) * sizeof(double));
     const bool sell = first_is_faster(
@@ -1409,20 +1524,24 @@ void SolverEngine::launch_iteration(const R& rows, int nb) {
         hipLaunchKernelGGL(k_kfinal, dim3(1), dim3(1024), 0, c_->stream, pslot(slot), parts, ctl, mode, onEarly);
     };
     if (kind_ == 0) {
-        // z = M r with rho = r.z formed in the same pass
+        // z = M r with rho = r.z formed in the same pass; each reduction's
+        // recurrence step runs at the start of the kernel that consumes it
+        // (producers keep to at most kFoldParts partials)
+        const dim3 gf(std::min(int(grid.x), kFoldParts));
+        const int nf = int(gf.x);
         if (amg_)
             amg_->apply_dot(r_, z_, r_, pslot(kSlotA), nv, ctl);
         else
             hipLaunchKernelGGL(k_out_dot<double>, vb, dim3(kBlock), 0, c_->stream, n_, r_, 0, r_, z_,
                                pslot(kSlotA), ctl);
-        fin(kSlotA, nv, kFinRhoCg, 0);
-        hipLaunchKernelGGL(k_cg_p, vb, dim3(kBlock), 0, c_->stream, n_, z_, p_, ctl);
-        TF_G(k_kspmv_dot<R>, rows, p_, q_, p_, pslot(kSlotB), ctl);
-        fin(kSlotB, ng, kFinCurvCg, 0);
-        hipLaunchKernelGGL(k_cg_update, ub, dim3(kBlock), 0, c_->stream, n_, p_, q_, x_, r_, pslot(kSlotC), ctl);
-        fin(kSlotC, nu, kFinUpdCg, 0);
-        TF_G(k_ktrue<R>, rows, rhs_, x_, r_, static_cast<double*>(nullptr), pslot(kSlotD), ctl, 0);
-        fin(kSlotD, ng, kFinTrue, 0);
+        hipLaunchKernelGGL(k_cg_dir, vb, dim3(kBlock), 0, c_->stream, n_, z_, p_, pslot(kSlotA), nv, ctl);
+        hipLaunchKernelGGL(k_kspmv_dot<R>, gf, dim3(kBlock), 0, c_->stream, rows, p_, q_, p_, pslot(kSlotB), ctl);
+        hipLaunchKernelGGL(k_cg_step, vb, dim3(kBlock), 0, c_->stream, n_, p_, q_, x_, r_, pslot(kSlotB), nf,
+                           pslot(kSlotC), ctl);
+        hipLaunchKernelGGL(k_cg_check<R>, gf, dim3(kBlock), 0, c_->stream, rows, rhs_, x_, r_, pslot(kSlotC), nv,
+                           pslot(kSlotD), ctl);
+        fin(kSlotD, nf, kFinTrue, 0);
+        (void)nu;
     }
     else {
         double* v = q_;

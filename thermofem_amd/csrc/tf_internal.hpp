@@ -304,6 +304,7 @@ struct DevSell {
     unsigned char* len = nullptr;
     void* col = nullptr;
     int* perm = nullptr;       // row at each position, nullptr when rows keep their order
+    int lanes = 1;             // lanes per row of the product kernels (1, 2 or 4; same results)
     template <class C>
     SellView<C> view() const {
         return SellView<C>{n, slices, base, len, static_cast<const C*>(col), perm};

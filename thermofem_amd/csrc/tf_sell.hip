@@ -124,6 +124,76 @@ __global__ __launch_bounds__(kBlock) void k_sell(SellView<C> m, const T* __restr
     epilogue<T, E>(row, acc, x, y, b, inv, w);
 }
 
+// The same row products with R lanes per row: a wave takes 64 / R rows of a
+// slice and lane segment q of a row the slots [q K', (q + 1) K') of the
+// slice's width K (K' = ceil(K / R)), so a slice is R waves and a lane loads
+// (and gathers) a fraction of the row at once.  The row sum stays the
+// sequential one: segment 0 adds its rounded products from 0, each later
+// segment continues from the previous segment's partial (handed over by a
+// shuffle) in slot order, and the last segment applies the epilogue -- the
+// same operations in the same order as k_sell, so results are identical.
+// Levels whose rows are long relative to their count (few slices, long
+// per-lane chains) run faster this way; the tuner picks R per level.
+constexpr int kSplitRegs = 16;  // products held per lane before the hand-over
+template <class T, class C, int E, int R>
+__global__ __launch_bounds__(kBlock) void k_sell_split(SellView<C> m, const T* __restrict__ v,
+                                                       const T* __restrict__ x, T* __restrict__ y,
+                                                       const T* __restrict__ b, const T* __restrict__ inv, T w) {
+#pragma clang fp contract(off)
+    constexpr int W = kWave / R;
+    const int units = m.slices * R;
+    const int blk = xcd_block((units + kSlicesPerBlock - 1) / kSlicesPerBlock);
+    if (blk < 0) return;
+    const int unit = blk * kSlicesPerBlock + int(threadIdx.x) / kWave;
+    if (unit >= units) return;  // (uniform over the wave)
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    const int slice = unit / R, q = lane / W;
+    const int rl = (unit % R) * W + lane % W;  // the row's lane in its slice
+    const int pos = slice * kSliceRows + rl;
+    const bool valid = pos < m.n;
+    const int len = valid ? int(m.len[pos]) : 0;
+    const int base = m.base[slice];
+    const int kseg = ((m.base[slice + 1] - base) / kSliceRows + R - 1) / R;
+    const int k0 = q * kseg, k1 = min(len, k0 + kseg);
+    const int cnt = k1 > k0 ? k1 - k0 : 0;
+    const int row = valid ? (m.perm ? m.perm[pos] : pos) : 0;
+    const T* vp = v + base + rl;
+    const C* cp = m.col + base + rl;
+    T prod[kSplitRegs];
+    if (cnt > 0) {
+        T pv[kSplitRegs], xv[kSplitRegs];
+        int pc[kSplitRegs];
+#pragma unroll
+        for (int u = 0; u < kSplitRegs; ++u) {
+            const int k = min(k0 + u, k1 - 1);
+            pv[u] = ld_entry(vp + k * kWave);
+            pc[u] = sell_col(ld_entry(cp + k * kWave), row);
+        }
+#pragma unroll
+        for (int u = 0; u < kSplitRegs; ++u) xv[u] = x[pc[u]];
+#pragma unroll
+        for (int u = 0; u < kSplitRegs; ++u) prod[u] = pv[u] * xv[u];
+    }
+    T acc = 0;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const T in = __shfl_up(acc, W);
+        if (q == t) {
+            acc = t == 0 ? T(0) : in;
+#pragma unroll
+            for (int u = 0; u < kSplitRegs; ++u) {
+                const T sum = acc + prod[u];
+                acc = u < cnt ? sum : acc;
+            }
+            for (int k = k0 + kSplitRegs; k < k1; ++k) {  // rows wider than R * kSplitRegs
+                const T p = ld_entry(vp + k * kWave) * x[sell_col(ld_entry(cp + k * kWave), row)];
+                acc = acc + p;
+            }
+        }
+    }
+    if (q == R - 1 && valid) epilogue<T, E>(row, acc, x, y, b, inv, w);
+}
+
 }  // namespace
 
 namespace {
@@ -222,6 +292,26 @@ void free_sell(tf_ctx& c, DevSell& s) {
 template <class T, int E>
 void sell_rows(tf_ctx& c, const DevSell& s, const T* v, const T* x, T* y, const T* b, const T* inv, T w) {
     if (s.n <= 0) return;
+    if (s.lanes == 2 || s.lanes == 4) {
+        const dim3 grid(round_xcd((s.slices * s.lanes + kSlicesPerBlock - 1) / kSlicesPerBlock));
+#define TF_SPLIT(C, R)                                                                                         \
+    hipLaunchKernelGGL((k_sell_split<T, C, E, R>), grid, dim3(kBlock), 0, c.stream, s.view<C>(), v, x, y, b, inv, w)
+        if (s.narrow) {
+            if (s.lanes == 2)
+                TF_SPLIT(short, 2);
+            else
+                TF_SPLIT(short, 4);
+        }
+        else {
+            if (s.lanes == 2)
+                TF_SPLIT(int, 2);
+            else
+                TF_SPLIT(int, 4);
+        }
+#undef TF_SPLIT
+        check(hipGetLastError(), "sell kernel launch");
+        return;
+    }
     const dim3 grid(round_xcd((s.slices + kSlicesPerBlock - 1) / kSlicesPerBlock));
     if (s.narrow)
         hipLaunchKernelGGL((k_sell<T, short, E>), grid, dim3(kBlock), 0, c.stream, s.view<short>(), v, x, y, b, inv,
