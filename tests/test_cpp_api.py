@@ -1,9 +1,10 @@
 """The C++20 drop-in API (thermofem_amd/include/thermofem/*.hpp) exercised by
-tests/cpp/api_test.cpp, which restates the reference's GoogleTest cases
-(solver_test.cpp, cg_test.cpp, multigrid_operations_test.cpp,
-amg_native_test.cpp, data_structure_test.cpp, matrix_market_test.cpp) over
-the HIP library.  The option, CSR-validation and Matrix Market cases need no
-device and run on the CPU; the full program runs on the GPU."""
+tests/cpp/api_test.cpp: suites of checks whose inputs and expected values are
+those of the reference's tests (solver_test.cpp, cg_test.cpp,
+multigrid_operations_test.cpp, amg_native_test.cpp, data_structure_test.cpp,
+matrix_market_test.cpp, linear_algebra_test.cpp), run over the HIP library.
+The option, CSR-validation and Matrix Market suites need no device and run on
+the CPU; the full program runs on the GPU."""
 import os
 import subprocess
 

@@ -1,12 +1,17 @@
 // solve-matrix: reads a Matrix Market matrix, solves A x = A x_exact with
 // BiCGStab (FP64, rtol 1e-8, atol 0, at most 1000 iterations, true residual
 // refreshed every 50) preconditioned by aggregation AMG or the identity, and
-// prints key=value lines (example/solve-matrix.cu:54-180,
-// docs/matrix-market.md:56-70).
+// prints key=value lines.  The command line, the keys, the manufactured
+// solution and the exit codes are the contract of example/solve-matrix.cu
+// and docs/matrix-market.md:40-70:
 //   usage: solve-matrix path [amg|identity] [v|w|f]
 //   exit 0: converged with host relative residual <= 1.01e-8 and relative
 //           solution error <= 1e-4; 2: solved but that check failed;
 //   1: bad arguments, unreadable or invalid matrix, or a runtime error.
+//
+// The program is organised as a request (parsed arguments), a problem (the
+// matrix with its manufactured right-hand side), the solve on the library,
+// and an independent host check of the answer with compensated sums.
 #include <thermofem/MatrixMarket.hpp>
 #include <thermofem/thermofem.hpp>
 
@@ -14,133 +19,205 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <map>
 
 namespace {
 
-const char* statusName(thermofem::SolveStatus s) {
-    switch (s) {
-    case thermofem::SolveStatus::Converged: return "Converged";
-    case thermofem::SolveStatus::MaxIterations: return "MaxIterations";
-    case thermofem::SolveStatus::Breakdown: return "Breakdown";
-    case thermofem::SolveStatus::NonFinite: return "NonFinite";
-    }
-    return "Unknown";
-}
+using Clock = std::chrono::steady_clock;
 
-// A x in long double, row by row in stored order.
-std::vector<double> hostProduct(const thermofem::io::HostCsr& a, const std::vector<double>& x) {
+struct Request {
+    std::string path;
+    bool amg = true;
+    std::string cycleName = "v";
+    thermofem::multigrid::Cycle cycle = thermofem::multigrid::Cycle::V;
+
+    static Request parse(int argc, char** argv) {
+        if (argc < 2 || argc > 4) throw std::invalid_argument("Usage: solve-matrix path [amg|identity] [v|w|f]");
+        static const std::map<std::string, thermofem::multigrid::Cycle> cycles{
+            {"v", thermofem::multigrid::Cycle::V},
+            {"w", thermofem::multigrid::Cycle::W},
+            {"f", thermofem::multigrid::Cycle::F}};
+        Request q;
+        q.path = argv[1];
+        const std::string pre = argc >= 3 ? argv[2] : "amg";
+        if (argc == 4) q.cycleName = argv[3];
+        const auto c = cycles.find(q.cycleName);
+        if (c == cycles.end()) throw std::invalid_argument("Unknown cycle");
+        if (pre != "amg" && pre != "identity") throw std::invalid_argument("Unknown preconditioner");
+        q.cycle = c->second;
+        q.amg = pre == "amg";
+        return q;
+    }
+};
+
+// Neumaier's compensated summation: FP64 sums with an error term, accurate
+// enough for the host check without extended precision.
+struct Sum {
+    double s = 0, c = 0;
+    void add(double v) {
+        const double t = s + v;
+        c += std::fabs(s) >= std::fabs(v) ? (s - t) + v : (v - t) + s;
+        s = t;
+    }
+    double value() const { return s + c; }
+};
+
+// y = A x with each row summed by compensated addition of the products.
+std::vector<double> rowProducts(const thermofem::io::HostCsr& a, const std::vector<double>& x) {
     std::vector<double> y(std::size_t(a.rows_));
-    for (int i = 0; i < a.rows_; ++i) {
-        long double acc = 0;
-        for (int k = a.offsets_[std::size_t(i)]; k < a.offsets_[std::size_t(i) + 1]; ++k)
-            acc += (long double)a.values_[std::size_t(k)] * x[std::size_t(a.indices_[std::size_t(k)])];
-        y[std::size_t(i)] = double(acc);
+    for (std::size_t i = 0; i < y.size(); ++i) {
+        Sum row;
+        for (int k = a.offsets_[i]; k < a.offsets_[i + 1]; ++k)
+            row.add(a.values_[std::size_t(k)] * x[std::size_t(a.indices_[std::size_t(k)])]);
+        y[i] = row.value();
     }
     return y;
 }
 
-double seconds(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
-    return std::chrono::duration<double>(b - a).count();
+struct Problem {
+    thermofem::io::HostCsr host;
+    std::vector<double> exact, rhs;
+    double readSeconds = 0;
+
+    explicit Problem(const std::string& path) {
+        const auto t0 = Clock::now();
+        std::ifstream in(path);
+        if (!in) throw std::runtime_error("Cannot open matrix file");
+        host = thermofem::io::readMatrixMarket(in);
+        if (host.rows_ <= 0 || host.rows_ != host.columns_)
+            throw std::invalid_argument("Expected nonempty square matrix");
+        readSeconds = std::chrono::duration<double>(Clock::now() - t0).count();
+        // manufactured solution (docs/matrix-market.md:46-50)
+        exact.resize(std::size_t(host.rows_));
+        for (std::size_t i = 0; i < exact.size(); ++i) {
+            const double t = double(i);
+            exact[i] = 1.0 + 0.25 * std::sin(0.013 * t) + 0.125 * std::cos(0.007 * t);
+        }
+        rhs = rowProducts(host, exact);
+    }
+    std::size_t rows() const { return exact.size(); }
+};
+
+// The host check of a solution: relative residual |b - A x| / |b| and
+// relative error against the manufactured solution.
+struct Check {
+    double residual = 0, error = 0;
+    bool finite = true;
+    Check(const Problem& p, const std::vector<double>& x) {
+        const auto ax = rowProducts(p.host, x);
+        Sum rr, bb, ee, xx;
+        for (std::size_t i = 0; i < x.size(); ++i) {
+            finite = finite && std::isfinite(x[i]) && std::isfinite(ax[i]);
+            const double r = p.rhs[i] - ax[i], e = x[i] - p.exact[i];
+            rr.add(r * r);
+            bb.add(p.rhs[i] * p.rhs[i]);
+            ee.add(e * e);
+            xx.add(p.exact[i] * p.exact[i]);
+        }
+        residual = std::sqrt(rr.value() / (bb.value() > 0 ? bb.value() : 1.0));
+        error = std::sqrt(ee.value() / xx.value());
+    }
+};
+
+const char* statusName(thermofem::SolveStatus s) {
+    static const char* const names[] = {"Converged", "MaxIterations", "Breakdown", "NonFinite"};
+    const auto k = std::size_t(s);
+    return k < 4 ? names[k] : "Unknown";
+}
+
+template <class V>
+void put(const char* key, const V& value) {
+    std::cout << key << '=' << value << '\n';
+}
+
+void printHierarchy(const thermofem::Amg<double>& amg, const Request& q, const Problem& p) {
+    put("cycle", q.cycleName);
+    put("levels", amg.levelCount());
+    put("stop_reason", amg.stopReason());
+    put("coarse_solver", amg.coarseSolver());
+    std::size_t rows = 0, nonzeros = 0, d = 0;
+    for (const auto& level : amg.levelStatistics()) {
+        rows += level.rows_;
+        nonzeros += level.nonzeros_;
+        std::cout << "level=" << d++ << " rows=" << level.rows_ << " nnz=" << level.nonzeros_ << " aggregates=";
+        for (int count : level.aggregateSizes_) std::cout << count << ',';
+        std::cout << '\n';
+    }
+    put("grid_complexity", double(rows) / double(p.rows()));
+    put("operator_complexity", double(nonzeros) / double(p.host.values_.size()));
+}
+
+int run(const Request& q) {
+    using namespace thermofem;
+    const Problem p(q.path);
+    put("matrix", q.path);
+    put("rows", p.rows());
+    put("nnz", p.host.values_.size());
+    put("preconditioner", q.amg ? "amg" : "identity");
+    put("read_seconds", p.readSeconds);
+
+    const int device = detail::currentDevice();
+    char gpu[256] = {0};
+    int runtime = 0;
+    detail::check(tf_device_info(device, gpu, sizeof gpu, &runtime), "device info");
+    put("device", device);
+    put("gpu", gpu);
+    put("hip_runtime", runtime);
+    BiCgStabOptions<double> options;
+    options.relativeTolerance_ = 1e-8;
+    put("rtol", options.relativeTolerance_);
+    put("atol", options.absoluteTolerance_);
+    put("max_iterations", options.maxIterations_);
+    put("refresh_interval", options.residualRefreshInterval_);
+
+    const std::size_t n = p.rows();
+    const CsrMatrix<double> A(n, n, p.host.offsets_, p.host.indices_, p.host.values_);
+    const Vector<double> b{std::span<const double>(p.rhs)};
+    Vector<double> x(n);
+    x.copyFromHost(std::vector<double>(n, 0.0));
+    AmgOptions<double> amgOptions;
+    amgOptions.cycle_ = q.cycle;
+    Amg<double>* hierarchy = nullptr;
+    std::unique_ptr<Preconditioner<double>> M;
+    if (q.amg) {
+        auto amg = std::make_unique<Amg<double>>(amgOptions);
+        hierarchy = amg.get();  // owned by the solver below; read after its setup
+        M = std::move(amg);
+    }
+    else {
+        M = std::make_unique<Identity<double>>();
+    }
+    BiCgStab<double> solver(options, std::move(M));
+
+    put("phase", "setup");
+    const auto s0 = Clock::now();
+    solver.setup(A);
+    const auto s1 = Clock::now();
+    put("setup_seconds", std::chrono::duration<double>(s1 - s0).count());
+    if (hierarchy) printHierarchy(*hierarchy, q, p);
+
+    put("phase", "solve");
+    const auto result = solver.apply(b, x);
+    const auto s2 = Clock::now();
+    const Check check(p, x.hostCopy());
+    put("status", statusName(result.status_));
+    put("iterations", result.iterations_);
+    put("solve_seconds", std::chrono::duration<double>(s2 - s1).count());
+    put("reported_relative_residual", result.relativeResidual_);
+    put("host_relative_residual", check.residual);
+    put("relative_solution_error", check.error);
+    const bool pass = check.finite && result.status_ == SolveStatus::Converged && check.residual <= 1.01e-8 &&
+                      check.error <= 1e-4;
+    put("validation", pass ? "PASS" : "FAIL");
+    return pass ? 0 : 2;
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
-    using namespace thermofem;
-    using Clock = std::chrono::steady_clock;
     std::cout << std::unitbuf << std::setprecision(12);
     try {
-        if (argc < 2 || argc > 4) throw std::invalid_argument("Usage: solve-matrix path [amg|identity] [v|w|f]");
-        const std::string mode = argc >= 3 ? argv[2] : "amg";
-        const std::string cycle = argc == 4 ? argv[3] : "v";
-        if (cycle != "v" && cycle != "w" && cycle != "f") throw std::invalid_argument("Unknown cycle");
-        if (mode != "amg" && mode != "identity") throw std::invalid_argument("Unknown preconditioner");
-
-        const auto t0 = Clock::now();
-        std::ifstream file(argv[1]);
-        if (!file) throw std::runtime_error("Cannot open matrix file");
-        const auto host = io::readMatrixMarket(file);
-        if (host.rows_ <= 0 || host.rows_ != host.columns_) throw std::invalid_argument("Expected nonempty square matrix");
-        const auto t1 = Clock::now();
-        const auto n = std::size_t(host.rows_);
-        std::cout << "matrix=" << argv[1] << "\nrows=" << n << "\nnnz=" << host.values_.size()
-                  << "\npreconditioner=" << mode << "\nread_seconds=" << seconds(t0, t1) << '\n';
-
-        std::vector<double> exact(n), initial(n, 0.0);
-        for (std::size_t i = 0; i < n; ++i)
-            exact[i] = 1.0 + 0.25 * std::sin(double(i) * 0.013) + 0.125 * std::cos(double(i) * 0.007);
-        const auto b = hostProduct(host, exact);
-
-        const int device = detail::currentDevice();
-        char name[256] = {0};
-        int runtime = 0;
-        detail::check(tf_device_info(device, name, sizeof name, &runtime), "device info");
-        std::cout << "device=" << device << "\ngpu=" << name << "\nhip_runtime=" << runtime << "\nrtol=" << 1e-8
-                  << "\natol=0\nmax_iterations=1000\nrefresh_interval=50\n";
-
-        const CsrMatrix<double> matrix(n, n, host.offsets_, host.indices_, host.values_);
-        const Vector<double> rhs{std::span<const double>(b)};
-        Vector<double> x{std::span<const double>(initial)};
-        AmgOptions<double> amgOptions;
-        amgOptions.cycle_ = cycle == "v" ? multigrid::Cycle::V : cycle == "w" ? multigrid::Cycle::W : multigrid::Cycle::F;
-        auto amg = std::make_unique<Amg<double>>(amgOptions);
-        Amg<double>* hierarchy = amg.get();
-        std::unique_ptr<Preconditioner<double>> preconditioner;
-        if (mode == "amg")
-            preconditioner = std::move(amg);
-        else
-            preconditioner = std::make_unique<Identity<double>>();
-        BiCgStabOptions<double> options;
-        options.relativeTolerance_ = 1e-8;
-        BiCgStab<double> solver(options, std::move(preconditioner));
-
-        std::cout << "phase=setup\n";
-        const auto t2 = Clock::now();
-        solver.setup(matrix);
-        const auto t3 = Clock::now();
-        std::cout << "setup_seconds=" << seconds(t2, t3) << '\n';
-        if (mode == "amg") {
-            std::cout << "cycle=" << cycle << "\nlevels=" << hierarchy->levelCount()
-                      << "\nstop_reason=" << hierarchy->stopReason() << "\ncoarse_solver=" << hierarchy->coarseSolver()
-                      << '\n';
-            std::size_t rowsTotal = 0, nnzTotal = 0;
-            const auto stats = hierarchy->levelStatistics();
-            for (std::size_t d = 0; d < stats.size(); ++d) {
-                rowsTotal += stats[d].rows_;
-                nnzTotal += stats[d].nonzeros_;
-                std::cout << "level=" << d << " rows=" << stats[d].rows_ << " nnz=" << stats[d].nonzeros_
-                          << " aggregates=";
-                for (int c : stats[d].aggregateSizes_) std::cout << c << ',';
-                std::cout << '\n';
-            }
-            std::cout << "grid_complexity=" << double(rowsTotal) / double(n)
-                      << "\noperator_complexity=" << double(nnzTotal) / double(host.values_.size()) << '\n';
-        }
-
-        std::cout << "phase=solve\n";
-        const auto report = solver.apply(rhs, x);
-        const auto t4 = Clock::now();
-        const auto solution = x.hostCopy();
-        const auto product = hostProduct(host, solution);
-        long double rr = 0, bb = 0, ee = 0, xx = 0;
-        bool finite = true;
-        for (std::size_t i = 0; i < n; ++i) {
-            finite = finite && std::isfinite(solution[i]) && std::isfinite(product[i]);
-            const long double r = (long double)b[i] - product[i], e = (long double)solution[i] - exact[i];
-            rr += r * r;
-            bb += (long double)b[i] * b[i];
-            ee += e * e;
-            xx += (long double)exact[i] * exact[i];
-        }
-        const double hostResidual = double(std::sqrt(rr / (bb > 0 ? bb : 1)));
-        const double solutionError = double(std::sqrt(ee / xx));
-        std::cout << "status=" << statusName(report.status_) << "\niterations=" << report.iterations_
-                  << "\nsolve_seconds=" << seconds(t3, t4) << "\nreported_relative_residual=" << report.relativeResidual_
-                  << "\nhost_relative_residual=" << hostResidual << "\nrelative_solution_error=" << solutionError << '\n';
-        const bool pass =
-            finite && report.status_ == SolveStatus::Converged && hostResidual <= 1.01e-8 && solutionError <= 1e-4;
-        std::cout << "validation=" << (pass ? "PASS" : "FAIL") << '\n';
-        return pass ? 0 : 2;
+        return run(Request::parse(argc, argv));
     }
     catch (const std::exception& e) {
         std::cerr << "error=" << e.what() << '\n';

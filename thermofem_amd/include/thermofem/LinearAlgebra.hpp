@@ -287,6 +287,32 @@ public:
         ctx_->sync();
     }
 
+    // One fused weighted-Jacobi sweep, out = x + (w D^-1)(rhs - A x), in one
+    // pass over the matrix (the library's addition to the reference's
+    // multiply + jacobi pair, with the same arithmetic); out must not alias x.
+    void sweep(T weight, const Vector<T>& inverse, const Vector<T>& rhs, const Vector<T>& x, Vector<T>& out) {
+        validate(inverse);
+        validate(rhs);
+        validate(x);
+        validate(out);
+        if (inverse.size() != x.size() || rhs.size() != x.size() || out.size() != x.size() ||
+            x.size() != matrix_.rowCount() || (x.size() && x.data() == out.data()))
+            throw std::invalid_argument("Jacobi shape or alias mismatch");
+        if (x.empty()) return;
+        thermofem::detail::DeviceScope scope(device_);
+        if constexpr (std::same_as<T, double>)
+            detail::check(tf_jacobi_fused_f64(ctx_->get(), native_.rows, native_.nnz, native_.rp.data(),
+                                              native_.ci.data(), values(), inverse.deviceData(), weight,
+                                              rhs.deviceData(), x.deviceData(), out.deviceData()),
+                          "jacobi sweep");
+        else
+            detail::check(tf_jacobi_fused_f32(ctx_->get(), native_.rows, native_.nnz, native_.rp.data(),
+                                              native_.ci.data(), values(), inverse.deviceData(), weight,
+                                              rhs.deviceData(), x.deviceData(), out.deviceData()),
+                          "jacobi sweep");
+        ctx_->sync();
+    }
+
     // Library-internal views for the solvers built on this context.
     [[nodiscard]] const detail::NativeCsr& native() const noexcept { return native_; }
     [[nodiscard]] tf_ctx* handle() const noexcept { return ctx_->get(); }

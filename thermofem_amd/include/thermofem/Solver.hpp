@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <functional>
 #include <memory>
+#include <optional>
 #include <string>
 
 #include "Multigrid.hpp"
@@ -67,7 +68,9 @@ void validateVectors(std::size_t n, int device, const Vector<T>& rhs, const Vect
 }
 }  // namespace iteration
 
-// Weighted Jacobi, x += w D^-1 (b - A x) per sweep; rhs and x may alias (Jacobi.cuh:26-93).
+// Weighted Jacobi, x += w D^-1 (b - A x) per sweep (Jacobi.cuh:26-93).  Each
+// sweep is one fused pass (IterationContext::sweep) between two iterate
+// buffers; the right-hand side is copied first, so rhs and x may alias.
 template <linalg::Scalar T, CsrIndex I = int>
 class Jacobi final : public Smoother<T, I> {
 public:
@@ -78,33 +81,31 @@ public:
     void setup(const CsrMatrix<T, I>& matrix) override {
         iteration::validateMatrix(matrix);
         detail::DeviceScope scope(matrix.rowOffsets().deviceId());
-        auto next = std::make_unique<State>(matrix);
-        next->inverse_ = linalg::inverseDiagonal(next->context_.matrix());
-        state_ = std::move(next);
+        auto w = std::make_unique<Work>(matrix);
+        w->dinv = linalg::inverseDiagonal(w->op.matrix());
+        work_ = std::move(w);
     }
     void apply(const Vector<T>& rhs, Vector<T>& x) override {
-        if (!state_) throw std::logic_error("Jacobi requires setup");
-        auto& s = *state_;
-        iteration::validateVectors(s.context_.matrix().rowCount(), s.context_.device(), rhs, x);
-        s.rhs_.copyDataFrom(rhs);
-        s.x_.copyDataFrom(x);
-        for (int sweep = 0; sweep < sweeps_; ++sweep) {
-            s.context_.multiply(s.x_, s.product_);
-            s.context_.jacobi(weight_, s.inverse_, s.rhs_, s.product_, s.x_);
-        }
-        x.copyDataFrom(s.x_);
+        if (!work_) throw std::logic_error("Jacobi requires setup");
+        Work& w = *work_;
+        iteration::validateVectors(w.op.matrix().rowCount(), w.op.device(), rhs, x);
+        w.b.copyDataFrom(rhs);
+        w.iterate[0].copyDataFrom(x);
+        int at = 0;
+        for (int k = 0; k < sweeps_; ++k, at ^= 1) w.op.sweep(weight_, w.dinv, w.b, w.iterate[at], w.iterate[at ^ 1]);
+        x.copyDataFrom(w.iterate[at]);
     }
 
 private:
-    struct State {
-        linalg::IterationContext<T, I> context_;
-        Vector<T> inverse_, rhs_, x_, product_;
-        explicit State(const CsrMatrix<T, I>& m)
-            : context_(m), inverse_(m.rowCount()), rhs_(m.rowCount()), x_(m.rowCount()), product_(m.rowCount()) {}
+    struct Work {
+        linalg::IterationContext<T, I> op;
+        Vector<T> dinv, b, iterate[2];
+        explicit Work(const CsrMatrix<T, I>& m)
+            : op(m), dinv(m.rowCount()), b(m.rowCount()), iterate{Vector<T>(m.rowCount()), Vector<T>(m.rowCount())} {}
     };
     int sweeps_;
     T weight_;
-    std::unique_ptr<State> state_;
+    std::unique_ptr<Work> work_;
 };
 
 // z = r (Identity.cuh:11-41).
@@ -179,27 +180,22 @@ public:
         setupGeneric(matrix);
     }
     void apply(const Vector<T>& rhs, Vector<T>& x) override {
-        if (factory_ ? !state_ : !engineReady_) throw std::logic_error("AMG requires setup");
+        if (factory_ ? !host_ : !engineReady_) throw std::logic_error("AMG requires setup");
         iteration::validateVectors(rows_, device_, rhs, x);
         detail::DeviceScope scope(device_);
         if (!factory_) {
             applyEngine(rhs, x);
             return;
         }
-        auto& first = *state_->levels_.front();
-        first.rhs_.copyDataFrom(rhs);
-        std::fill(state_->visits_.begin(), state_->visits_.end(), 0);
-        CycleAdapter adapter{*state_};
-        multigrid::executeCycle(options_.cycle_, 0, adapter, true);
-        x.copyDataFrom(first.x_);
+        host_->run(options_.cycle_, rhs, x);
     }
     [[nodiscard]] std::size_t levelCount() const noexcept {
-        if (factory_) return state_ ? state_->levels_.size() : 0;
+        if (factory_) return host_ ? host_->size() : 0;
         return engineReady_ ? std::size_t(tf_amg_level_count(engine_)) : 0;
     }
     [[nodiscard]] std::vector<AmgLevelStatistics> levelStatistics() const {
         require();
-        if (factory_) return state_->statistics_;
+        if (factory_) return host_->stats;
         std::vector<AmgLevelStatistics> out(levelCount());
         for (std::size_t d = 0; d < out.size(); ++d) {
             int64_t rows = 0, nnz = 0, hist[4] = {0, 0, 0, 0};
@@ -212,15 +208,15 @@ public:
     }
     [[nodiscard]] std::string stopReason() const {
         require();
-        return factory_ ? state_->stopReason_ : std::string(tf_amg_stop_reason(engine_));
+        return factory_ ? host_->stop : std::string(tf_amg_stop_reason(engine_));
     }
     [[nodiscard]] std::string coarseSolver() const {
         require();
-        return factory_ ? state_->coarseSolver_ : std::string(tf_amg_coarse_solver(engine_));
+        return factory_ ? host_->coarseSolver : std::string(tf_amg_coarse_solver(engine_));
     }
     [[nodiscard]] std::vector<std::size_t> lastCycleVisits() const {
         require();
-        if (factory_) return state_->visits_;
+        if (factory_) return host_->visits;
         std::vector<int64_t> v(levelCount());
         detail::check(tf_amg_visits(engine_, v.data()), "AMG visits");
         return std::vector<std::size_t>(v.begin(), v.end());
@@ -300,112 +296,112 @@ private:
         x.copyDataFrom(staged);
     }
 
-    // Host-driven hierarchy with injected smoothers (Amg.cuh:81-143,197-257).
-    struct Level {
-        linalg::IterationContext<T, I> context_;
-        Vector<T> rhs_, x_, residual_;
-        std::unique_ptr<Smoother<T, I>> pre_, post_;
-        std::unique_ptr<multigrid::AggregateTransfer> transfer_;
-        explicit Level(const CsrMatrix<T, I>& a)
-            : context_(a), rhs_(a.rowCount()), x_(a.rowCount()), residual_(a.rowCount()) {}
-    };
-    struct State {
-        std::vector<std::unique_ptr<Level>> levels_;
-        std::vector<AmgLevelStatistics> statistics_;
-        std::vector<std::size_t> visits_;
-        multigrid::CoarseFactor factor_;
-        std::unique_ptr<Smoother<T, I>> fallback_;
-        std::string stopReason_, coarseSolver_;
-    };
-    struct CycleAdapter {
-        State& state_;
-        bool isCoarsest(std::size_t d) const { return d + 1 == state_.levels_.size(); }
-        void begin(std::size_t d) { ++state_.visits_[d]; }
-        void clear(std::size_t d) {
-            auto& l = *state_.levels_[d];
-            l.context_.clear(l.x_);
+    // The host-driven hierarchy (injected smoothers, float levels, 64-bit
+    // indices): per-level operators, vectors, transfers and smoothers, kept as
+    // parallel arrays indexed by level, and the V/W/F schedule of
+    // multigrid::executeCycle run over them (Amg.cuh:81-143,197-257).  The
+    // coarsest level is solved by the pivoted LU or, where it could not be
+    // factored, by Jacobi(coarseSweeps, weight).
+    struct HostLevels {
+        struct Vectors {
+            Vector<T> b, x, r;
+            explicit Vectors(std::size_t n) : b(n), x(n), r(n) {}
+        };
+        std::vector<std::unique_ptr<linalg::IterationContext<T, I>>> op;
+        std::vector<std::unique_ptr<Vectors>> vec;
+        std::vector<multigrid::AggregateTransfer> down;  // level d -> d + 1
+        std::vector<std::unique_ptr<Smoother<T, I>>> pre, post;
+        std::vector<AmgLevelStatistics> stats;
+        std::vector<std::size_t> visits;
+        multigrid::CoarseFactor factor;
+        std::unique_ptr<Smoother<T, I>> coarseJacobi;
+        std::string stop, coarseSolver;
+
+        std::size_t size() const { return op.size(); }
+        void add(const CsrMatrix<T, I>& a, std::size_t nonzeros) {
+            op.push_back(std::make_unique<linalg::IterationContext<T, I>>(a));
+            vec.push_back(std::make_unique<Vectors>(a.rowCount()));
+            pre.emplace_back();
+            post.emplace_back();
+            stats.push_back({a.rowCount(), nonzeros, {}});
         }
-        void coarseSolve(std::size_t d) {
-            auto& l = *state_.levels_[d];
-            if (state_.factor_.factored_)
-                multigrid::solveCoarse(state_.factor_, l.rhs_, l.x_);
-            else
-                state_.fallback_->apply(l.rhs_, l.x_);
+        void run(multigrid::Cycle kind, const Vector<T>& rhs, Vector<T>& x) {
+            vec.front()->b.copyDataFrom(rhs);
+            visits.assign(size(), 0);
+            multigrid::executeCycle(kind, 0, *this, true);
+            x.copyDataFrom(vec.front()->x);
         }
+        // multigrid::CycleOperations over level d
+        bool isCoarsest(std::size_t d) const { return d + 1 == size(); }
+        void begin(std::size_t d) { ++visits[d]; }
+        void clear(std::size_t d) { op[d]->clear(vec[d]->x); }
         void preSmooth(std::size_t d) {
-            auto& l = *state_.levels_[d];
-            if (l.pre_) l.pre_->apply(l.rhs_, l.x_);
+            if (pre[d]) pre[d]->apply(vec[d]->b, vec[d]->x);
         }
         void postSmooth(std::size_t d) {
-            auto& l = *state_.levels_[d];
-            if (l.post_) l.post_->apply(l.rhs_, l.x_);
+            if (post[d]) post[d]->apply(vec[d]->b, vec[d]->x);
         }
-        void restrictRhs(std::size_t d) {
-            auto& l = *state_.levels_[d];
-            multigrid::restrictAggregates(*l.transfer_, l.rhs_, state_.levels_[d + 1]->rhs_);
-        }
+        void restrictRhs(std::size_t d) { multigrid::restrictAggregates(down[d], vec[d]->b, vec[d + 1]->b); }
         void restrictResidual(std::size_t d) {
-            auto& l = *state_.levels_[d];
-            l.context_.residual(l.rhs_, l.x_, l.residual_);
-            multigrid::restrictAggregates(*l.transfer_, l.residual_, state_.levels_[d + 1]->rhs_);
+            op[d]->residual(vec[d]->b, vec[d]->x, vec[d]->r);
+            multigrid::restrictAggregates(down[d], vec[d]->r, vec[d + 1]->b);
         }
-        void prolong(std::size_t d) {
-            auto& l = *state_.levels_[d];
-            multigrid::prolongAggregates(*l.transfer_, state_.levels_[d + 1]->x_, l.x_);
+        void prolong(std::size_t d) { multigrid::prolongAggregates(down[d], vec[d + 1]->x, vec[d]->x); }
+        void coarseSolve(std::size_t d) {
+            if (factor.factored_)
+                multigrid::solveCoarse(factor, vec[d]->b, vec[d]->x);
+            else
+                coarseJacobi->apply(vec[d]->b, vec[d]->x);
         }
     };
+    // Why coarsening ends at a level of `rows` rows that is the `count`-th
+    // (empty: it goes on), before the strength graph is built.
+    std::string stopBefore(std::size_t rows, std::size_t count) const {
+        if (rows <= std::size_t(options_.coarseRows_)) return "coarse size";
+        if (count == std::size_t(options_.maxLevels_)) return "level limit";
+        return {};
+    }
     void setupGeneric(const CsrMatrix<T, I>& matrix) {
-        auto next = std::make_unique<State>();
-        multigrid::GpuAggregation operations(options_.compactSortKeys_);
-        auto native = operations.canonical(multigrid::toNativeMatrix(matrix));
-        for (int depth = 0;; ++depth) {
-            operations.validate(native);
-            auto typed = multigrid::fromNativeMatrix<T, I>(native);
-            (void)linalg::inverseDiagonal(typed);
-            auto level = std::make_unique<Level>(typed);
-            next->statistics_.push_back({native.rowCount(), native.nonZeroCount(), {}});
-            if (native.rowCount() <= std::size_t(options_.coarseRows_)) {
-                next->stopReason_ = "coarse size";
-                next->levels_.push_back(std::move(level));
-                break;
-            }
-            if (depth + 1 == options_.maxLevels_) {
-                next->stopReason_ = "level limit";
-                next->levels_.push_back(std::move(level));
-                break;
-            }
-            auto graph = operations.strength(native, double(options_.strengthThreshold_));
+        auto h = std::make_unique<HostLevels>();
+        multigrid::GpuAggregation aggregation(options_.compactSortKeys_);
+        auto level = aggregation.canonical(multigrid::toNativeMatrix(matrix));
+        for (;;) {
+            aggregation.validate(level);
+            auto typed = multigrid::fromNativeMatrix<T, I>(level);
+            (void)linalg::inverseDiagonal(typed);  // rejects a zero or non-finite diagonal
+            h->add(typed, level.nonZeroCount());
+            h->stop = stopBefore(level.rowCount(), h->size());
+            if (!h->stop.empty()) break;
+            auto graph = aggregation.strength(level, double(options_.strengthThreshold_));
             if (!graph.nonZeroCount()) {
-                next->stopReason_ = "empty strength graph";
-                next->levels_.push_back(std::move(level));
+                h->stop = "empty strength graph";
                 break;
             }
-            auto transfer = operations.aggregate(graph, options_.matchingRounds_, options_.absorbSingletons_);
+            auto transfer = aggregation.aggregate(graph, options_.matchingRounds_, options_.absorbSingletons_);
             if (transfer.coarseRows_ >= transfer.fineRows_) {
-                next->stopReason_ = "no contraction";
-                next->levels_.push_back(std::move(level));
+                h->stop = "no contraction";
                 break;
             }
-            auto coarse = operations.galerkin(native, transfer);
-            next->statistics_.back().aggregateSizes_ = transfer.histogram_;
-            level->transfer_ = std::make_unique<multigrid::AggregateTransfer>(std::move(transfer));
-            if (options_.preSweeps_) level->pre_ = makeSmoother(typed);
-            if (options_.postSweeps_) level->post_ = makeSmoother(typed);
-            next->levels_.push_back(std::move(level));
-            native = std::move(coarse);
+            auto next = aggregation.galerkin(level, transfer);
+            const std::size_t d = h->size() - 1;
+            h->stats[d].aggregateSizes_ = transfer.histogram_;
+            if (options_.preSweeps_) h->pre[d] = makeSmoother(typed);
+            if (options_.postSweeps_) h->post[d] = makeSmoother(typed);
+            h->down.push_back(std::move(transfer));
+            level = std::move(next);
         }
-        if (native.rowCount() <= std::size_t(std::max(64, options_.coarseRows_)))
-            next->factor_ = multigrid::factorCoarse(native);
-        if (next->factor_.factored_) {
-            next->coarseSolver_ = "pivoted GPU LU";
+        if (level.rowCount() <= std::size_t(std::max(64, options_.coarseRows_)))
+            h->factor = multigrid::factorCoarse(level);
+        if (h->factor.factored_) {
+            h->coarseSolver = "pivoted GPU LU";
         }
         else {
-            next->coarseSolver_ = next->factor_.singular_ ? "Jacobi (singular LU)" : "Jacobi (coarsening stopped)";
-            next->fallback_ = std::make_unique<Jacobi<T, I>>(options_.coarseSweeps_, options_.jacobiWeight_);
-            next->fallback_->setup(next->levels_.back()->context_.matrix());
+            h->coarseSolver = h->factor.singular_ ? "Jacobi (singular LU)" : "Jacobi (coarsening stopped)";
+            h->coarseJacobi = std::make_unique<Jacobi<T, I>>(options_.coarseSweeps_, options_.jacobiWeight_);
+            h->coarseJacobi->setup(h->op.back()->matrix());
         }
-        next->visits_.resize(next->levels_.size());
-        state_ = std::move(next);
+        h->visits.assign(h->size(), 0);
+        host_ = std::move(h);
         rows_ = matrix.rowCount();
         device_ = matrix.rowOffsets().deviceId();
     }
@@ -416,12 +412,12 @@ private:
         return smoother;
     }
     void require() const {
-        if (factory_ ? !state_ : !engineReady_) throw std::logic_error("AMG requires setup");
+        if (factory_ ? !host_ : !engineReady_) throw std::logic_error("AMG requires setup");
     }
 
     AmgOptions<T> options_;
     SmootherFactory factory_;
-    std::unique_ptr<State> state_;
+    std::unique_ptr<HostLevels> host_;
     tf_amg* engine_ = nullptr;
     int engineDevice_ = -1;
     bool engineReady_ = false;
@@ -446,8 +442,80 @@ struct BiCgStabOptions {
 };
 
 namespace detail {
+// The scalars of a Krylov recurrence and the tests each reduction's value
+// goes through, as one state machine: the host twin of the control block the
+// library's device-resident iteration keeps (a step either continues or
+// records the status the recurrence stops with; Cg.cuh:123-189,
+// BiCgStab.cuh:144-247).  count is the iteration under way (1-based).
+template <linalg::Scalar T>
+struct Recurrence {
+    T normB{}, tol{}, res{}, rho{}, rhoPrev{}, alpha{1}, beta{}, omega{1};
+    int count = 0, limit, refreshEvery;
+    bool restart = true;  // the next direction starts afresh from the (preconditioned) residual
+    std::optional<SolveStatus> stop;
+
+    Recurrence(int maxIterations, int refreshInterval) : limit(maxIterations), refreshEvery(refreshInterval) {}
+    bool running() const { return !stop && count < limit; }
+    bool halt(SolveStatus why) {
+        stop = why;
+        return false;
+    }
+    bool finite(T v) { return std::isfinite(v) || halt(SolveStatus::NonFinite); }
+    int iterations() const { return stop ? count : limit; }
+    // status from the final true residual
+    SolveStatus classify(T residual) const {
+        const SolveStatus s = stop.value_or(SolveStatus::MaxIterations);
+        if (!std::isfinite(residual) || !std::isfinite(normB)) return SolveStatus::NonFinite;
+        if (residual <= tol) return SolveStatus::Converged;
+        return s == SolveStatus::Converged ? SolveStatus::MaxIterations : s;
+    }
+    // CG: rho = r.z; breakdown unless positive; beta for a continued direction
+    bool cgRho(T value) {
+        ++count;
+        rho = value;
+        if (!finite(rho)) return false;
+        if (rho <= 0) return halt(SolveStatus::Breakdown);
+        if (restart) return true;
+        beta = rho / rhoPrev;
+        return finite(beta);
+    }
+    // CG: alpha = rho / p.Ap; breakdown unless the curvature is positive
+    bool cgCurvature(T value) {
+        if (!finite(value)) return false;
+        if (value <= 0) return halt(SolveStatus::Breakdown);
+        alpha = rho / value;
+        return finite(alpha);
+    }
+    // BiCGStab: rho = r_hat.r; breakdown at zero; beta for a continued direction
+    bool biRho(T value) {
+        ++count;
+        rho = value;
+        if (!finite(rho)) return false;
+        if (rho == 0) return halt(SolveStatus::Breakdown);
+        if (restart) return true;
+        beta = (rho / rhoPrev) * (alpha / omega);
+        return finite(beta);
+    }
+    // BiCGStab: alpha = rho / r_hat.v
+    bool biAlpha(T denominator) {
+        if (!finite(denominator)) return false;
+        if (denominator == 0) return halt(SolveStatus::Breakdown);
+        alpha = rho / denominator;
+        return finite(alpha);
+    }
+    // BiCGStab: omega = t.s / t.t
+    bool biOmega(T tt, T ts) {
+        if (!finite(tt) || !finite(ts)) return false;
+        if (tt == 0) return halt(SolveStatus::Breakdown);
+        omega = ts / tt;
+        if (!finite(omega)) return false;
+        return omega != 0 || halt(SolveStatus::Breakdown);
+    }
+};
+
 // Shared machinery of Cg and BiCgStab: option checks, the state snapshot, the
-// engine path and finish() (Cg.cuh:105-119, BiCgStab.cuh:122-136).
+// engine path, the start of a host-driven solve, its residual refresh and
+// its report.
 template <linalg::Scalar T, CsrIndex I, class Options>
 class KrylovBase : public Solver<T, I> {
 protected:
@@ -533,17 +601,46 @@ protected:
         }
         return false;
     }
-    SolveResult<T> finish(SolveStatus status, int iterations, T normB, T tolerance) {
+    // The report of a host-driven solve: the true residual b - A x decides
+    // the status (Cg.cuh:105-119, BiCgStab.cuh:122-136).
+    SolveResult<T> report(const Recurrence<T>& k) {
         auto& s = *state_;
         s.context_.residual(s.rhs_, s.x_, s.r_);
-        const T residual = s.context_.norm(s.r_);
-        if (!std::isfinite(residual) || !std::isfinite(normB))
-            status = SolveStatus::NonFinite;
-        else if (residual <= tolerance)
-            status = SolveStatus::Converged;
-        else if (status == SolveStatus::Converged)
-            status = SolveStatus::MaxIterations;
-        return {status, iterations, residual, normB > 0 ? residual / normB : residual};
+        const T res = s.context_.norm(s.r_);
+        return {k.classify(res), k.iterations(), res, k.normB > 0 ? res / k.normB : res};
+    }
+    // normB, the tolerance and the initial residual r = b - A x: a recurrence
+    // that has already stopped when either is non-finite or x already meets
+    // the tolerance.
+    Recurrence<T> start() {
+        auto& s = *state_;
+        const auto& o = options_;
+        Recurrence<T> k(o.maxIterations_, o.residualRefreshInterval_);
+        k.normB = s.context_.norm(s.rhs_);
+        k.tol = std::max(o.absoluteTolerance_, o.relativeTolerance_ * k.normB);
+        s.context_.residual(s.rhs_, s.x_, s.r_);
+        k.res = s.context_.norm(s.r_);
+        if (!std::isfinite(k.normB) || !std::isfinite(k.res))
+            k.halt(SolveStatus::NonFinite);
+        else if (k.res <= k.tol)
+            k.halt(SolveStatus::Converged);
+        return k;
+    }
+    // |r| after an update: stops on a non-finite value; when the residual meets
+    // the tolerance or a refresh is due, recomputes r = b - A x and stops if
+    // that one meets it (or is non-finite), otherwise restarts the direction.
+    // True when the recurrence continues.
+    bool refresh(Recurrence<T>& k, T updatedNorm, Vector<T>* shadow = nullptr) {
+        auto& s = *state_;
+        if (!k.finite(updatedNorm)) return false;
+        k.res = updatedNorm;
+        if (k.res > k.tol && k.count % k.refreshEvery != 0) return true;
+        s.context_.residual(s.rhs_, s.x_, s.r_);
+        k.res = s.context_.norm(s.r_);
+        if (!std::isfinite(k.res) || k.res <= k.tol) return k.halt(SolveStatus::MaxIterations);
+        if (shadow) shadow->copyDataFrom(s.r_);
+        k.restart = true;
+        return true;
     }
     Options options_;
     std::unique_ptr<Preconditioner<T, I>> preconditioner_;
@@ -564,54 +661,31 @@ public:
         : Base(options, preconditioner ? std::move(preconditioner) : std::make_unique<Identity<T, I>>(), 0, "CG") {}
 
 private:
+    // z = M r, rho, direction, q = A p, alpha, x and r updated, |r| with the
+    // refresh test -- each reduction checked by the recurrence.
     SolveResult<T> solve() override {
-        auto& s = *this->state_;
-        auto& op = s.context_;
-        const auto& o = this->options_;
-        const T normB = op.norm(s.rhs_);
-        const T tol = std::max(o.absoluteTolerance_, o.relativeTolerance_ * normB);
-        op.residual(s.rhs_, s.x_, s.r_);
-        T residual = op.norm(s.r_);
-        if (!std::isfinite(normB) || !std::isfinite(residual)) return this->finish(SolveStatus::NonFinite, 0, normB, tol);
-        if (residual <= tol) return this->finish(SolveStatus::Converged, 0, normB, tol);
-        T previousRho = 0;
-        bool fresh = true;
-        for (int k = 0; k < o.maxIterations_; ++k) {
-            const int it = k + 1;
-            this->preconditioner_->apply(s.r_, s.z_);
-            const T rho = op.dot(s.r_, s.z_);
-            if (!std::isfinite(rho)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-            if (rho <= 0) return this->finish(SolveStatus::Breakdown, it, normB, tol);
-            if (fresh) {
-                s.p_.copyDataFrom(s.z_);
-                fresh = false;
+        auto& v = *this->state_;
+        auto& A = v.context_;
+        auto k = this->start();
+        while (k.running()) {
+            this->preconditioner_->apply(v.r_, v.z_);
+            if (!k.cgRho(A.dot(v.r_, v.z_))) break;
+            if (k.restart) {
+                v.p_.copyDataFrom(v.z_);
+                k.restart = false;
             }
             else {
-                const T beta = rho / previousRho;
-                if (!std::isfinite(beta)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-                op.scale(beta, s.p_);
-                op.axpy(T(1), s.z_, s.p_);
+                A.scale(k.beta, v.p_);
+                A.axpy(T(1), v.z_, v.p_);
             }
-            op.multiply(s.p_, s.product_);
-            const T curvature = op.dot(s.p_, s.product_);
-            if (!std::isfinite(curvature)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-            if (curvature <= 0) return this->finish(SolveStatus::Breakdown, it, normB, tol);
-            const T alpha = rho / curvature;
-            if (!std::isfinite(alpha)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-            op.axpy(alpha, s.p_, s.x_);
-            op.axpy(-alpha, s.product_, s.r_);
-            residual = op.norm(s.r_);
-            if (!std::isfinite(residual)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-            if (residual <= tol || it % o.residualRefreshInterval_ == 0) {
-                op.residual(s.rhs_, s.x_, s.r_);
-                residual = op.norm(s.r_);
-                if (!std::isfinite(residual) || residual <= tol)
-                    return this->finish(SolveStatus::MaxIterations, it, normB, tol);
-                fresh = true;
-            }
-            previousRho = rho;
+            A.multiply(v.p_, v.product_);
+            if (!k.cgCurvature(A.dot(v.p_, v.product_))) break;
+            A.axpy(k.alpha, v.p_, v.x_);
+            A.axpy(-k.alpha, v.product_, v.r_);
+            if (!this->refresh(k, A.norm(v.r_))) break;
+            k.rhoPrev = k.rho;
         }
-        return this->finish(SolveStatus::MaxIterations, o.maxIterations_, normB, tol);
+        return this->report(k);
     }
 };
 
@@ -625,82 +699,58 @@ public:
         : Base(options, preconditioner ? std::move(preconditioner) : std::make_unique<Amg<T, I>>(), 1, "BiCGStab") {}
 
 private:
+    // Right-preconditioned: p_hat = M p, v = A p_hat, alpha, s, and either the
+    // early exit on |s| (refresh as a restart) or s_hat = M s, t = A s_hat,
+    // omega, x and r updated, |r| with the refresh test.
     SolveResult<T> solve() override {
-        auto& st = *this->state_;
-        auto& op = st.context_;
-        const auto& o = this->options_;
+        auto& w = *this->state_;
+        auto& A = w.context_;
         auto& M = *this->preconditioner_;
-        const T normB = op.norm(st.rhs_);
-        const T tol = std::max(o.absoluteTolerance_, o.relativeTolerance_ * normB);
-        op.residual(st.rhs_, st.x_, st.r_);
-        T residual = op.norm(st.r_);
-        if (!std::isfinite(normB) || !std::isfinite(residual)) return this->finish(SolveStatus::NonFinite, 0, normB, tol);
-        if (residual <= tol) return this->finish(SolveStatus::Converged, 0, normB, tol);
-        st.shadow_.copyDataFrom(st.r_);
-        T rhoPrevious = 1, alpha = 1, omega = 1;
-        bool fresh = true;
-        Vector<T>& v = st.product_;
-        for (int k = 0; k < o.maxIterations_; ++k) {
-            const int it = k + 1;
-            const T rho = op.dot(st.shadow_, st.r_);
-            if (!std::isfinite(rho)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-            if (rho == 0) return this->finish(SolveStatus::Breakdown, it, normB, tol);
-            if (fresh) {
-                st.p_.copyDataFrom(st.r_);
-                fresh = false;
+        Vector<T>& v = w.product_;
+        auto k = this->start();
+        if (k.running()) w.shadow_.copyDataFrom(w.r_);
+        while (k.running()) {
+            if (!k.biRho(A.dot(w.shadow_, w.r_))) break;
+            if (k.restart) {
+                w.p_.copyDataFrom(w.r_);
+                k.restart = false;
             }
             else {
-                const T beta = (rho / rhoPrevious) * (alpha / omega);
-                if (!std::isfinite(beta)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-                op.axpy(-omega, v, st.p_);
-                op.scale(beta, st.p_);
-                op.axpy(T(1), st.r_, st.p_);
+                A.axpy(-k.omega, v, w.p_);
+                A.scale(k.beta, w.p_);
+                A.axpy(T(1), w.r_, w.p_);
             }
-            M.apply(st.p_, st.ph_);
-            op.multiply(st.ph_, v);
-            const T denominator = op.dot(st.shadow_, v);
-            if (!std::isfinite(denominator)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-            if (denominator == 0) return this->finish(SolveStatus::Breakdown, it, normB, tol);
-            alpha = rho / denominator;
-            if (!std::isfinite(alpha)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-            st.s_.copyDataFrom(st.r_);
-            op.axpy(-alpha, v, st.s_);
-            op.axpy(alpha, st.ph_, st.x_);
-            const T normS = op.norm(st.s_);
-            if (!std::isfinite(normS)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-            if (normS <= tol) {
-                op.residual(st.rhs_, st.x_, st.r_);
-                residual = op.norm(st.r_);
-                if (residual <= tol || !std::isfinite(residual))
-                    return this->finish(SolveStatus::MaxIterations, it, normB, tol);
-                st.shadow_.copyDataFrom(st.r_);
-                fresh = true;
+            M.apply(w.p_, w.ph_);
+            A.multiply(w.ph_, v);
+            if (!k.biAlpha(A.dot(w.shadow_, v))) break;
+            w.s_.copyDataFrom(w.r_);
+            A.axpy(-k.alpha, v, w.s_);
+            A.axpy(k.alpha, w.ph_, w.x_);
+            const T normS = A.norm(w.s_);
+            if (!k.finite(normS)) break;
+            if (normS <= k.tol) {
+                // s already meets the tolerance: the recomputed residual decides
+                // (a restart keeps rhoPrev)
+                A.residual(w.rhs_, w.x_, w.r_);
+                k.res = A.norm(w.r_);
+                if (k.res <= k.tol || !std::isfinite(k.res)) {
+                    k.halt(SolveStatus::MaxIterations);
+                    break;
+                }
+                w.shadow_.copyDataFrom(w.r_);
+                k.restart = true;
                 continue;
             }
-            M.apply(st.s_, st.sh_);
-            op.multiply(st.sh_, st.t_);
-            const T tt = op.dot(st.t_, st.t_), ts = op.dot(st.t_, st.s_);
-            if (!std::isfinite(tt) || !std::isfinite(ts)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-            if (tt == 0) return this->finish(SolveStatus::Breakdown, it, normB, tol);
-            omega = ts / tt;
-            if (!std::isfinite(omega)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-            if (omega == 0) return this->finish(SolveStatus::Breakdown, it, normB, tol);
-            op.axpy(omega, st.sh_, st.x_);
-            st.r_.copyDataFrom(st.s_);
-            op.axpy(-omega, st.t_, st.r_);
-            residual = op.norm(st.r_);
-            if (!std::isfinite(residual)) return this->finish(SolveStatus::NonFinite, it, normB, tol);
-            if (residual <= tol || it % o.residualRefreshInterval_ == 0) {
-                op.residual(st.rhs_, st.x_, st.r_);
-                residual = op.norm(st.r_);
-                if (residual <= tol || !std::isfinite(residual))
-                    return this->finish(SolveStatus::MaxIterations, it, normB, tol);
-                st.shadow_.copyDataFrom(st.r_);
-                fresh = true;
-            }
-            rhoPrevious = rho;
+            M.apply(w.s_, w.sh_);
+            A.multiply(w.sh_, w.t_);
+            if (!k.biOmega(A.dot(w.t_, w.t_), A.dot(w.t_, w.s_))) break;
+            A.axpy(k.omega, w.sh_, w.x_);
+            w.r_.copyDataFrom(w.s_);
+            A.axpy(-k.omega, w.t_, w.r_);
+            if (!this->refresh(k, A.norm(w.r_), &w.shadow_)) break;
+            k.rhoPrev = k.rho;
         }
-        return this->finish(SolveStatus::MaxIterations, o.maxIterations_, normB, tol);
+        return this->report(k);
     }
 };
 
