@@ -818,15 +818,22 @@ __global__ void k_merge_write(int rows, const int* __restrict__ start, const u64
 // ---- matching (GpuAggregation.cu:402-505) ----
 // A round after one that accepted nothing would reproduce the same
 // proposals and accept nothing again, so it is skipped (prevChanged == 0).
+// A vertex whose scan found no candidate is marked kNoCandidate (still an
+// unmatched vertex to every reader of mate, which only tests mate < 0): the
+// unmatched vertices only ever become fewer, so it can never find one later,
+// and later rounds skip its scan (its proposal stays "none").
+constexpr int kNoCandidate = -2;
+
 __global__ void k_propose(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ w,
-                          const int* __restrict__ mate, int* __restrict__ proposal, const int* prevChanged) {
+                          int* __restrict__ mate, int* __restrict__ proposal, const int* prevChanged, int mark) {
     if (prevChanged && *prevChanged == 0) return;
     const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (i >= n) return;
     int best = -1;
     double score = 0;
     u64 priority = 0;
-    if (mate[i] < 0) {
+    const int mi = mate[i];
+    if (mi < 0 && mi != kNoCandidate) {
         // The best candidate is the maximum of a total order (weight, then
         // priority, then the smaller column), so entries can be examined in
         // chunks: the loads of 8 entries and their mates' lookups go out
@@ -856,6 +863,7 @@ __global__ void k_propose(int n, const int* __restrict__ rp, const int* __restri
                 }
             }
         }
+        if (best < 0 && mark) mate[i] = kNoCandidate;
     }
     proposal[i] = best;
 }
@@ -1029,10 +1037,14 @@ __global__ __launch_bounds__(kBlock) void k_pack_members(int nc, const int* __re
 
 // ---------------- host helpers ----------------
 
+// mark: record vertices left without a candidate (kNoCandidate) so later
+// rounds skip them; the single-round entry point leaves mate as the
+// reference's matching would.
 void match_round_flags(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* w, int* mate,
-                       int* proposal, const int* prevChanged, int* changed) {
+                       int* proposal, const int* prevChanged, int* changed, bool mark = true) {
     (void)nnz;  // one thread per row measured faster here than a group of lanes per row
-    hipLaunchKernelGGL(k_propose, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, w, mate, proposal, prevChanged);
+    hipLaunchKernelGGL(k_propose, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, w, mate, proposal, prevChanged,
+                       int(mark));
     check(hipGetLastError(), "propose launch");
     hipLaunchKernelGGL(k_accept, grid1(n), dim3(kBlock), 0, c.stream, n, proposal, mate, prevChanged, changed);
     check(hipGetLastError(), "accept launch");
@@ -1342,7 +1354,7 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
 }
 
 void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal) {
-    match_round_flags(c, n, n > 0 ? c.read_int(rp + n) : 0, rp, ci, w, mate, proposal, nullptr, nullptr);
+    match_round_flags(c, n, n > 0 ? c.read_int(rp + n) : 0, rp, ci, w, mate, proposal, nullptr, nullptr, false);
 }
 
 int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map, int64_t nnz) {

@@ -69,6 +69,36 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(int n, const int* in, int
     }
 }
 
+// Up to this many tiles, each tile adds the totals of the tiles before it
+// itself (at most 16 KB of reads a block), which saves the launch that would
+// scan the totals.
+constexpr int kDirectPrefixTiles = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply_direct(int n, const int* in, int* out,
+                                                              const int* __restrict__ totals) {
+    __shared__ int sBase;
+    int before = 0;
+    for (int t = int(threadIdx.x); t < int(blockIdx.x); t += kBlock) before += totals[t];
+    before = block_sum(before);
+    if (threadIdx.x == 0) sBase = before;
+    __syncthreads();
+    const int base = int(blockIdx.x) * kScanTile + int(threadIdx.x) * kScanItems;
+    int items[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        items[k] = base + k < n ? in[base + k] : 0;
+        s += items[k];
+    }
+    int total;
+    int run = block_excl_scan(s, &total) + sBase;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += items[k];
+    }
+}
+
 void scan_rec(tf_ctx& c, int n, const int* in, int* out, int* scratch) {
     const int nb = (n + kScanTile - 1) / kScanTile;
     if (nb <= 1) {
@@ -79,6 +109,11 @@ void scan_rec(tf_ctx& c, int n, const int* in, int* out, int* scratch) {
     int* aux = scratch;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, c.stream, n, in, aux);
     check(hipGetLastError(), "scan reduce launch");
+    if (nb <= kDirectPrefixTiles) {
+        hipLaunchKernelGGL(k_scan_apply_direct, dim3(nb), dim3(kBlock), 0, c.stream, n, in, out, aux);
+        check(hipGetLastError(), "scan apply launch");
+        return;
+    }
     scan_rec(c, nb, aux, aux, scratch + nb);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, c.stream, n, in, out, aux);
     check(hipGetLastError(), "scan apply launch");
