@@ -77,12 +77,28 @@ __global__ void k_csr_check(int n, int nnz, const int* __restrict__ rp, const in
         atomicOr(flags, 2);
         return;
     }
-    int previous = -1;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
-        if (ci[j] < 0 || ci[j] >= n || !isfinite(v[j])) atomicOr(flags, 2);
-        if (ci[j] <= previous) atomicOr(flags, 1);
-        previous = ci[j];
+    // entries in chunks of 8 whose loads go out together (indices clamped
+    // into the row, extra slots ignored); one flag update per thread
+    int previous = -1, bad = 0;
+    const int e = rp[i + 1];
+    for (int j0 = rp[i]; j0 < e; j0 += 8) {
+        int cc[8];
+        double vv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = min(j0 + u, e - 1);
+            cc[u] = ci[j];
+            vv[u] = v[j];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (j0 + u >= e) break;
+            if (cc[u] < 0 || cc[u] >= n || !isfinite(vv[u])) bad |= 2;
+            if (cc[u] <= previous) bad |= 1;
+            previous = cc[u];
+        }
     }
+    if (bad) atomicOr(flags, bad);
 }
 
 // Device CSR structure check of CsrMatrix::fromBuffers (CsrMatrix.cu:12-31).
@@ -105,14 +121,29 @@ __global__ void k_validate(int n, const int* __restrict__ rp, const int* __restr
     const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (i >= n) return;
     double diag = 0, mx = 0;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
-        if (!isfinite(a[j])) atomicOr(bad, 1);
-        if (ci[j] == i)
-            diag = a[j];
-        else
-            mx = fmax(mx, fabs(a[j]));
+    bool invalid = false;
+    // chunks of 8 entries whose loads go out together (clamped indices)
+    const int e = rp[i + 1];
+    for (int j0 = rp[i]; j0 < e; j0 += 8) {
+        int cc[8];
+        double aa[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = min(j0 + u, e - 1);
+            cc[u] = ci[j];
+            aa[u] = a[j];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (j0 + u >= e) break;
+            invalid |= !isfinite(aa[u]);
+            if (cc[u] == i)
+                diag = aa[u];
+            else
+                mx = fmax(mx, fabs(aa[u]));
+        }
     }
-    if (diag == 0 || !isfinite(diag) || !isfinite(1.0 / diag)) atomicOr(bad, 1);
+    if (invalid || diag == 0 || !isfinite(diag) || !isfinite(1.0 / diag)) atomicOr(bad, 1);
     if (rowmax) rowmax[i] = mx;
 }
 
@@ -198,9 +229,12 @@ __global__ void k_struct_sym(int n, const int* __restrict__ rp, const int* __res
 template <int G>
 __global__ void k_strength_sym(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                const double* __restrict__ a, const double* __restrict__ rowmax, double theta,
-                               int* __restrict__ scol, double* __restrict__ sw, int* __restrict__ cnt) {
+                               int* __restrict__ scol, double* __restrict__ sw, int* __restrict__ cnt,
+                               int* nonsym) {
     // G lanes per row, one entry per lane: the loads of a row are coalesced and
-    // the binary searches in the neighbours' rows run side by side.
+    // the binary searches in the neighbours' rows run side by side.  With
+    // nonsym, an entry without its transpose raises the flag (the pattern
+    // was not known to be symmetric: the result is then discarded).
     const int i = int((int64_t(blockIdx.x) * kBlock + threadIdx.x) / G);
     const int lane = int(threadIdx.x) % G;
     if (i == n && lane == 0) cnt[n] = 0;
@@ -216,6 +250,7 @@ __global__ void k_strength_sym(int n, const int* __restrict__ rp, const int* __r
             const double qi = (wi >= theta && wi > 0) ? wi : 0.0;
             const double mc = rowmax[c];
             const int k = find_col(ci, rp[c], rp[c + 1], i);
+            if (k < 0 && nonsym) atomicOr(nonsym, 1);
             const double wc = (mc > 0 && k >= 0) ? fabs(a[k]) / mc : 0.0;
             const double qc = (wc >= theta && wc > 0) ? wc : 0.0;
             w = qi > qc ? qi : qc;
@@ -1276,16 +1311,25 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
     double* rowmax = rowmax_workspace(c, n);
     if (!prevalidated && !validate_rowmax(c, n, rp, ci, v, rowmax))
         throw Error(TF_E_NUMERIC, "Native AMG: missing/zero diagonal, non-finite value or reciprocal in hierarchy");
-    if (symmetric < 0) {
+    // An unknown pattern takes the symmetric path with a transpose check in
+    // the same pass (flag slot `deferred`, read back with the graph's size);
+    // a matrix that fails it takes the general path.
+    const bool probe = symmetric < 0;
+    if (probe && learned) {
         c.clear_flags(1);
         hipLaunchKernelGGL(k_struct_sym, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, c.dflags);
         check(hipGetLastError(), "symmetry check launch");
         c.read_flags(1);
         symmetric = c.hflags[0] == 0 ? 1 : 0;
     }
-    if (symmetricOut) *symmetricOut = symmetric;
-    require(!learned || symmetric, "learned affinities need a matrix with a symmetric pattern");
-    if (symmetric) {
+    require(!learned || symmetric == 1, "learned affinities need a matrix with a symmetric pattern");
+    const int slot = std::max(deferred, 0);
+    if (symmetric != 0) {
+        int* nonsym = nullptr;
+        if (symmetric < 0) {
+            nonsym = c.dflags + slot;
+            check(hipMemsetAsync(nonsym, 0, sizeof(int), c.stream), "symmetry flag clear");
+        }
         int* cnt = c.wst<int>(kCount, size_t(n) + 1);
         int* scol = c.wst<int>(kKeys, size_t(nnz) + 1);
         double* sw = c.wst<double>(kVals, size_t(nnz) + 1);
@@ -1298,7 +1342,7 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
 #define TF_STRENGTH(G)                                                                                               \
     case G:                                                                                                          \
         hipLaunchKernelGGL(k_strength_sym<G>, grid_rows(int64_t(n) + 1, G), dim3(kBlock), 0, c.stream, n, rp, ci, v, \
-                           rowmax, theta, scol, sw, cnt);                                                            \
+                           rowmax, theta, scol, sw, cnt, nonsym);                                                    \
         break;
             switch (lanes) {
                 TF_STRENGTH(4) TF_STRENGTH(8) TF_STRENGTH(16) TF_STRENGTH(32)
@@ -1310,7 +1354,14 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
         g.n = n;
         g.rp = dev_alloc<int>(c, size_t(n) + 1);
         scan_excl(c, n + 1, cnt, g.rp);
-        g.nnz = deferred > 0 ? c.read_int_with_flags(g.rp + n, deferred) : c.read_int(g.rp + n);
+        const int flags = nonsym ? slot + 1 : deferred;
+        g.nnz = flags > 0 ? c.read_int_with_flags(g.rp + n, flags) : c.read_int(g.rp + n);
+        if (nonsym) symmetric = c.hflags[slot] == 0 ? 1 : 0;
+        if (symmetricOut) *symmetricOut = symmetric;
+        if (!symmetric) {
+            dev_free(c, g.rp);
+            return strength_graph(c, n, nnz, rp, ci, v, theta, 0, true, nullptr, nullptr, deferred);
+        }
         g.ci = dev_alloc<int>(c, size_t(g.nnz));
         g.v = dev_alloc<double>(c, size_t(g.nnz));
 #define TF_COMPACT(G)                                                                                            \
@@ -1325,6 +1376,7 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
         check(hipGetLastError(), "strength compaction launch");
         return g;
     }
+    if (symmetricOut) *symmetricOut = 0;
     int* cnt = c.wst<int>(kCount, size_t(n) + 1);
     fill_i32(c, n + 1, 0, cnt);
     hipLaunchKernelGGL(k_strength_count, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, v, rowmax, theta, cnt);
