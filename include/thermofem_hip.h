@@ -362,6 +362,35 @@ int tf_sell_jacobi_f64(tf_ctx* ctx, const tf_sell* sell, const double* inv, doub
 int tf_sell_jacobi_f32(tf_ctx* ctx, const tf_sell* sell, const float* inv, float w, const float* b,
                        const float* x_in, float* x_out);
 
+/* ---- structured-stencil row products: the same three operations over a
+ * storage without column indices, for operators whose rows all take their
+ * columns from one list of at most 16 offsets from the row (structured-grid
+ * and finite-difference matrices): a 16-bit term mask per row and one dense
+ * value array per offset; for a bit-for-bit symmetric operator only the
+ * offsets >= 0 are stored and a(i, i-d) is read as a(i-d, i).  No reference
+ * counterpart: the products replace IterationContext::multiply / residual
+ * (IterationContext.cu:26-32,75,155) for such operators, with results equal
+ * to the CSR entry points' bit for bit.  The CSR must be canonical (sorted,
+ * unique columns per row).  The handle keeps FP64 and FP32 values and
+ * belongs to the context it was created on. */
+typedef struct tf_stencil tf_stencil;
+/* TF_E_INVALID (no handle) if the matrix is not canonical or its rows use
+ * more than 16 distinct offsets. */
+int tf_stencil_create(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v,
+                      tf_stencil** stencil);
+int tf_stencil_destroy(tf_stencil* stencil);
+/* terms: distinct offsets; arrays: value arrays stored; mirrored: 1 when the
+ * negative offsets read the positive ones' arrays */
+int tf_stencil_info(const tf_stencil* stencil, int* terms, int* arrays, int* mirrored);
+int tf_stencil_spmv_f64(tf_ctx* ctx, const tf_stencil* stencil, const double* x, double* y);
+int tf_stencil_spmv_f32(tf_ctx* ctx, const tf_stencil* stencil, const float* x, float* y);
+int tf_stencil_residual_f64(tf_ctx* ctx, const tf_stencil* stencil, const double* b, const double* x, double* r);
+int tf_stencil_residual_f32(tf_ctx* ctx, const tf_stencil* stencil, const float* b, const float* x, float* r);
+int tf_stencil_jacobi_f64(tf_ctx* ctx, const tf_stencil* stencil, const double* inv, double w, const double* b,
+                          const double* x_in, double* x_out);
+int tf_stencil_jacobi_f32(tf_ctx* ctx, const tf_stencil* stencil, const float* inv, float w, const float* b,
+                          const float* x_in, float* x_out);
+
 /* ---- IO: io::readMatrixMarket (include/IO/MatrixMarket.cuh, src/IO/MatrixMarket.cu:35-122) ----
  * Host CSR of a Matrix Market file (rows in file order, duplicates kept,
  * symmetric storage mirrored).  Call with NULL arrays to get rows, cols and

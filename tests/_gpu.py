@@ -180,3 +180,51 @@ class Sell:
             self.close()
         except Exception:
             pass
+
+
+class Stencil:
+    """A structured-stencil handle (tf_stencil_*) built from a host CSR matrix."""
+
+    def __init__(self, a):
+        self.n = len(a[0]) - 1
+        self.rp, self.ci, self.v = up(a[0], np.int32), up(a[1], np.int32), up(a[2], np.float64)
+        self.h = C.c_void_p()
+        check(lib().tf_stencil_create(ctx().h, self.n, len(a[1]), self.rp.ptr, self.ci.ptr, self.v.ptr,
+                                      C.byref(self.h)))
+
+    def info(self):
+        terms, arrays, mirrored = C.c_int(), C.c_int(), C.c_int()
+        check(lib().tf_stencil_info(self.h, C.byref(terms), C.byref(arrays), C.byref(mirrored)))
+        return terms.value, arrays.value, bool(mirrored.value)
+
+    def spmv(self, x, dtype=np.float64):
+        dx, y = up(x, dtype), DeviceArray(self.n, dtype)
+        fn = lib().tf_stencil_spmv_f64 if dtype == np.float64 else lib().tf_stencil_spmv_f32
+        check(fn(ctx().h, self.h, dx.ptr, y.ptr))
+        ctx().sync()
+        return y.get()
+
+    def residual(self, b, x, dtype=np.float64):
+        db, dx, r = up(b, dtype), up(x, dtype), DeviceArray(self.n, dtype)
+        fn = lib().tf_stencil_residual_f64 if dtype == np.float64 else lib().tf_stencil_residual_f32
+        check(fn(ctx().h, self.h, db.ptr, dx.ptr, r.ptr))
+        ctx().sync()
+        return r.get()
+
+    def jacobi(self, inv, w, b, x, dtype=np.float64):
+        di, db, dx, out = up(inv, dtype), up(b, dtype), up(x, dtype), DeviceArray(self.n, dtype)
+        fn = lib().tf_stencil_jacobi_f64 if dtype == np.float64 else lib().tf_stencil_jacobi_f32
+        check(fn(ctx().h, self.h, di.ptr, w, db.ptr, dx.ptr, out.ptr))
+        ctx().sync()
+        return out.get()
+
+    def close(self):
+        if self.h:
+            check(lib().tf_stencil_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -51,6 +51,15 @@ _SIGS = {
     "tf_sell_residual_f32": (i32, [vp, vp, vp, vp, vp]),
     "tf_sell_jacobi_f64": (i32, [vp, vp, vp, f64, vp, vp, vp]),
     "tf_sell_jacobi_f32": (i32, [vp, vp, vp, f32, vp, vp, vp]),
+    "tf_stencil_create": (i32, [vp, i32, i64, vp, vp, vp, C.POINTER(vp)]),
+    "tf_stencil_destroy": (i32, [vp]),
+    "tf_stencil_info": (i32, [vp, I32P, I32P, I32P]),
+    "tf_stencil_spmv_f64": (i32, [vp, vp, vp, vp]),
+    "tf_stencil_spmv_f32": (i32, [vp, vp, vp, vp]),
+    "tf_stencil_residual_f64": (i32, [vp, vp, vp, vp, vp]),
+    "tf_stencil_residual_f32": (i32, [vp, vp, vp, vp, vp]),
+    "tf_stencil_jacobi_f64": (i32, [vp, vp, vp, f64, vp, vp, vp]),
+    "tf_stencil_jacobi_f32": (i32, [vp, vp, vp, f32, vp, vp, vp]),
     "tf_read_matrix_market": (i32, [C.c_char_p, I32P, I32P, I64P, I32P, I32P, F64P]),
     "tf_get_device": (i32, [I32P]),
     "tf_set_device": (i32, [i32]),

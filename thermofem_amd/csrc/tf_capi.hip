@@ -208,6 +208,72 @@ TF_SELL_ABI(double, f64, v64)
 TF_SELL_ABI(float, f32, v32)
 #undef TF_SELL_ABI
 
+struct tf_stencil {
+    tf_ctx* ctx = nullptr;
+    tf::DevStencil s;
+    double* v64 = nullptr;
+    float* v32 = nullptr;
+};
+
+int tf_stencil_create(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int* ci, const double* v,
+                      tf_stencil** stencil) {
+    return guard([&] {
+        require(ctx != nullptr, "null context");
+        DeviceScope scope(ctx->device);
+        require(stencil != nullptr && n >= 1 && nnz >= 0 && nnz <= INT32_MAX, "invalid stencil arguments");
+        *stencil = nullptr;
+        require(csr_check(*ctx, n, nnz, rp, ci, v) == 0, "stencil storage needs a canonical CSR matrix");
+        auto h = std::make_unique<tf_stencil>();
+        h->ctx = ctx;
+        require(stencil_build(*ctx, n, nnz, rp, ci, v, h->s, &h->v64),
+                "matrix not representable as a stencil (more than 16 distinct column offsets)");
+        const size_t count = size_t(h->s.arrays) * size_t(n);
+        h->v32 = dev_alloc<float>(*ctx, count);
+        convert<double, float>(*ctx, count, h->v64, h->v32);
+        ctx->sync();
+        *stencil = h.release();
+    });
+}
+
+int tf_stencil_destroy(tf_stencil* h) {
+    return guard([&] {
+        if (!h) return;
+        DeviceScope scope(h->ctx->device);
+        h->ctx->sync();
+        free_stencil(*h->ctx, h->s);
+        dev_free(*h->ctx, h->v64);
+        dev_free(*h->ctx, h->v32);
+        delete h;
+    });
+}
+
+int tf_stencil_info(const tf_stencil* h, int* terms, int* arrays, int* mirrored) {
+    return guard([&] {
+        require(h != nullptr, "null handle");
+        if (terms) *terms = h->s.terms;
+        if (arrays) *arrays = h->s.arrays;
+        if (mirrored) *mirrored = h->s.mirrored ? 1 : 0;
+    });
+}
+
+#define TF_STENCIL_ABI(T, SUF, VAL)                                                                                  \
+    int tf_stencil_spmv_##SUF(tf_ctx* ctx, const tf_stencil* h, const T* x, T* y) {                                    \
+        TF_CTX_GUARD(ctx, require(h != nullptr && x != y, "SpMV shape or alias mismatch");                              \
+                     stencil_spmv<T>(*ctx, h->s, h->VAL, x, y));                                                        \
+    }                                                                                                                   \
+    int tf_stencil_residual_##SUF(tf_ctx* ctx, const tf_stencil* h, const T* b, const T* x, T* r) {                    \
+        TF_CTX_GUARD(ctx, require(h != nullptr && r != b && r != x, "Residual shape or alias mismatch");                \
+                     stencil_residual<T>(*ctx, h->s, h->VAL, b, x, r));                                                 \
+    }                                                                                                                   \
+    int tf_stencil_jacobi_##SUF(tf_ctx* ctx, const tf_stencil* h, const T* inv, T w, const T* b, const T* x_in,        \
+                                T* x_out) {                                                                             \
+        TF_CTX_GUARD(ctx, require(h != nullptr && x_in != x_out, "fused Jacobi needs distinct input and output iterates"); \
+                     stencil_jacobi<T>(*ctx, h->s, h->VAL, inv, w, b, x_in, x_out));                                    \
+    }
+TF_STENCIL_ABI(double, f64, v64)
+TF_STENCIL_ABI(float, f32, v32)
+#undef TF_STENCIL_ABI
+
 int tf_read_matrix_market(const char* path, int* rows, int* cols, int64_t* nnz, int* host_rp, int* host_ci,
                           double* host_v) {
     return guard([&] {

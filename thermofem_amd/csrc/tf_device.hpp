@@ -182,6 +182,51 @@ struct SellView {
     const int* perm;           // row at each position (nullptr: identity)
 };
 
+// Read-only view of a structured-stencil operator (tf_stencil.hip): every
+// row's columns are row + off[t] for terms t of one ascending offset list;
+// mask[row] bit t says whether the row has term t, and term t's value of
+// that row is vals[src[t] + row + shift[t]] (shift is 0, or -d for a term
+// -d whose values are read from the +d array of a symmetric operator).
+// Terms past the list's end repeat its last term (their mask bits are 0).
+constexpr int kStencilTerms = 16;
+struct StencilView {
+    int n;
+    const unsigned short* mask;
+    int off[kStencilTerms];
+    int shift[kStencilTerms];
+    int src[kStencilTerms];
+};
+
+// Sum of row `row` of a stencil operator for the calling lane: NC chunks of
+// 8 terms, each issuing all its value loads and gathers before the adds
+// (row indices clamped into range for terms the row lacks, which are not
+// added).  Products are rounded and added one by one in ascending column
+// order, the CSR order.
+template <class T, int NC, class XF>
+__device__ __forceinline__ T stencil_row_sum(const StencilView& m, const T* __restrict__ vals, int row,
+                                             XF&& xval) {
+#pragma clang fp contract(off)
+    const unsigned mask = m.mask[row];
+    T acc = 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        T pv[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int t = c * 8 + u;
+            pv[u] = vals[m.src[t] + min(max(row + m.shift[t], 0), m.n - 1)];
+            xv[u] = xval(min(max(row + m.off[t], 0), m.n - 1));
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const T p = pv[u] * xv[u];
+            const T s = acc + p;
+            acc = (mask >> (c * 8 + u)) & 1u ? s : acc;
+        }
+    }
+    return acc;
+}
+
 __device__ __forceinline__ int sell_col(short c, int row) { return row + int(c); }
 __device__ __forceinline__ int sell_col(int c, int) { return c; }
 

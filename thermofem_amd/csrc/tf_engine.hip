@@ -37,6 +37,7 @@ constexpr int kFusedProlongRows = 65536;
 // one lane per row, 16-bit column offsets, no LDS staging (tf_sell.hip).
 constexpr double kSellMaxPadding = 0.3;
 constexpr int kSellMinRows = 256;
+constexpr int kStencilMinRows = 4096;  // below this the probe and fill cost more than they save
 // Levels with at least this many nonzeros are tuned: sliced ELL and the CSR
 // kernel are both timed on the level's own operator and the faster one is
 // kept (sliced ELL wins on stencil-like levels, not always on coarsened ones).
@@ -206,6 +207,22 @@ __device__ __forceinline__ void for_sell_rows(const SellView<C>& m, const double
     }
 }
 
+// The same over a structured-stencil operator: one lane per row, blocks of
+// 256 consecutive rows, XCD-contiguous.
+template <int NC, class XF, class F>
+__device__ __forceinline__ void for_stencil_rows(const StencilView& m, const double* __restrict__ v, XF&& xval,
+                                                 F&& f) {
+    const int nblocks = (m.n + kBlock - 1) / kBlock;
+    const int per = int(gridDim.x) / kXcds;
+    const int xcd = int(blockIdx.x % kXcds), idx = int(blockIdx.x / kXcds);
+    const int chunk = (nblocks + kXcds - 1) / kXcds;
+    const int end = min(nblocks, (xcd + 1) * chunk);
+    for (int lb = xcd * chunk + idx; lb < end; lb += per) {
+        const int row = lb * kBlock + int(threadIdx.x);
+        if (row < m.n) f(row, stencil_row_sum<double, NC>(m, v, row, xval));
+    }
+}
+
 // Row sources of the fused kernels below.
 struct CsrRows {
     int n, nblocks, rpb;
@@ -233,6 +250,20 @@ struct SellRows {
     template <class XF, class F>
     __device__ __forceinline__ void each_fn(XF&& xval, F&& f) const {
         for_sell_rows(m, v, xval, f);
+    }
+};
+
+template <int NC>
+struct StencilRows {
+    StencilView m;
+    const double* v;
+    template <class F>
+    __device__ __forceinline__ void each(const double* x, F&& f) const {
+        for_stencil_rows<NC>(m, v, [&](int c) { return x[c]; }, f);
+    }
+    template <class XF, class F>
+    __device__ __forceinline__ void each_fn(XF&& xval, F&& f) const {
+        for_stencil_rows<NC>(m, v, xval, f);
     }
 };
 
@@ -579,6 +610,9 @@ void free_level(tf_ctx& c, AmgLevel& l, bool f32) {
     free_sell(c, l.sell);
     dev_free(c, l.svals);
     l.svals = nullptr;
+    free_stencil(c, l.st);
+    dev_free(c, l.stvals);
+    l.stvals = nullptr;
     for (void** p : {&l.inv, &l.b, &l.x, &l.x2, &l.r}) {
         dev_free(c, *p);
         *p = nullptr;
@@ -850,6 +884,23 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
         for (size_t d = 0; d < next.size(); ++d) {
             AmgLevel& L = next[d];
             if (L.native.n < kSellMinRows || (factored && d + 1 == next.size())) continue;
+            // The finest operator of a structured grid sweeps in the stencil
+            // storage (coarsened levels never have one offset list).
+            double* st64 = nullptr;
+            if (d == 0 && L.native.n >= kStencilMinRows &&
+                stencil_build(*c_, L.native.n, L.native.nnz, L.native.rp, L.native.ci, L.native.v, L.st, &st64)) {
+                if (f32) {
+                    const size_t count = size_t(L.st.arrays) * size_t(L.native.n);
+                    L.stvals = dev_alloc<float>(*c_, count);
+                    convert<double, float>(*c_, count, st64, static_cast<float*>(L.stvals));
+                    c_->sync();
+                    dev_free(*c_, st64);
+                }
+                else {
+                    L.stvals = st64;
+                }
+                continue;
+            }
             if (!sell_build(*c_, L.native.n, L.native.nnz, L.native.rp, L.native.ci, L.sell, kSellMaxPadding))
                 continue;
             bool keep;
@@ -1225,7 +1276,11 @@ void AmgEngine::smooth(size_t d, int sweeps, int first) {
             l.zero = false;
         }
         else {
-            if (l.sell)
+            if (l.st)
+                stencil_jacobi<L>(*c_, l.st, static_cast<const L*>(l.stvals), static_cast<const L*>(l.inv),
+                                  L(opt_.omega), static_cast<const L*>(l.b), static_cast<const L*>(l.x),
+                                  static_cast<L*>(l.x2));
+            else if (l.sell)
                 sell_jacobi<L>(*c_, l.sell, static_cast<const L*>(l.svals), static_cast<const L*>(l.inv),
                                L(opt_.omega), static_cast<const L*>(l.b), static_cast<const L*>(l.x),
                                static_cast<L*>(l.x2));
@@ -1259,7 +1314,10 @@ void AmgEngine::restrict_to(size_t d, bool ofResidual, bool fuseFirstSweep) {
                              static_cast<L*>(nx.b), inv, w, xc);
     }
     else {
-        if (l.sell)
+        if (l.st)
+            stencil_residual<L>(*c_, l.st, static_cast<const L*>(l.stvals), static_cast<const L*>(l.b),
+                                static_cast<const L*>(l.x), static_cast<L*>(l.r));
+        else if (l.sell)
             sell_residual<L>(*c_, l.sell, static_cast<const L*>(l.svals), static_cast<const L*>(l.b),
                              static_cast<const L*>(l.x), static_cast<L*>(l.r));
         else
@@ -1405,6 +1463,8 @@ SolverEngine::~SolverEngine() {
     free_csr(*c_, A_);
     free_sell(*c_, As_);
     dev_free(*c_, Asv_);
+    free_stencil(*c_, Ast_);
+    dev_free(*c_, Astv_);
     dev_free(*c_, vec_);
     if (ctl_) (void)hipFree(ctl_);
     if (hctl_) (void)hipHostFree(hctl_);
@@ -1424,6 +1484,9 @@ void SolverEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const
     free_sell(*c_, As_);
     dev_free(*c_, Asv_);
     Asv_ = nullptr;
+    free_stencil(*c_, Ast_);
+    dev_free(*c_, Astv_);
+    Astv_ = nullptr;
     A_ = A;
     n_ = n;
     const int nv = kind_ == 0 ? 6 : 11;
@@ -1447,7 +1510,10 @@ void SolverEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const
         ph_ = take();
         shat_ = take();
     }
-    if (n >= kSellMinRows && sell_build(*c_, A_.n, A_.nnz, A_.rp, A_.ci, As_, kSellMaxPadding)) {
+    if (n >= kStencilMinRows && stencil_build(*c_, A_.n, A_.nnz, A_.rp, A_.ci, A_.v, Ast_, &Astv_)) {
+        // structured-grid operator: the iteration's products read the stencil arrays
+    }
+    else if (n >= kSellMinRows && sell_build(*c_, A_.n, A_.nnz, A_.rp, A_.ci, As_, kSellMaxPadding)) {
         Asv_ = dev_alloc<double>(*c_, size_t(As_.stored));
         sell_values<double, double>(*c_, As_, A_.rp, A_.v, Asv_);
         if (!sell_spmv_pays()) {
@@ -1497,7 +1563,14 @@ double SolverEngine::norm_host(const double* x) {
 
 // One iteration of the solver as a fixed launch sequence.
 void SolverEngine::launch_part() {
-    if (!As_) {
+    if (Ast_) {
+        const int nb = (Ast_.n + kBlock - 1) / kBlock;
+        if (Ast_.chunks() == 1)
+            launch_iteration(StencilRows<1>{Ast_.view(), Astv_}, nb);
+        else
+            launch_iteration(StencilRows<2>{Ast_.view(), Astv_}, nb);
+    }
+    else if (!As_) {
         const int rpb = stream_rows_per_block<double>(A_.n, A_.nnz);
         const int nb = (A_.n + rpb - 1) / rpb;
         launch_iteration(CsrRows{A_.n, nb, rpb, A_.rp, A_.ci, A_.v}, nb);
@@ -1570,7 +1643,9 @@ void SolverEngine::launch_iteration(const R& rows, int nb) {
 }
 
 void SolverEngine::true_residual() {
-    if (As_)
+    if (Ast_)
+        stencil_residual<double>(*c_, Ast_, Astv_, rhs_, x_, r_);
+    else if (As_)
         sell_residual<double>(*c_, As_, Asv_, rhs_, x_, r_);
     else
         residual<double>(*c_, A_.n, A_.nnz, A_.rp, A_.ci, A_.v, rhs_, x_, r_);

@@ -31,6 +31,8 @@ struct AmgLevel {
     DevAgg transfer;        // to the next coarser level (empty on the coarsest)
     DevSell sell;           // sliced-ELL copy for the sweeps and residuals (empty: CSR kernels)
     void* svals = nullptr;  // its values, level precision
+    DevStencil st;          // structured-stencil copy (level 0 of a stencil operator; replaces sell)
+    void* stvals = nullptr; // its value arrays, level precision
     bool zero = false;      // iterate is known to be zero (cleared lazily)
     bool preapplied = false; // x already holds the first sweep from zero (fused upstream)
     int64_t hist[4] = {0, 0, 0, 0};
@@ -313,6 +315,8 @@ private:
     DevCsr A_;
     DevSell As_;             // sliced-ELL copy of A_ for the iteration's products (empty: CSR)
     double* Asv_ = nullptr;
+    DevStencil Ast_;         // structured-stencil copy of A_ (used instead of As_ when built)
+    double* Astv_ = nullptr;
     int n_ = 0;
     double* vec_ = nullptr;  // all work vectors, contiguous
     double *rhs_ = nullptr, *x_ = nullptr, *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *q_ = nullptr;
