@@ -433,10 +433,11 @@ __global__ __launch_bounds__(kBlock) void k_max_final(int nparts, const int* __r
 // or max for the graph: the order of the stable sort and reduce-by-key of
 // GpuAggregation.cu:178-221, which the host oracle fixes).  A run's rank
 // among the row's runs is its place in the sorted row.  Results go to
-// staging at gstart[I]; ucnt[I] = distinct columns.  Rows of up to 32 raw
-// entries are merged by 32-lane groups; longer rows (up to kMergeCap) are
-// queued for a wave each; longer still raise the overflow flag and the whole
-// product takes the global sort path.
+// staging at gstart[I]; ucnt[I] = distinct columns.  Rows of up to 16 or 32
+// raw entries are merged by groups of as many lanes, one key a lane, sorted
+// in registers; longer rows (up to kMergeCap) are queued for a wave each,
+// whose 64..512 keys are sorted in registers too (1..8 a lane); longer still
+// raise the overflow flag and the whole product takes the global sort path.
 constexpr int kNoCol = 0x7fffffff;  // dropped entries and padding sort last
 constexpr int kMergeSmall = 32;
 constexpr int kMergeCap = 512;
@@ -517,23 +518,56 @@ __device__ int merge_gather(int I, int mb, int me, const int* __restrict__ offse
     return m;
 }
 
-// Ascending bitonic sort of K[0, W) (W a power of two) by the group's lanes.
-template <int G>
-__device__ void merge_sort(u64* K, int W, int lane) {
+__device__ __forceinline__ u64 shfl_xor_u64(u64 v, int j) {
+    const int lo = __shfl_xor(int(unsigned(v)), j, kWave), hi = __shfl_xor(int(unsigned(v >> 32)), j, kWave);
+    return (u64(unsigned(hi)) << 32) | unsigned(lo);
+}
+
+// Ascending bitonic sort of W = 64 R keys held by a wave in registers (key r
+// of lane l is element 64 r + l): partners less than 64 apart are exchanged
+// by shuffles, the others between the lane's own registers.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(u64 (&key)[R], int lane) {
+    constexpr int W = R * kWave;
+#pragma unroll
     for (int k = 2; k <= W; k <<= 1)
+#pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = lane; i < W; i += G) {
-                const int p = i ^ j;
-                if (p > i) {
-                    const u64 a = K[i], b = K[p];
-                    if ((a > b) == ((i & k) == 0)) {
-                        K[i] = b;
-                        K[p] = a;
+            if (j >= kWave) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int p = r ^ (j / kWave);
+                    if (p > r) {
+                        const bool up = ((r * kWave + lane) & k) == 0;
+                        const u64 a = key[r], b = key[p];
+                        if ((a > b) == up) {
+                            key[r] = b;
+                            key[p] = a;
+                        }
                     }
                 }
             }
-            wave_lds_sync();
+            else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const u64 other = shfl_xor_u64(key[r], j);
+                    const bool low = (lane & j) == 0, up = ((r * kWave + lane) & k) == 0;
+                    key[r] = (low == up) ? (other < key[r] ? other : key[r]) : (other > key[r] ? other : key[r]);
+                }
+            }
         }
+}
+
+// Sorts K[0, W) (W = 64 R) of one wave through registers.
+template <int R>
+__device__ __forceinline__ void wave_sort_lds(u64* K, int lane) {
+    u64 key[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) key[r] = K[r * kWave + lane];
+    wave_bitonic<R>(key, lane);
+#pragma unroll
+    for (int r = 0; r < R; ++r) K[r * kWave + lane] = key[r];
+    wave_lds_sync();
 }
 
 // Reduces the runs of equal columns of the sorted K[0, W) (W >= G): lane l
@@ -575,9 +609,9 @@ __device__ int merge_runs(const u64* K, const double* V, int W, int lane, int ba
     return __shfl(incl, G - 1, G);
 }
 
-// Rows of up to kMergeSmall raw entries, 32 lanes each; longer rows are
+// Rows of up to G (16 or 32) raw entries, G lanes each; longer rows are
 // appended to the worklist wl (count in *wlCount) for k_merge_large.
-template <bool Max>
+template <bool Max, int G>
 __global__ __launch_bounds__(kBlock) void k_merge_small(int nc, const int* __restrict__ offsets,
                                                         const int* __restrict__ members, const int* __restrict__ map,
                                                         const int* __restrict__ rp, const int* __restrict__ ci,
@@ -586,9 +620,9 @@ __global__ __launch_bounds__(kBlock) void k_merge_small(int nc, const int* __res
                                                         int* __restrict__ scol, double* __restrict__ sval,
                                                         int* __restrict__ ucnt, int* __restrict__ wl, int* wlCount,
                                                         int* bad) {
-    constexpr int G = kMergeSmall, NG = kBlock / G;
-    __shared__ u64 lk[NG][kMergeSmall];
-    __shared__ double lv[NG][kMergeSmall];
+    constexpr int NG = kBlock / G;
+    __shared__ u64 lk[NG][G];
+    __shared__ double lv[NG][G];
     __shared__ int mstart[NG][G], moff[NG][G];
     const int g = int(threadIdx.x) / G, lane = int(threadIdx.x) % G;
     const int I = int(blockIdx.x) * NG + g;
@@ -598,7 +632,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_small(int nc, const int* __res
     int mb, me;
     member_range(I, offsets, mb, me);
     const int r = raw[I], base = gstart[I];
-    if (r > kMergeSmall) {
+    if (r > G) {
         if (lane == 0) wl[atomicAdd(wlCount, 1)] = I;
         return;
     }
@@ -616,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_small(int nc, const int* __res
         }
     lk[g][lane] = key;
     wave_lds_sync();
-    const int u = merge_runs<G, Max>(lk[g], lv[g], kMergeSmall, lane, base, scol, sval, bad);
+    const int u = merge_runs<G, Max>(lk[g], lv[g], G, lane, base, scol, sval, bad);
     if (lane == 0) ucnt[I] = u;
 }
 
@@ -656,7 +690,12 @@ __global__ __launch_bounds__(kBlock) void k_merge_large(int nc, const int* __res
                                       moff[g], lane);
         for (int e = m + lane; e < W; e += G) lk[g][e] = pack_key(kNoCol, e);
         wave_lds_sync();
-        merge_sort<G>(lk[g], W, lane);
+        switch (W / G) {
+            case 1: wave_sort_lds<1>(lk[g], lane); break;
+            case 2: wave_sort_lds<2>(lk[g], lane); break;
+            case 4: wave_sort_lds<4>(lk[g], lane); break;
+            default: wave_sort_lds<8>(lk[g], lane); break;
+        }
         const int u = merge_runs<G, Max>(lk[g], lv[g], W, lane, gstart[I], scol, sval, bad);
         if (lane == 0) ucnt[I] = u;
         wave_lds_sync();
@@ -1153,16 +1192,22 @@ DevCsr gather_merge(tf_ctx& c, int nc, const int* offsets, const int* members, c
     // flags: 0 non-finite merge, 1 worklist length, 2 overflow, 3 output nnz
     c.clear_flags(3);
     {
-        // Rows short on average (<= 28 entries): 32-lane groups, the few long
-        // rows queued for waves; otherwise a wave per row throughout.
+        // Rows short on average: groups of 16 lanes (<= 12 entries on
+        // average) or 32 (<= 28), the few long rows queued for waves;
+        // otherwise a wave per row throughout.
         const bool small = nnzIn <= int64_t(28) * nc;
-        const int ng = kBlock / kMergeSmall;
+        const bool tiny = nnzIn <= int64_t(12) * nc;
+        const int ng = kBlock / (tiny ? 16 : kMergeSmall);
         const dim3 grid((nc + ng - 1) / ng), large(std::max(1, std::min((nc + 3) / 4, 4096)));
         int* list = small ? wl : nullptr;
 #define TF_MERGE(MX)                                                                                           \
-    if (small) {                                                                                               \
-        hipLaunchKernelGGL(k_merge_small<MX>, grid, dim3(kBlock), 0, c.stream, nc, offsets, members, map, rp, ci, \
-                           v, int(graph), raw, start, scol, sval, ucnt, wl, c.dflags + 1, c.dflags);             \
+    if (tiny) {                                                                                                \
+        hipLaunchKernelGGL((k_merge_small<MX, 16>), grid, dim3(kBlock), 0, c.stream, nc, offsets, members, map, rp, \
+                           ci, v, int(graph), raw, start, scol, sval, ucnt, wl, c.dflags + 1, c.dflags);         \
+    }                                                                                                          \
+    else if (small) {                                                                                          \
+        hipLaunchKernelGGL((k_merge_small<MX, kMergeSmall>), grid, dim3(kBlock), 0, c.stream, nc, offsets, members, \
+                           map, rp, ci, v, int(graph), raw, start, scol, sval, ucnt, wl, c.dflags + 1, c.dflags); \
     }                                                                                                          \
     hipLaunchKernelGGL(k_merge_large<MX>, large, dim3(kBlock), 0, c.stream, nc, offsets, members, map, rp, ci, v, \
                        int(graph), raw, start, scol, sval, ucnt, list, c.dflags + 1, c.dflags, c.dflags + 2)

@@ -227,7 +227,7 @@ int tf_stencil_create(tf_ctx* ctx, int n, int64_t nnz, const int* rp, const int*
         h->ctx = ctx;
         require(stencil_build(*ctx, n, nnz, rp, ci, v, h->s, &h->v64),
                 "matrix not representable as a stencil (more than 16 distinct column offsets)");
-        const size_t count = size_t(h->s.arrays) * size_t(n);
+        const size_t count = size_t(h->s.stored);
         h->v32 = dev_alloc<float>(*ctx, count);
         convert<double, float>(*ctx, count, h->v64, h->v32);
         ctx->sync();

@@ -184,18 +184,25 @@ struct SellView {
 
 // Read-only view of a structured-stencil operator (tf_stencil.hip): every
 // row's columns are row + off[t] for terms t of one ascending offset list;
-// mask[row] bit t says whether the row has term t, and term t's value of
-// that row is vals[src[t] + row + shift[t]] (shift is 0, or -d for a term
-// -d whose values are read from the +d array of a symmetric operator).
-// Terms past the list's end repeat its last term (their mask bits are 0).
+// mask[row] bit t says whether the row has term t.  The stored value arrays
+// are interleaved by tiles of 64 rows (one wave): value a of row r sits at
+// stencil_index(r, a, tile) with tile = 64 * arrays, so a wave's loads of
+// its rows' values are one contiguous run.  Term t of row `row` reads
+// array src[t] / 64 at row + shift[t] (shift is 0, or -d for a term -d whose
+// values are read from the +d array of a symmetric operator).  Terms past
+// the list's end repeat its last term (their mask bits are 0).
 constexpr int kStencilTerms = 16;
 struct StencilView {
     int n;
+    int tile;
     const unsigned short* mask;
     int off[kStencilTerms];
     int shift[kStencilTerms];
-    int src[kStencilTerms];
+    int src[kStencilTerms];  // 64 * array index
 };
+__host__ __device__ __forceinline__ int stencil_index(int r, int slot64, int tile) {
+    return (r >> 6) * tile + slot64 + (r & 63);
+}
 
 // Sum of row `row` of a stencil operator for the calling lane: NC chunks of
 // 8 terms, each issuing all its value loads and gathers before the adds
@@ -214,7 +221,7 @@ __device__ __forceinline__ T stencil_row_sum(const StencilView& m, const T* __re
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int t = c * 8 + u;
-            pv[u] = vals[m.src[t] + min(max(row + m.shift[t], 0), m.n - 1)];
+            pv[u] = vals[stencil_index(min(max(row + m.shift[t], 0), m.n - 1), m.src[t], m.tile)];
             xv[u] = xval(min(max(row + m.off[t], 0), m.n - 1));
         }
 #pragma unroll

@@ -887,14 +887,23 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
             // The finest operator of a structured grid sweeps in the stencil
             // storage (coarsened levels never have one offset list).
             double* st64 = nullptr;
-            if (d == 0 && L.native.n >= kStencilMinRows &&
-                stencil_build(*c_, L.native.n, L.native.nnz, L.native.rp, L.native.ci, L.native.v, L.st, &st64)) {
-                if (f32) {
-                    const size_t count = size_t(L.st.arrays) * size_t(L.native.n);
+            const bool shared = d == 0 && fineStencil_ && fineStencil_->n == L.native.n;
+            if (shared) {
+                stencil_copy(*c_, *fineStencil_, L.st);
+                st64 = const_cast<double*>(fineStencil64_);
+            }
+            if (shared || (d == 0 && L.native.n >= kStencilMinRows &&
+                           stencil_build(*c_, L.native.n, L.native.nnz, L.native.rp, L.native.ci, L.native.v, L.st,
+                                         &st64))) {
+                if (shared && !f32) {
+                    L.stvals = dev_alloc<double>(*c_, size_t(L.st.stored));
+                    convert<double, double>(*c_, size_t(L.st.stored), st64, static_cast<double*>(L.stvals));
+                }
+                else if (f32) {
+                    const size_t count = size_t(L.st.stored);
                     L.stvals = dev_alloc<float>(*c_, count);
                     convert<double, float>(*c_, count, st64, static_cast<float*>(L.stvals));
-                    c_->sync();
-                    dev_free(*c_, st64);
+                    if (!shared) dev_free(*c_, st64);  // (stream-ordered reuse)
                 }
                 else {
                     L.stvals = st64;
@@ -1523,7 +1532,17 @@ void SolverEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const
             Asv_ = nullptr;
         }
     }
-    if (amg_) amg_->setup(A_.n, A_.nnz, A_.rp, A_.ci, A_.v, true);
+    if (amg_) {
+        amg_->share_fine_stencil(Ast_ ? &Ast_ : nullptr, Astv_);
+        try {
+            amg_->setup(A_.n, A_.nnz, A_.rp, A_.ci, A_.v, true);
+        }
+        catch (...) {
+            amg_->share_fine_stencil(nullptr, nullptr);
+            throw;
+        }
+        amg_->share_fine_stencil(nullptr, nullptr);
+    }
     c_->sync();
 }
 

@@ -142,6 +142,12 @@ public:
     // Device arrays; canonicalInput skips the canonical-form check/copy (the
     // caller guarantees sorted unique columns and finite values).
     void setup(int n, int64_t nnz, const int* rp, const int* ci, const double* v, bool canonicalInput = false);
+    // The next setups' finest level takes its stencil storage from this one
+    // (same matrix, FP64 values) instead of building its own; nullptr clears.
+    void share_fine_stencil(const DevStencil* s, const double* vals64) {
+        fineStencil_ = s;
+        fineStencil64_ = vals64;
+    }
     // z = M^-1 r (FP64 device vectors; may alias).  Enqueued on the context stream.
     void apply(const double* rhs, double* x);
     // apply(rhs, x) whose last step (the level-0 iterate written out as
@@ -174,6 +180,8 @@ public:
     AmgOptionsC opt_;
 
 private:
+    const DevStencil* fineStencil_ = nullptr;
+    const double* fineStencil64_ = nullptr;
     template <class L>
     void cycle(int kind, size_t d, bool zero);
     template <class L>

@@ -326,15 +326,17 @@ void sell_jacobi(tf_ctx& c, const DevSell& s, const T* v, const T* inv, T w, con
 
 // Structured-stencil copy of a CSR operator (tf_stencil.hip): a 16-bit term
 // mask per row and one dense value array per stored offset, no column
-// indices; `arrays` value arrays of n elements each, laid out one after the
-// other, live in separate buffers of the element type the kernels run in.
+// indices; the `arrays` value arrays, interleaved by 64-row tiles (`stored`
+// elements in all), live in separate buffers of the element type the
+// kernels run in.
 struct DevStencil {
     int n = 0, terms = 0, arrays = 0;
+    int64_t stored = 0;
     bool mirrored = false;  // symmetric: the negative offsets read the positive ones' arrays
     unsigned short* mask = nullptr;
     int off[kStencilTerms] = {}, shift[kStencilTerms] = {}, src[kStencilTerms] = {};
     StencilView view() const {
-        StencilView v{n, mask, {}, {}, {}};
+        StencilView v{n, 64 * arrays, mask, {}, {}, {}};
         for (int t = 0; t < kStencilTerms; ++t) {
             v.off[t] = off[t];
             v.shift[t] = shift[t];
@@ -347,11 +349,13 @@ struct DevStencil {
 };
 // Builds the storage when every row's columns come from one list of at most
 // kStencilTerms offsets (found on a sample of rows, then checked on every
-// entry) and fills its FP64 values (*vals64, arrays * n doubles); false,
+// entry) and fills its FP64 values (*vals64, s.stored doubles); false,
 // with nothing kept, otherwise.
 bool stencil_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, DevStencil& s,
                    double** vals64);
 void free_stencil(tf_ctx& c, DevStencil& s);
+// to = from with a mask array of its own (the values are the caller's).
+void stencil_copy(tf_ctx& c, const DevStencil& from, DevStencil& to);
 template <class T>
 void stencil_spmv(tf_ctx& c, const DevStencil& s, const T* v, const T* x, T* y);
 template <class T>

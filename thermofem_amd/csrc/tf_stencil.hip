@@ -3,9 +3,9 @@
 // structured-grid matrices (the 7-point heat operator on an nx*ny*nz grid
 // has the offsets -nx*ny, -nx, -1, 0, 1, nx, nx*ny).  No column indices are
 // stored.  Each row keeps a 16-bit mask of the terms it has, and the values
-// sit in dense arrays, one per stored offset (the DIA layout), so the lanes
-// of consecutive rows load consecutive elements and gather x at fixed
-// offsets.  When the operator is symmetric bit for bit, the arrays of the
+// sit in dense arrays, one per stored offset (the DIA layout), interleaved by
+// tiles of 64 rows: a wave reads its rows' values as one contiguous run and
+// gathers x at fixed offsets.  When the operator is symmetric bit for bit, the arrays of the
 // negative offsets are not stored: a(i, i-d) is read as a(i-d, i) from the
 // +d array, which holds the off-diagonal bytes to half.  For the 128^3 heat
 // operator the FP64 product then moves 4 doubles and 2 mask bytes per row
@@ -15,7 +15,7 @@
 //
 // The offset list comes from a sample of rows (512 around the middle, 512
 // spread over the matrix); the fill pass then checks every entry against it
-// and a second pass checks the mirrored values bit for bit, so a matrix that
+// and every mirrored value against its transpose bit for bit, so a matrix that
 // is not a stencil, or not symmetric, is never stored wrongly: it keeps its
 // sliced-ELL or CSR copy, or its full (non-mirrored) stencil arrays.
 #include <algorithm>
@@ -77,10 +77,10 @@ __global__ __launch_bounds__(kProbeThreads) void k_stencil_probe(int n, const in
 }
 
 // The term list as the build kernels use it: term t has offset off[t];
-// own[t] >= 0 is the start of the array that stores it, -1 when its values
-// are read mirrored from the +|off| array mir[t].
+// own[t] >= 0 is 64 * the index of the array that stores it, -1 when its
+// values are read mirrored from the +|off| array at mir[t].
 struct StencilTable {
-    int terms, arrays, n;
+    int terms, arrays, n, tile;
     int off[kStencilTerms];
     int own[kStencilTerms];
     int mir[kStencilTerms];
@@ -96,45 +96,43 @@ __device__ __forceinline__ int find_term(const StencilTable& tb, int off) {
 
 // Row masks and stored values; every stored array slot of the row is
 // written (0 for a term the row lacks).  flags[0] = an entry whose offset
-// is not in the list.
+// is not in the list.  For a mirrored table, every entry of a negative term
+// is compared (bit for bit) with its transpose, looked up in the row it
+// mirrors: flags[1] = an entry whose transpose differs or is missing.
 __global__ __launch_bounds__(kBlock) void k_stencil_fill(StencilTable tb, const int* __restrict__ rp,
                                                          const int* __restrict__ ci, const double* __restrict__ v,
                                                          unsigned short* __restrict__ mask,
                                                          double* __restrict__ vals, int* flags) {
     const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (i >= tb.n) return;
-    for (int a = 0; a < tb.arrays; ++a) vals[size_t(a) * tb.n + i] = 0.0;
+    for (int a = 0; a < tb.arrays; ++a) vals[stencil_index(i, a * 64, tb.tile)] = 0.0;
     unsigned m = 0;
-    bool unknown = false;
+    bool unknown = false, differs = false;
     for (int j = rp[i]; j < rp[i + 1]; ++j) {
-        const int t = find_term(tb, ci[j] - i);
+        const int c = ci[j];
+        const int t = find_term(tb, c - i);
         if (t < 0) {
             unknown = true;
             continue;
         }
         m |= 1u << t;
-        if (tb.own[t] >= 0) vals[tb.own[t] + i] = v[j];
+        if (tb.own[t] >= 0) {
+            vals[stencil_index(i, tb.own[t], tb.tile)] = v[j];
+        }
+        else {
+            // row c < i holds the transpose (c, i) among its last entries
+            bool found = false;
+            for (int q = rp[c + 1] - 1; q >= rp[c] && ci[q] >= i; --q) {
+                if (ci[q] == i) {
+                    found = true;
+                    differs |= __double_as_longlong(v[q]) != __double_as_longlong(v[j]);
+                }
+            }
+            differs |= !found;
+        }
     }
     mask[i] = static_cast<unsigned short>(m);
     if (unknown) flags[0] = 1;
-}
-
-// flags[1] = an entry of a mirrored term whose value differs (bit for bit)
-// from its transpose's stored value.
-__global__ __launch_bounds__(kBlock) void k_stencil_mirror(StencilTable tb, const int* __restrict__ rp,
-                                                           const int* __restrict__ ci,
-                                                           const double* __restrict__ v,
-                                                           const double* __restrict__ vals, int* flags) {
-    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i >= tb.n) return;
-    bool differs = false;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
-        const int off = ci[j] - i;
-        if (off >= 0) break;  // ascending columns: the negative offsets come first
-        const int t = find_term(tb, off);
-        if (t < 0 || tb.own[t] >= 0) continue;
-        differs |= __double_as_longlong(v[j]) != __double_as_longlong(vals[tb.mir[t] + i + off]);
-    }
     if (differs) flags[1] = 1;
 }
 
@@ -161,9 +159,10 @@ StencilTable make_table(int n, int terms, const int* off, bool mirrored) {
         tb.off[t] = off[t];
         tb.own[t] = -1;
         tb.mir[t] = -1;
-        if (!mirrored || off[t] >= 0) tb.own[t] = (arrays++) * n;
+        if (!mirrored || off[t] >= 0) tb.own[t] = (arrays++) * 64;
     }
     tb.arrays = arrays;
+    tb.tile = arrays * 64;
     if (mirrored)
         for (int t = 0; t < terms; ++t)
             if (off[t] < 0)
@@ -172,17 +171,13 @@ StencilTable make_table(int n, int terms, const int* off, bool mirrored) {
     return tb;
 }
 
-// Runs the fill (and, mirrored, the transpose check) and reads the flags.
+// Runs the fill (with, mirrored, the transpose check) and reads the flags.
 void fill_stencil(tf_ctx& c, const StencilTable& tb, const int* rp, const int* ci, const double* v,
-                  unsigned short* mask, double* vals, int* flags, bool mirrored, int* hostFlags) {
+                  unsigned short* mask, double* vals, int* flags, int* hostFlags) {
     check(hipMemsetAsync(flags, 0, 2 * sizeof(int), c.stream), "stencil flag clear");
     const dim3 grid((tb.n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_stencil_fill, grid, dim3(kBlock), 0, c.stream, tb, rp, ci, v, mask, vals, flags);
     check(hipGetLastError(), "stencil fill launch");
-    if (mirrored) {
-        hipLaunchKernelGGL(k_stencil_mirror, grid, dim3(kBlock), 0, c.stream, tb, rp, ci, v, vals, flags);
-        check(hipGetLastError(), "stencil mirror launch");
-    }
     check(hipMemcpyAsync(c.hflags, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, c.stream), "stencil flags");
     c.sync();
     hostFlags[0] = c.hflags[0];
@@ -214,7 +209,8 @@ bool stencil_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, 
           "stencil probe download");
     c.sync();
     const int terms = c.hflags[0];
-    if (terms <= 0 || int64_t(terms) * n >= int64_t(INT_MAX)) return false;  // array starts are ints
+    const int64_t tiles = (int64_t(n) + 63) / 64;
+    if (terms <= 0 || tiles * 64 * terms >= int64_t(INT_MAX)) return false;  // element indices are ints
     int off[kStencilTerms];
     for (int t = 0; t < terms; ++t) off[t] = c.hflags[1 + t];
     bool mirrored = true;
@@ -226,13 +222,15 @@ bool stencil_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, 
     int hf[2] = {0, 0};
     for (int attempt = 0; attempt < 2; ++attempt) {
         const StencilTable tb = make_table(n, terms, off, mirrored);
-        vals = dev_alloc<double>(c, size_t(tb.arrays) * size_t(n));
-        fill_stencil(c, tb, rp, ci, v, mask, vals, flags, mirrored, hf);
+        const size_t stored = size_t(tiles) * size_t(tb.tile);
+        vals = dev_alloc<double>(c, stored);
+        fill_stencil(c, tb, rp, ci, v, mask, vals, flags, hf);
         if (hf[0]) break;  // an offset outside the list: not a stencil
         if (!mirrored || !hf[1]) {
             s.n = n;
             s.terms = terms;
             s.arrays = tb.arrays;
+            s.stored = int64_t(stored);
             s.mirrored = mirrored;
             s.mask = mask;
             for (int t = 0; t < kStencilTerms; ++t) {
@@ -251,6 +249,14 @@ bool stencil_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, 
     dev_free(c, vals);
     dev_free(c, mask);
     return false;
+}
+
+void stencil_copy(tf_ctx& c, const DevStencil& from, DevStencil& to) {
+    to = from;
+    to.mask = dev_alloc<unsigned short>(c, size_t(from.n));
+    check(hipMemcpyAsync(to.mask, from.mask, size_t(from.n) * sizeof(unsigned short), hipMemcpyDeviceToDevice,
+                         c.stream),
+          "stencil mask copy");
 }
 
 void free_stencil(tf_ctx& c, DevStencil& s) {
