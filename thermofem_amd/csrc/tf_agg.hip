@@ -518,27 +518,29 @@ __device__ int merge_gather(int I, int mb, int me, const int* __restrict__ offse
     return m;
 }
 
+template <int G>
 __device__ __forceinline__ u64 shfl_xor_u64(u64 v, int j) {
-    const int lo = __shfl_xor(int(unsigned(v)), j, kWave), hi = __shfl_xor(int(unsigned(v >> 32)), j, kWave);
+    const int lo = __shfl_xor(int(unsigned(v)), j, G), hi = __shfl_xor(int(unsigned(v >> 32)), j, G);
     return (u64(unsigned(hi)) << 32) | unsigned(lo);
 }
 
-// Ascending bitonic sort of W = 64 R keys held by a wave in registers (key r
-// of lane l is element 64 r + l): partners less than 64 apart are exchanged
-// by shuffles, the others between the lane's own registers.
-template <int R>
-__device__ __forceinline__ void wave_bitonic(u64 (&key)[R], int lane) {
-    constexpr int W = R * kWave;
+// Ascending bitonic sort of W = G R keys held by a group of G lanes in
+// registers (key r of lane l is element G r + l): partners less than G
+// apart are exchanged by shuffles, the others between the lane's own
+// registers.
+template <int G, int R>
+__device__ __forceinline__ void group_bitonic(u64 (&key)[R], int lane) {
+    constexpr int W = R * G;
 #pragma unroll
     for (int k = 2; k <= W; k <<= 1)
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= kWave) {
+            if (j >= G) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const int p = r ^ (j / kWave);
+                    const int p = r ^ (j / G);
                     if (p > r) {
-                        const bool up = ((r * kWave + lane) & k) == 0;
+                        const bool up = ((r * G + lane) & k) == 0;
                         const u64 a = key[r], b = key[p];
                         if ((a > b) == up) {
                             key[r] = b;
@@ -550,23 +552,23 @@ __device__ __forceinline__ void wave_bitonic(u64 (&key)[R], int lane) {
             else {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const u64 other = shfl_xor_u64(key[r], j);
-                    const bool low = (lane & j) == 0, up = ((r * kWave + lane) & k) == 0;
+                    const u64 other = shfl_xor_u64<G>(key[r], j);
+                    const bool low = (lane & j) == 0, up = ((r * G + lane) & k) == 0;
                     key[r] = (low == up) ? (other < key[r] ? other : key[r]) : (other > key[r] ? other : key[r]);
                 }
             }
         }
 }
 
-// Sorts K[0, W) (W = 64 R) of one wave through registers.
-template <int R>
-__device__ __forceinline__ void wave_sort_lds(u64* K, int lane) {
+// Sorts K[0, W) (W = G R) of one group through registers.
+template <int G, int R>
+__device__ __forceinline__ void group_sort_lds(u64* K, int lane) {
     u64 key[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) key[r] = K[r * kWave + lane];
-    wave_bitonic<R>(key, lane);
+    for (int r = 0; r < R; ++r) key[r] = K[r * G + lane];
+    group_bitonic<G, R>(key, lane);
 #pragma unroll
-    for (int r = 0; r < R; ++r) K[r * kWave + lane] = key[r];
+    for (int r = 0; r < R; ++r) K[r * G + lane] = key[r];
     wave_lds_sync();
 }
 
@@ -655,7 +657,9 @@ __global__ __launch_bounds__(kBlock) void k_merge_small(int nc, const int* __res
 }
 
 // The worklist's rows, a wave each, sorted at the next power of two >= 64
-// of their raw length; rows past kMergeCap raise *overflow.
+// of their raw length; rows past kMergeCap raise *overflow.  (Groups of 32
+// lanes with two keys a lane measured slower: the LDS of eight rows a block
+// halves the resident waves.)
 template <bool Max>
 __global__ __launch_bounds__(kBlock) void k_merge_large(int nc, const int* __restrict__ offsets,
                                                         const int* __restrict__ members, const int* __restrict__ map,
@@ -691,10 +695,10 @@ __global__ __launch_bounds__(kBlock) void k_merge_large(int nc, const int* __res
         for (int e = m + lane; e < W; e += G) lk[g][e] = pack_key(kNoCol, e);
         wave_lds_sync();
         switch (W / G) {
-            case 1: wave_sort_lds<1>(lk[g], lane); break;
-            case 2: wave_sort_lds<2>(lk[g], lane); break;
-            case 4: wave_sort_lds<4>(lk[g], lane); break;
-            default: wave_sort_lds<8>(lk[g], lane); break;
+            case 1: group_sort_lds<G, 1>(lk[g], lane); break;
+            case 2: group_sort_lds<G, 2>(lk[g], lane); break;
+            case 4: group_sort_lds<G, 4>(lk[g], lane); break;
+            default: group_sort_lds<G, 8>(lk[g], lane); break;
         }
         const int u = merge_runs<G, Max>(lk[g], lv[g], W, lane, gstart[I], scol, sval, bad);
         if (lane == 0) ucnt[I] = u;

@@ -366,7 +366,8 @@ __global__ __launch_bounds__(1024) void k_kfinal(const double* __restrict__ part
 // The preconditioner's output step fused with the dot product that follows
 // it: z = FP64(level-0 iterate) (or 0), sum(r .* z) per block with the
 // thread order and block reduction of k_kdot, so the partials are the ones
-// k_kdot would form from z.
+// k_kdot would form from z.  With z == nullptr only the dot is formed (the
+// consumer reads the iterate itself).
 template <class L>
 __global__ __launch_bounds__(kBlock) void k_out_dot(int n, const L* __restrict__ xl, int zero,
                                                     const double* __restrict__ r, double* __restrict__ z,
@@ -376,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_out_dot(int n, const L* __restrict__
 #pragma unroll 4
     for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
         const double zi = zero ? 0.0 : double(xl[i]);
-        z[i] = zi;
+        if (z) z[i] = zi;
         s += r[i] * zi;
     }
     s = block_sum(s);
@@ -448,7 +449,10 @@ __device__ __forceinline__ double fold_partials(const double* __restrict__ parts
 
 // rho = r.z (finaliser of the preconditioner's output step), then the CG
 // direction p = z (fresh) or z + beta p.
-__global__ __launch_bounds__(kBlock) void k_cg_dir(int n, const double* __restrict__ z, double* __restrict__ p,
+// (z in FP64, or the preconditioner's level-0 iterate in level precision
+// Z, widened exactly; zero: z = 0)
+template <class Z>
+__global__ __launch_bounds__(kBlock) void k_cg_dir(int n, const Z* __restrict__ z, int zero, double* __restrict__ p,
                                                    const double* __restrict__ partsIn, int nIn, KCtl* ctl) {
     __shared__ double sBeta;
     __shared__ int sRun, sFresh;
@@ -472,8 +476,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, const double* __restri
     const double beta = sBeta;
     const bool fresh = sFresh != 0;
 #pragma unroll 4
-    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock)
-        p[i] = fresh ? z[i] : z[i] + beta * p[i];
+    for (int i = int(blockIdx.x) * kBlock + int(threadIdx.x); i < n; i += int(gridDim.x) * kBlock) {
+        const double zi = zero ? 0.0 : double(z[i]);
+        p[i] = fresh ? zi : zi + beta * p[i];
+    }
 }
 
 // alpha = rho / p.q (finaliser of the product A p), then x += alpha p,
@@ -1406,7 +1412,7 @@ void AmgEngine::cycle(int kind, size_t d, bool zero) {
 
 template <class L>
 void AmgEngine::apply_typed(const double* rhs, double* x, const double* with, double* partials, int blocks,
-                            const KCtl* ctl) {
+                            const KCtl* ctl, Output* out) {
     AmgLevel& l0 = levels_[0];
     const int n = l0.native.n;
     const bool first = opt_.cycle != 2 && levels_.size() > 1 && opt_.pre >= 1;
@@ -1416,6 +1422,11 @@ void AmgEngine::apply_typed(const double* rhs, double* x, const double* with, do
     l0.preapplied = first;
     std::fill(visits_.begin(), visits_.end(), 0);
     cycle<L>(opt_.cycle, 0, true);
+    if (out) {
+        out->x = l0.x;
+        out->f32 = sizeof(L) == sizeof(float);
+        out->zero = l0.zero;
+    }
     if (with) {
         hipLaunchKernelGGL(k_out_dot<L>, dim3(blocks), dim3(kBlock), 0, c_->stream, n,
                            static_cast<const L*>(l0.x), l0.zero ? 1 : 0, with, x, partials, ctl);
@@ -1438,12 +1449,13 @@ void AmgEngine::apply(const double* rhs, double* x) {
 }
 
 void AmgEngine::apply_dot(const double* rhs, double* x, const double* with, double* partials, int blocks,
-                          const KCtl* ctl) {
+                          const KCtl* ctl, Output* out) {
     require(ready(), "AMG requires setup", TF_E_INVALID);
+    require(x || out, "AMG output needs a vector or a descriptor", TF_E_INVALID);
     if (opt_.precision == 32)
-        apply_typed<float>(rhs, x, with, partials, blocks, ctl);
+        apply_typed<float>(rhs, x, with, partials, blocks, ctl, out);
     else
-        apply_typed<double>(rhs, x, with, partials, blocks, ctl);
+        apply_typed<double>(rhs, x, with, partials, blocks, ctl, out);
 }
 
 // --------------------------------------------------------------- Krylov
@@ -1621,12 +1633,24 @@ void SolverEngine::launch_iteration(const R& rows, int nb) {
         // (producers keep to at most kFoldParts partials)
         const dim3 gf(std::min(int(grid.x), kFoldParts));
         const int nf = int(gf.x);
-        if (amg_)
-            amg_->apply_dot(r_, z_, r_, pslot(kSlotA), nv, ctl);
-        else
-            hipLaunchKernelGGL(k_out_dot<double>, vb, dim3(kBlock), 0, c_->stream, n_, r_, 0, r_, z_,
-                               pslot(kSlotA), ctl);
-        hipLaunchKernelGGL(k_cg_dir, vb, dim3(kBlock), 0, c_->stream, n_, z_, p_, pslot(kSlotA), nv, ctl);
+        // the direction reads z where the preconditioner leaves it (its
+        // level-0 iterate, or r itself without one): z is not written out
+        if (amg_) {
+            AmgEngine::Output z;
+            amg_->apply_dot(r_, nullptr, r_, pslot(kSlotA), nv, ctl, &z);
+            if (z.f32)
+                hipLaunchKernelGGL(k_cg_dir<float>, vb, dim3(kBlock), 0, c_->stream, n_,
+                                   static_cast<const float*>(z.x), int(z.zero), p_, pslot(kSlotA), nv, ctl);
+            else
+                hipLaunchKernelGGL(k_cg_dir<double>, vb, dim3(kBlock), 0, c_->stream, n_,
+                                   static_cast<const double*>(z.x), int(z.zero), p_, pslot(kSlotA), nv, ctl);
+        }
+        else {
+            hipLaunchKernelGGL(k_out_dot<double>, vb, dim3(kBlock), 0, c_->stream, n_, r_, 0, r_,
+                               static_cast<double*>(nullptr), pslot(kSlotA), ctl);
+            hipLaunchKernelGGL(k_cg_dir<double>, vb, dim3(kBlock), 0, c_->stream, n_, r_, 0, p_, pslot(kSlotA), nv,
+                               ctl);
+        }
         hipLaunchKernelGGL(k_kspmv_dot<R>, gf, dim3(kBlock), 0, c_->stream, rows, p_, q_, p_, pslot(kSlotB), ctl);
         hipLaunchKernelGGL(k_cg_step, vb, dim3(kBlock), 0, c_->stream, n_, p_, q_, x_, r_, pslot(kSlotB), nf,
                            pslot(kSlotC), ctl);

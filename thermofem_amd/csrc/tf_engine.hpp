@@ -153,8 +153,16 @@ public:
     // apply(rhs, x) whose last step (the level-0 iterate written out as
     // FP64 x) also forms sum(with .* x) as block partials partials[b],
     // b < blocks, in the Krylov reductions' layout; that step is skipped once
-    // the control block has stopped.
-    void apply_dot(const double* rhs, double* x, const double* with, double* partials, int blocks, const KCtl* ctl);
+    // the control block has stopped.  With x == nullptr nothing is written
+    // and *out describes where the result stays: the level-0 iterate in
+    // level precision (valid until the next apply), or zero.
+    struct Output {
+        const void* x = nullptr;
+        bool f32 = false;
+        bool zero = false;
+    };
+    void apply_dot(const double* rhs, double* x, const double* with, double* partials, int blocks, const KCtl* ctl,
+                   Output* out = nullptr);
     bool ready() const { return !levels_.empty(); }
     // Explicit inverse of the coarsest operator (nullptr: coarse Jacobi).
     const double* coarse_inverse() const { return factored_ ? inverse_ : nullptr; }
@@ -192,7 +200,7 @@ private:
     void coarse(size_t d);
     template <class L>
     void apply_typed(const double* rhs, double* x, const double* with = nullptr, double* partials = nullptr,
-                     int blocks = 0, const KCtl* ctl = nullptr);
+                     int blocks = 0, const KCtl* ctl = nullptr, Output* out = nullptr);
     template <class L>
     void run_tail(int kind, bool zero);
     // Builds the tail from the first level at or below `from` that fits its
