@@ -306,7 +306,7 @@ def test_validate_rowmax_closed_form_and_rejections():
             O.rowmax(rp, ci, w)
 
 
-def _cycle_f32(a, kind, b, coarse_f64=True, round_levels=True):
+def _cycle_f32(a, kind, b, coarse_f64=True, round_levels=True, coarse_rows=4):
     """The mixed-precision cycle restated in numpy: FP64 hierarchy (the pinned
     strength / aggregation / Galerkin), level matrices, inverse diagonals and
     vectors in float32, x += (w d^-1)(b - A x) per sweep, restriction as the
@@ -318,7 +318,7 @@ def _cycle_f32(a, kind, b, coarse_f64=True, round_levels=True):
     cur = a
     while True:
         mats.append(cur)
-        if len(cur[0]) - 1 <= 4:
+        if len(cur[0]) - 1 <= coarse_rows:
             break
         t = O.aggregate(O.strength(*cur, 0.25), 8, True)
         aggs.append(t)
@@ -433,3 +433,15 @@ def test_oracle_fp32_cycle_pinned_on_heat3d():
     assert np.array_equal(x, _cycle_f32(a, 0, b))
     x64 = O.Amg(coarse_rows=4).setup(a).apply(b)
     assert np.max(np.abs(x - x64)) <= 1e-5 * np.max(np.abs(x64))
+
+
+def test_oracle_fp32_coarse_solve_stays_fp64():
+    """With a 60-row coarsest level, the FP32 cycle still equals the float32
+    restatement bit for bit, and the restatement with the coarse solve done in
+    float32 (a plausible precision mistake) does not."""
+    a, b = P.heat3d_lognormal(10, seed=1)
+    amg = O.Amg(coarse_rows=64, level_precision=32).setup(a)
+    assert len(amg.level_matrix(amg.levels() - 1)[0]) - 1 > 16
+    x = amg.apply(b)
+    assert np.array_equal(x, _cycle_f32(a, 0, b, coarse_rows=64))
+    assert not np.array_equal(x, _cycle_f32(a, 0, b, coarse_f64=False, coarse_rows=64))
