@@ -66,16 +66,36 @@ struct Layout {
 
 // ------------------------------------------------------------ graph build
 
+// The graph kernels give each row a group of kRowLanes lanes that walk its
+// entries kRowLanes at a time (coalesced loads and stores); an entry's place
+// in the row's edge list is the number of off-diagonal entries before it,
+// counted with a ballot over the group.
+constexpr int kRowLanes = 16;
+
+struct RowLane {
+    int i, lane, gbase;
+};
+__device__ __forceinline__ RowLane row_lane() {
+    const int t = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    return RowLane{t / kRowLanes, t % kRowLanes, (int(threadIdx.x) % kWave) / kRowLanes * kRowLanes};
+}
+// ballot bits of the calling lane's group, and how many of them lie below it
+__device__ __forceinline__ unsigned group_bits(bool pred, const RowLane& r) {
+    const unsigned long long b = __ballot(pred);
+    return unsigned((b >> r.gbase) & ((1ull << kRowLanes) - 1));
+}
+
 __global__ void k_offdiag_count(int n, const int* __restrict__ rp, const int* __restrict__ ci, int* cnt) {
-    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i > n) return;
-    if (i == n) {
-        cnt[n] = 0;
+    const RowLane r = row_lane();
+    if (r.i > n) return;
+    if (r.i == n) {
+        if (r.lane == 0) cnt[n] = 0;
         return;
     }
     int c = 0;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) c += ci[j] != i;
-    cnt[i] = c;
+    for (int j = rp[r.i] + r.lane; j < rp[r.i + 1]; j += kRowLanes) c += ci[j] != r.i;
+    c = group_sum<kRowLanes>(c);
+    if (r.lane == 0) cnt[r.i] = c;
 }
 
 // Edges of row i in CSR order, node features, |d_i| and m_i.
@@ -83,24 +103,42 @@ __global__ void k_edges_nodes(int n, const int* __restrict__ rp, const int* __re
                               const double* __restrict__ v, const int* __restrict__ erp, int* __restrict__ ei,
                               int* __restrict__ ej, double* __restrict__ ea, double* __restrict__ dabs,
                               double* __restrict__ mrow, float* __restrict__ f, float* __restrict__ deg) {
-    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i >= n) return;
+    const RowLane r = row_lane();
+    const bool live = r.i < n;  // (every lane of a group takes part in the ballots)
+    const int i = live ? r.i : n - 1;
     double d = 0, m = 0, sum = 0;
     int at = erp[i];
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
-        const int c = ci[j];
-        if (c == i) {
-            d += v[j];
-            continue;
+    const int s = rp[i], e = rp[i + 1];
+    for (int j0 = s; j0 < e; j0 += kRowLanes) {
+        const int j = j0 + r.lane;
+        const bool valid = j < e;
+        const int c = valid ? ci[j] : i;
+        const bool off = valid && c != i;
+        const unsigned bits = group_bits(off, r);
+        if (valid) {
+            const double x = v[j];
+            if (c == i) {
+                d += x;
+            }
+            else {
+                const double ax = fabs(x);
+                const int q = at + __popc(bits & ((1u << r.lane) - 1));
+                if (live) {
+                    ei[q] = i;
+                    ej[q] = c;
+                    ea[q] = ax;
+                }
+                m = fmax(m, ax);
+                sum += ax;
+            }
         }
-        const double a = fabs(v[j]);
-        ei[at] = i;
-        ej[at] = c;
-        ea[at] = a;
-        ++at;
-        m = fmax(m, a);
-        sum += a;
+        at += __popc(bits);
     }
+    d = group_sum<kRowLanes>(d);
+    sum = group_sum<kRowLanes>(sum);
+#pragma unroll
+    for (int o = kRowLanes / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, kRowLanes));
+    if (!live || r.lane != 0) return;
     const int dg = erp[i + 1] - erp[i];
     d = fabs(d);
     dabs[i] = d;
@@ -112,9 +150,11 @@ __global__ void k_edges_nodes(int n, const int* __restrict__ rp, const int* __re
     f[size_t(i) * FN + 3] = 1.0f;
 }
 
-// twin edge (j, i) of every edge by binary search in row j; bad when absent.
+// twin edge (j, i) of every edge by binary search in row j (bad when
+// absent), and the edge's features from it.
 __global__ void k_twins(int64_t E, const int* __restrict__ erp, const int* __restrict__ ei,
-                        const int* __restrict__ ej, int* __restrict__ twin, int* bad) {
+                        const int* __restrict__ ej, int* __restrict__ twin, int* bad, const double* __restrict__ ea,
+                        const double* __restrict__ dabs, const double* __restrict__ mrow, float* __restrict__ g) {
     const int64_t p = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (p >= E) return;
     const int i = ei[p], j = ej[p];
@@ -132,17 +172,9 @@ __global__ void k_twins(int64_t E, const int* __restrict__ erp, const int* __res
             hi = mid - 1;
     }
     if (found < 0) atomicOr(bad, 1);
-    twin[p] = found < 0 ? int(p) : found;
-}
-
-__global__ void k_edge_features(int64_t E, const int* __restrict__ ei, const int* __restrict__ ej,
-                                const int* __restrict__ twin, const double* __restrict__ ea,
-                                const double* __restrict__ dabs, const double* __restrict__ mrow,
-                                float* __restrict__ g) {
-    const int64_t p = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (p >= E) return;
-    const int i = ei[p], j = ej[p];
-    const double s = 0.5 * (ea[p] + ea[twin[p]]);
+    const int t = found < 0 ? int(p) : found;
+    twin[p] = t;
+    const double s = 0.5 * (ea[p] + ea[t]);
     const double ri = s / mrow[i], rj = s / mrow[j];
     g[size_t(p) * FE + 0] = float(s / sqrt(dabs[i] * dabs[j]));
     g[size_t(p) * FE + 1] = float(fmin(ri, rj));
@@ -321,6 +353,237 @@ __global__ __launch_bounds__(kBlock) void k_score(int64_t E, const float* __rest
     }
 }
 
+// ------------------------------------------------ fused inference (MFMA)
+//
+// Inference never materialises the edge activations: one kernel per
+// message-passing layer forms each edge's message, its row mean and the node
+// update, and one kernel forms the scorer's features, its hidden layer and
+// the score.  Every product runs on the matrix cores as
+// v_mfma_f32_16x16x32_bf16 (16 rows x 32 inputs x 16 outputs; lane l holds
+// row l % 16, inputs 8 (l / 16) .. +7 of A and of B's column l % 16; the
+// accumulator's lane l register q is row 4 (l / 16) + q, column l % 16 --
+// the layout was checked on gfx950 with distinct values).  The weights are
+// bfloat16 values already (the rounded copy the training path uses), so they
+// go in exactly; an FP32 activation x goes in as two bfloat16 terms,
+// hi = bf16(x) and lo = bf16(x - hi), whose products are accumulated in FP32:
+// x is represented to 2^-17 of itself, so the scores agree with the FP32
+// reference to ~1e-6 while the products run at the bfloat16 matrix rate.
+typedef float mf4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ mf4 mfma32(bf8 a, bf8 b, mf4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+}
+
+// acc += x W^T for one 32-input chunk: x split into hi + lo.
+__device__ __forceinline__ mf4 mfma_split(const float (&x)[8], bf8 w, mf4 acc) {
+    bf8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 h = static_cast<__bf16>(x[e]);
+        hi[e] = h;
+        lo[e] = static_cast<__bf16>(x[e] - static_cast<float>(h));
+    }
+    acc = mfma32(hi, w, acc);
+    return mfma32(lo, w, acc);
+}
+
+// B operand of a 32-input chunk for output half hf: inputs k0 + 8a .. +7 of
+// output 16 hf + c (0 past the layer's `in` inputs).
+__device__ __forceinline__ bf8 wchunk(const float* __restrict__ W, int in, int hf, int c, int k0) {
+    bf8 b;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int k = k0 + e;
+        b[e] = static_cast<__bf16>(k < in ? W[(16 * hf + c) * in + k] : 0.0f);
+    }
+    return b;
+}
+
+__device__ __forceinline__ void load8(const float* __restrict__ row, int a, float (&v)[8]) {
+    const float4 p = *reinterpret_cast<const float4*>(row + 8 * a);
+    const float4 q = *reinterpret_cast<const float4*>(row + 8 * a + 4);
+    v[0] = p.x; v[1] = p.y; v[2] = p.z; v[3] = p.w;
+    v[4] = q.x; v[5] = q.y; v[6] = q.z; v[7] = q.w;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+constexpr int kFuseWaves = kBlock / kWave;
+
+// Work items [0, count) for a persistent grid (a multiple of kXcds blocks):
+// XCD x walks the x-th contiguous eighth, its waves interleaved, so the rows
+// of h that neighbouring items gather stay in that XCD's L2.
+struct XcdRange {
+    int64_t first, end, step;
+};
+__device__ __forceinline__ XcdRange xcd_range(int64_t count, int wave) {
+    const int per = int(gridDim.x) / kXcds;
+    const int x = int(blockIdx.x % kXcds), idx = int(blockIdx.x / kXcds);
+    const int64_t span = (count + kXcds - 1) / kXcds;
+    const int64_t begin = x * span;
+    return XcdRange{begin + int64_t(idx) * kFuseWaves + wave, min(count, begin + span), int64_t(per) * kFuseWaves};
+}
+
+// One message-passing layer for chunks of 16 rows, a wave each:
+//   m_p = relu(Wm [h_j ; g_p] + bm) for the chunk's edges, 16 a step;
+//   the chunk's row sums M += S m with S[r][p] = 1 when edge p is in row r
+//   (a matrix-core product too, so every row's messages are reduced in one
+//   fixed order), then M_i /= deg_i;
+//   hout_i = relu(Wu [h_i ; M_i] + bu).
+__global__ __launch_bounds__(kBlock) void k_gnn_layer(int n, const int* __restrict__ erp, const int* __restrict__ ei,
+                                                      const int* __restrict__ ej, const float* __restrict__ g,
+                                                      const float* __restrict__ h, const float* __restrict__ Wm,
+                                                      const float* __restrict__ bm, const float* __restrict__ Wu,
+                                                      const float* __restrict__ bu, float* __restrict__ hout) {
+    constexpr int INM = H + FE, INU = 2 * H;
+    __shared__ float sm[kFuseWaves][16][H + 1];
+    __shared__ int srow[kFuseWaves][16];
+    const int wv = int(threadIdx.x) / kWave, l = int(threadIdx.x) % kWave;
+    const int a = l / 16, c = l % 16;
+    bf8 wM[2][2], wU[2][2];
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) {
+        wM[hf][0] = wchunk(Wm, INM, hf, c, 8 * a);
+        wM[hf][1] = wchunk(Wm, INM, hf, c, H + 8 * a);
+        wU[hf][0] = wchunk(Wu, INU, hf, c, 8 * a);
+        wU[hf][1] = wchunk(Wu, INU, hf, c, H + 8 * a);
+    }
+    const float bm0 = bm[c], bm1 = bm[16 + c], bu0 = bu[c], bu1 = bu[16 + c];
+    const XcdRange rg = xcd_range((n + 15) / 16, wv);
+    for (int ch = int(rg.first); ch < rg.end; ch += int(rg.step)) {
+        const int r0 = ch * 16, rows = min(16, n - r0);
+        const int e0 = erp[r0], e1 = erp[r0 + rows];
+        mf4 M0 = {0, 0, 0, 0}, M1 = {0, 0, 0, 0};
+        for (int t0 = e0; t0 < e1; t0 += 16) {
+            const int cnt = min(16, e1 - t0);
+            const int e = t0 + min(c, cnt - 1);
+            float x[8];
+            load8(h + size_t(ej[e]) * H, a, x);
+            float gx[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) gx[q] = (a == 0 && q < FE) ? g[size_t(e) * FE + q] : 0.0f;
+            if (l < 16) srow[wv][l] = l < cnt ? ei[e] - r0 : -1;
+            mf4 acc0 = {bm0, bm0, bm0, bm0}, acc1 = {bm1, bm1, bm1, bm1};
+            acc0 = mfma_split(x, wM[0][0], acc0);
+            acc1 = mfma_split(x, wM[1][0], acc1);
+            acc0 = mfma_split(gx, wM[0][1], acc0);
+            acc1 = mfma_split(gx, wM[1][1], acc1);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                sm[wv][4 * a + q][c] = fmaxf(acc0[q], 0.0f);
+                sm[wv][4 * a + q][16 + c] = fmaxf(acc1[q], 0.0f);
+            }
+            wave_lds_sync();
+            // M += S m: A = S (row c, edges 8a..8a+7), B = m (edges 8a..+7, channel c / 16 + c);
+            // edges 16..31 of the product are zero
+            bf8 S;
+            float m0[8], m1[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int p = 8 * a + q;
+                const bool in = p < 16;
+                S[q] = static_cast<__bf16>(in && srow[wv][in ? p : 0] == c ? 1.0f : 0.0f);
+                m0[q] = in ? sm[wv][in ? p : 0][c] : 0.0f;
+                m1[q] = in ? sm[wv][in ? p : 0][16 + c] : 0.0f;
+            }
+            bf8 h0, l0, h1, l1;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                h0[q] = static_cast<__bf16>(m0[q]);
+                l0[q] = static_cast<__bf16>(m0[q] - static_cast<float>(h0[q]));
+                h1[q] = static_cast<__bf16>(m1[q]);
+                l1[q] = static_cast<__bf16>(m1[q] - static_cast<float>(h1[q]));
+            }
+            M0 = mfma32(S, h0, M0);
+            M0 = mfma32(S, l0, M0);
+            M1 = mfma32(S, h1, M1);
+            M1 = mfma32(S, l1, M1);
+            wave_lds_sync();
+        }
+        // M (rows 4a + q, channels c / 16 + c) / deg -> LDS -> the update's A operand
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = min(r0 + 4 * a + q, n - 1);
+            const int dg = erp[row + 1] - erp[row];
+            const float d = float(dg > 0 ? dg : 1);
+            sm[wv][4 * a + q][c] = M0[q] / d;
+            sm[wv][4 * a + q][16 + c] = M1[q] / d;
+        }
+        wave_lds_sync();
+        const int r = min(r0 + c, n - 1);
+        float x[8], mx[8];
+        load8(h + size_t(r) * H, a, x);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) mx[q] = sm[wv][c][8 * a + q];
+        mf4 acc0 = {bu0, bu0, bu0, bu0}, acc1 = {bu1, bu1, bu1, bu1};
+        acc0 = mfma_split(x, wU[0][0], acc0);
+        acc1 = mfma_split(x, wU[1][0], acc1);
+        acc0 = mfma_split(mx, wU[0][1], acc0);
+        acc1 = mfma_split(mx, wU[1][1], acc1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = r0 + 4 * a + q;
+            if (row < n) {
+                hout[size_t(row) * H + c] = fmaxf(acc0[q], 0.0f);
+                hout[size_t(row) * H + 16 + c] = fmaxf(acc1[q], 0.0f);
+            }
+        }
+        wave_lds_sync();
+    }
+}
+
+// The scorer for 16 edges a step per wave: z_p = relu(We [h_i + h_j ;
+// |h_i - h_j| ; g_p] + be), w_p = sigmoid(wo . z_p + bo).
+__global__ __launch_bounds__(kBlock) void k_gnn_score(int64_t E, const int* __restrict__ ei,
+                                                      const int* __restrict__ ej, const float* __restrict__ g,
+                                                      const float* __restrict__ h, const float* __restrict__ We,
+                                                      const float* __restrict__ be, const float* __restrict__ wo,
+                                                      const float* __restrict__ bo, float* __restrict__ w) {
+    constexpr int IN = 2 * H + FE;
+    const int wv = int(threadIdx.x) / kWave, l = int(threadIdx.x) % kWave;
+    const int a = l / 16, c = l % 16;
+    bf8 wE[2][3];
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) wE[hf][k] = wchunk(We, IN, hf, c, 32 * k + 8 * a);
+    const float be0 = be[c], be1 = be[16 + c], wo0 = wo[c], wo1 = wo[16 + c], b0 = bo[0];
+    const int64_t tiles = (E + 15) / 16;
+    const XcdRange rg = xcd_range(tiles, wv);
+    for (int64_t t = rg.first; t < rg.end; t += rg.step) {
+        const int64_t t0 = t * 16;
+        const int64_t e = min(t0 + c, E - 1);
+        float hi[8], hj[8], xs[8], xd[8], gx[8];
+        load8(h + size_t(ei[e]) * H, a, hi);
+        load8(h + size_t(ej[e]) * H, a, hj);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            xs[q] = hi[q] + hj[q];
+            xd[q] = fabsf(hi[q] - hj[q]);
+            gx[q] = (a == 0 && q < FE) ? g[size_t(e) * FE + q] : 0.0f;
+        }
+        mf4 acc0 = {be0, be0, be0, be0}, acc1 = {be1, be1, be1, be1};
+        acc0 = mfma_split(xs, wE[0][0], acc0);
+        acc1 = mfma_split(xs, wE[1][0], acc1);
+        acc0 = mfma_split(xd, wE[0][1], acc0);
+        acc1 = mfma_split(xd, wE[1][1], acc1);
+        acc0 = mfma_split(gx, wE[0][2], acc0);
+        acc1 = mfma_split(gx, wE[1][2], acc1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float sc = fmaf(wo0, fmaxf(acc0[q], 0.0f), wo1 * fmaxf(acc1[q], 0.0f));
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) sc += __shfl_xor(sc, o, 16);
+            const int64_t p = t0 + 4 * a + q;
+            if (c == 0 && p < E) w[p] = 1.0f / (1.0f + expf(-(b0 + sc)));
+        }
+    }
+}
+
 // dz_p = ds_p wo masked by z_p > 0
 __global__ void k_dz(int64_t E, const float* __restrict__ ds, const float* __restrict__ wo,
                      const float* __restrict__ z, float* __restrict__ dz) {
@@ -431,20 +694,37 @@ __global__ void k_affinity(int n, int k, const int* __restrict__ rp, const int* 
 // Scatter of the edge weights to the matrix positions (0 on the diagonal).
 __global__ void k_to_positions(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                const int* __restrict__ erp, const float* __restrict__ w, double* __restrict__ out) {
-    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i >= n) return;
+    const RowLane r = row_lane();
+    const bool live = r.i < n;
+    const int i = live ? r.i : n - 1;
     int at = erp[i];
-    for (int p = rp[i]; p < rp[i + 1]; ++p) out[p] = ci[p] == i ? 0.0 : double(w[at++]);
+    const int e = rp[i + 1];
+    for (int j0 = rp[i]; j0 < e; j0 += kRowLanes) {
+        const int j = j0 + r.lane;
+        const bool valid = j < e;
+        const bool off = valid && ci[j] != i;
+        const unsigned bits = group_bits(off, r);
+        if (live && valid) out[j] = off ? double(w[at + __popc(bits & ((1u << r.lane) - 1))]) : 0.0;
+        at += __popc(bits);
+    }
 }
 
 // Edge targets from matrix-position targets.
 __global__ void k_from_positions(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                  const int* __restrict__ erp, const float* __restrict__ t, float* __restrict__ out) {
-    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i >= n) return;
+    const RowLane r = row_lane();
+    const bool live = r.i < n;
+    const int i = live ? r.i : n - 1;
     int at = erp[i];
-    for (int p = rp[i]; p < rp[i + 1]; ++p)
-        if (ci[p] != i) out[at++] = t[p];
+    const int e = rp[i + 1];
+    for (int j0 = rp[i]; j0 < e; j0 += kRowLanes) {
+        const int j = j0 + r.lane;
+        const bool valid = j < e;
+        const bool off = valid && ci[j] != i;
+        const unsigned bits = group_bits(off, r);
+        if (live && off) out[at + __popc(bits & ((1u << r.lane) - 1))] = t[j];
+        at += __popc(bits);
+    }
 }
 
 unsigned blocks_of(int64_t count) { return unsigned((count + kBlock - 1) / kBlock); }
@@ -516,6 +796,8 @@ struct GnnModel::Impl {
         deg = falloc(capN);
         g = falloc(capE * FE);
         for (int l = 0; l <= kLayers; ++l) h[l] = falloc(capN * H);
+        w = falloc(capE);
+        if (!training) return;  // inference keeps no edge activations
         for (int l = 0; l < kLayers; ++l) {
             xm[l] = falloc(capE * (H + FE));
             m[l] = falloc(capE * H);
@@ -523,8 +805,6 @@ struct GnnModel::Impl {
         }
         xe = falloc(capE * (2 * H + FE));
         z = falloc(capE * H);
-        w = falloc(capE);
-        if (!training) return;  // gradients only for training
         t = falloc(capE);
         ds = falloc(capE);
         dE = falloc(capE * (2 * H + FE));
@@ -538,8 +818,8 @@ struct GnnModel::Impl {
         require(nn >= 1, "the network needs a nonempty matrix");
         // edge count
         int* tmp = dev_alloc<int>(*c, size_t(nn) + 1);
-        hipLaunchKernelGGL(k_offdiag_count, dim3(blocks_of(int64_t(nn) + 1)), dim3(kBlock), 0, c->stream, nn, rp, ci,
-                           tmp);
+        hipLaunchKernelGGL(k_offdiag_count, dim3(blocks_of((int64_t(nn) + 1) * kRowLanes)), dim3(kBlock), 0,
+                           c->stream, nn, rp, ci, tmp);
         int* trp = dev_alloc<int>(*c, size_t(nn) + 1);
         scan_excl(*c, nn + 1, tmp, trp);
         const int64_t ee = c->read_int(trp + nn);
@@ -552,18 +832,16 @@ struct GnnModel::Impl {
         dev_free(*c, trp);
         n = nn;
         E = ee;
-        hipLaunchKernelGGL(k_edges_nodes, dim3(blocks_of(n)), dim3(kBlock), 0, c->stream, n, rp, ci, v, erp, ei, ej,
+        hipLaunchKernelGGL(k_edges_nodes, dim3(blocks_of(int64_t(n) * kRowLanes)), dim3(kBlock), 0, c->stream, n, rp,
+                           ci, v, erp, ei, ej,
                            ea, dabs, mrow, f, deg);
         c->clear_flags(1);
         if (E > 0)
             hipLaunchKernelGGL(k_twins, dim3(blocks_of(E)), dim3(kBlock), 0, c->stream, E, erp, ei, ej, twin,
-                               c->dflags);
+                               c->dflags, ea, dabs, mrow, g);
+        check(hipGetLastError(), "graph build launch");
         c->read_flags(1);
         require(c->hflags[0] == 0, "learned affinities need a matrix with a symmetric pattern");
-        if (E > 0)
-            hipLaunchKernelGGL(k_edge_features, dim3(blocks_of(E)), dim3(kBlock), 0, c->stream, E, ei, ej, twin, ea,
-                               dabs, mrow, g);
-        check(hipGetLastError(), "graph build launch");
     }
 
     void dense(int64_t rows, int in, const float* X, int Woff, int boff, float* Y) {
@@ -605,6 +883,23 @@ struct GnnModel::Impl {
                                target, ds, lpart);
         }
         check(hipGetLastError(), "network forward launch");
+    }
+
+    // inference: the fused layer and scorer kernels
+    void forward_infer() {
+        dense(n, FN, f, L.W0, L.b0, h[0]);
+        const int chunks = (n + 15) / 16;
+        const unsigned lb = unsigned(round_xcd(std::min(2048, (chunks + kFuseWaves - 1) / kFuseWaves)));
+        for (int l = 0; l < kLayers; ++l)
+            hipLaunchKernelGGL(k_gnn_layer, dim3(lb), dim3(kBlock), 0, c->stream, n, erp, ei, ej, g, h[l],
+                               wq + L.Wm[l], wq + L.bm[l], wq + L.Wu[l], wq + L.bu[l], h[l + 1]);
+        if (E > 0) {
+            const int64_t tiles = (E + 15) / 16;
+            const unsigned sb = unsigned(round_xcd(int(std::min<int64_t>(8192, (tiles + kFuseWaves - 1) / kFuseWaves))));
+            hipLaunchKernelGGL(k_gnn_score, dim3(sb), dim3(kBlock), 0, c->stream, E, ei, ej, g, h[kLayers],
+                               wq + L.We, wq + L.be, wq + L.wo, wq + L.bo, w);
+        }
+        check(hipGetLastError(), "network inference launch");
     }
 
     double loss_value() {
@@ -700,9 +995,9 @@ void GnnModel::get_params(float* host) {
 
 void GnnModel::infer(int n, int64_t nnz, const int* rp, const int* ci, const double* v, double* out) {
     p_->graph(n, nnz, rp, ci, v, false);
-    p_->forward(nullptr);
-    hipLaunchKernelGGL(k_to_positions, dim3(blocks_of(n)), dim3(kBlock), 0, p_->c->stream, n, rp, ci, p_->erp, p_->w,
-                       out);
+    p_->forward_infer();
+    hipLaunchKernelGGL(k_to_positions, dim3(blocks_of(int64_t(n) * kRowLanes)), dim3(kBlock), 0, p_->c->stream, n, rp,
+                       ci, p_->erp, p_->w, out);
     check(hipGetLastError(), "edge weight scatter launch");
 }
 
@@ -710,8 +1005,8 @@ double GnnModel::loss_grad(int n, int64_t nnz, const int* rp, const int* ci, con
                            float* hostGrad) {
     require(nnz >= 0, "negative nonzero count");
     p_->graph(n, nnz, rp, ci, v, true);
-    hipLaunchKernelGGL(k_from_positions, dim3(blocks_of(n)), dim3(kBlock), 0, p_->c->stream, n, rp, ci, p_->erp, target,
-                       p_->t);
+    hipLaunchKernelGGL(k_from_positions, dim3(blocks_of(int64_t(n) * kRowLanes)), dim3(kBlock), 0, p_->c->stream, n,
+                       rp, ci, p_->erp, target, p_->t);
     require(p_->E > 0 && p_->E <= (int64_t(1) << 22) * kBlock, "training needs 1 .. 2^30 edges");
     p_->forward(p_->t);
     const double l = p_->loss_value();
