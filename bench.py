@@ -49,13 +49,15 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch each iteration's kernels without hipGraph")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the level-0 SpMV launches and a copy kernel of known size (for rocprofv3 --pmc)")
-    ap.add_argument("--workload", choices=["heat3d", "convdiff", "fem", "batch", "learned"], default="heat3d",
+    ap.add_argument("--workload", choices=["heat3d", "convdiff", "fem", "batch", "learned", "infer"], default="heat3d",
                     help="convdiff: 96^3 upwind convection-diffusion, BiCGStab with V/W/F cycles, FP64 and FP32 "
                          "levels, cell Peclet 10/30/100; fem: P1 tets on 64^3 cells, 64 right-hand sides in one "
                          "block CG vs 64 single solves, and the k=64 SpMM; batch: 32 independent heat problems "
                          "(edges U{16..96}) in one segmented solve vs 32 separate solves; learned: held-out "
                          "rotated anisotropic and jump-coefficient Q1 problems, strength vs learned affinities "
-                         "(each one JSON line of its own)")
+                         "(each one JSON line of its own); infer: the committed network's inference "
+                         "(tf_gnn_infer: graph build, two message-passing layers, scorer) on the 128^3 log-normal heat "
+                         "operator and the held-out rotated anisotropic Q1 operator")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solver runs on a single "
@@ -79,6 +81,9 @@ def main():
         return
     if args.workload == "learned":
         learned(args)
+        return
+    if args.workload == "infer":
+        infer(args)
         return
 
     nx = args.nx
@@ -579,6 +584,39 @@ def learned(args):
                       "data": "synthetic", "config": {"workload": f"held-out Q1 anisotropic/jump {nc}^3 cells",
                                                       "n": (nc - 1) ** 3},
                       "fallback_rate": fallbacks / len(cases), "runs": rows}), flush=True)
+
+
+def infer(args):
+    """Learned-affinity inference alone: tf_gnn_infer on the 128^3 log-normal
+    heat operator and on the rotated anisotropic Q1 operator (128^3 cells,
+    seed 7), each call timed to completion (the entry point synchronises)."""
+    import numpy as np
+    from thermofem_amd import problems as P
+    from thermofem_amd._lib import check, lib
+    from thermofem_amd.device import Context, DeviceArray, DeviceCsr
+    from thermofem_amd.gnn import AffinityModel
+
+    model = AffinityModel.trained()
+    ctx = Context(0)
+    nc = args.nx
+    for name, make in (("heat3d log-normal", lambda: P.heat3d_lognormal(nc, seed=1)[0]),
+                       ("anisotropic seed 7 (Q1)", lambda: P.anisotropic_diffusion(nc, seed=7)[0])):
+        a = make()
+        n, nnz = len(a[0]) - 1, len(a[1])
+        A = DeviceCsr(*a)
+        w = DeviceArray(nnz, np.float64)
+        times = []
+        for it in range(args.warmup + args.steps):
+            ctx.sync()
+            t0 = time.perf_counter()
+            check(lib().tf_gnn_infer(model.handle, n, nnz, *A.args(), w.ptr))  # returns when done
+            if it >= args.warmup:
+                times.append(time.perf_counter() - t0)
+        print(json.dumps({"metric": f"learned-affinity inference s, {name} {nc}^3", "value":
+                          round(statistics.median(times), 6), "unit": "s", "higher_is_better": False, "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "dtype": "f32 (bf16 weights, split-bf16 MFMA)",
+                          "data": "synthetic", "config": {"workload": f"{name} {nc}^3", "n": n, "nnz": nnz}}),
+              flush=True)
 
 
 def csr_matvec(rp, ci, v, x):
