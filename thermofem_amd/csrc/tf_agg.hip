@@ -902,11 +902,10 @@ __global__ void k_merge_write(int rows, const int* __restrict__ start, const u64
 // and later rounds skip its scan (its proposal stays "none").
 constexpr int kNoCandidate = -2;
 
-__global__ void k_propose(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ w,
-                          int* __restrict__ mate, int* __restrict__ proposal, const int* prevChanged, int mark) {
-    if (prevChanged && *prevChanged == 0) return;
-    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i >= n) return;
+// The proposal of vertex i (its best unmatched candidate, or -1).
+__device__ __forceinline__ void propose_vertex(int i, const int* __restrict__ rp, const int* __restrict__ ci,
+                                               const double* __restrict__ w, int* __restrict__ mate,
+                                               int* __restrict__ proposal, int mark) {
     int best = -1;
     double score = 0;
     u64 priority = 0;
@@ -946,16 +945,90 @@ __global__ void k_propose(int n, const int* __restrict__ rp, const int* __restri
     proposal[i] = best;
 }
 
+// Mutual proposals become pairs; true when vertex i was matched.
+__device__ __forceinline__ bool accept_vertex(int i, const int* __restrict__ proposal, int* __restrict__ mate) {
+    const int p = proposal[i];
+    if (p >= 0 && proposal[p] == i) {
+        mate[i] = p;
+        return true;
+    }
+    return false;
+}
+
+__global__ void k_propose(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ w,
+                          int* __restrict__ mate, int* __restrict__ proposal, const int* prevChanged, int mark) {
+    if (prevChanged && *prevChanged == 0) return;
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    propose_vertex(i, rp, ci, w, mate, proposal, mark);
+}
+
 __global__ void k_accept(int n, const int* __restrict__ proposal, int* __restrict__ mate, const int* prevChanged,
                          int* changed) {
     if (prevChanged && *prevChanged == 0) return;
     const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (i >= n) return;
-    const int p = proposal[i];
-    if (p >= 0 && proposal[p] == i) {
-        mate[i] = p;
-        if (changed) *changed = 1;
+    if (accept_vertex(i, proposal, mate) && changed) *changed = 1;
+}
+
+// The whole matching of a small graph (n <= kPairBlockRows) in one
+// workgroup: the rounds (a round without an acceptance ends them, as the
+// per-round kernels skip every later round), then the roots' exclusive
+// scan in LDS and the aggregate ids.  The same operations in the same
+// order as the per-round kernels, so the same matching.
+constexpr int kPairThreads = 1024;
+constexpr int kPairBlockRows = 4096;
+__global__ __launch_bounds__(kPairThreads) void k_pair_block(int n, const int* __restrict__ rp,
+                                                             const int* __restrict__ ci,
+                                                             const double* __restrict__ w, int rounds,
+                                                             int* __restrict__ mate, int* __restrict__ proposal,
+                                                             int* __restrict__ map, int* count) {
+    __shared__ int prefix[kPairBlockRows + 1];
+    __shared__ int wsum[kPairThreads / kWave];
+    __shared__ int changed;
+    const int t = int(threadIdx.x);
+    for (int i = t; i < n; i += kPairThreads) mate[i] = -1;
+    __syncthreads();
+    for (int r = 0; r < rounds; ++r) {
+        if (t == 0) changed = 0;
+        for (int i = t; i < n; i += kPairThreads) propose_vertex(i, rp, ci, w, mate, proposal, 1);
+        __syncthreads();
+        bool any = false;
+        for (int i = t; i < n; i += kPairThreads) any |= accept_vertex(i, proposal, mate);
+        if (any) changed = 1;
+        __syncthreads();
+        if (!changed) break;
+        __syncthreads();  // (changed is reset only after every thread has read it)
     }
+    // roots: flag[i] = 1 for an unmatched vertex or the smaller end of a pair;
+    // thread t scans items [4t, 4t + 4) of the n + 1 flags
+    constexpr int kItems = (kPairBlockRows + 1 + kPairThreads - 1) / kPairThreads;
+    int f[kItems], local = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = t * kItems + k;
+        f[k] = (i < n && (mate[i] < 0 || i < mate[i])) ? 1 : 0;
+        local += f[k];
+    }
+    int incl = local;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int v = __shfl_up(incl, o, kWave);
+        if ((t & (kWave - 1)) >= o) incl += v;
+    }
+    if ((t & (kWave - 1)) == kWave - 1) wsum[t / kWave] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int q = 0; q < t / kWave; ++q) before += wsum[q];
+    int run = before + incl - local;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = t * kItems + k;
+        if (i <= n && i <= kPairBlockRows) prefix[i] = run;
+        run += f[k];
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += kPairThreads) map[i] = prefix[mate[i] < 0 ? i : min(i, mate[i])];
+    if (t == 0) *count = prefix[n];
 }
 
 __global__ void k_roots(int n, const int* __restrict__ mate, int* __restrict__ flags) {
@@ -974,6 +1047,13 @@ __global__ void k_compose(int n, int* __restrict__ map, const int* __restrict__ 
 }
 
 // ---- members (GpuAggregation.cu:586-632) ----
+// sizes[0, count) = 0 and the overflow flag cleared, in one launch
+__global__ void k_members_reset(int count, int* __restrict__ sizes, int* flag) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i < count) sizes[i] = 0;
+    if (i == 0) *flag = 0;
+}
+
 __global__ void k_slot_members(int n, const int* __restrict__ map, int* sizes, int* __restrict__ slots) {
     const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (i >= n) return;
@@ -1003,12 +1083,14 @@ __global__ void k_sort_members(int nc, const int* __restrict__ sizes, int* __res
 }
 
 // ---- absorption (GpuAggregation.cu:646-720) ----
+// (also copies map into next, the map the acceptance then edits)
 __global__ void k_propose_absorption(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                      const double* __restrict__ w, const int* __restrict__ map,
                                      const int* __restrict__ sizes, int* __restrict__ proposal,
-                                     double* __restrict__ scores) {
+                                     double* __restrict__ scores, int* __restrict__ next) {
     const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (i >= n) return;
+    next[i] = map[i];
     int best = -1;
     double score = 0;
     if (sizes[map[i]] == 1) {
@@ -1264,10 +1346,9 @@ DevCsr gather_merge(tf_ctx& c, int nc, const int* offsets, const int* members, c
 void build_members(tf_ctx& c, int n, int nc, const int* map, int** sizesOut, int** slotsOut) {
     int* sizes = c.wst<int>(kSizes, size_t(nc) + 1);
     int* slots = c.wst<int>(kSlots, size_t(nc) * 4 + 1);
-    fill_i32(c, nc + 1, 0, sizes);
+    hipLaunchKernelGGL(k_members_reset, grid1(int64_t(nc) + 1), dim3(kBlock), 0, c.stream, nc + 1, sizes, c.dflags);
     hipLaunchKernelGGL(k_slot_members, grid1(n), dim3(kBlock), 0, c.stream, n, map, sizes, slots);
     check(hipGetLastError(), "slot members launch");
-    c.clear_flags(1);
     hipLaunchKernelGGL(k_sort_members, grid1(nc), dim3(kBlock), 0, c.stream, nc, sizes, slots, c.dflags);
     check(hipGetLastError(), "sort members launch");
     *sizesOut = sizes;
@@ -1459,9 +1540,17 @@ void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w
 }
 
 int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map, int64_t nnz) {
-    if (nnz < 0) nnz = n > 0 ? c.read_int(rp + n) : 0;
     int* mate = c.wst<int>(kMate, size_t(n));
     int* proposal = c.wst<int>(kProposal, size_t(n));
+    if (n >= 1 && n <= kPairBlockRows) {
+        // one launch instead of two per round
+        int* count = c.wst<int>(kChanged, 8);
+        hipLaunchKernelGGL(k_pair_block, dim3(1), dim3(kPairThreads), 0, c.stream, n, rp, ci, w, rounds, mate,
+                           proposal, map, count);
+        check(hipGetLastError(), "small matching launch");
+        return c.read_int(count);
+    }
+    if (nnz < 0) nnz = n > 0 ? c.read_int(rp + n) : 0;
     fill_i32(c, n, -1, mate);
     int* changed = c.wst<int>(kChanged, 8);
     fill_i32(c, 8, 0, changed);
@@ -1520,10 +1609,8 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb, int* hostH
             int* cur = t.map;
             for (int round = 0; round < 2; ++round) {
                 hipLaunchKernelGGL(k_propose_absorption, grid1(n), dim3(kBlock), 0, c.stream, n, g.rp, g.ci, g.v, cur,
-                                   sizes, proposal, scores);
+                                   sizes, proposal, scores, next);
                 check(hipGetLastError(), "propose absorption launch");
-                check(hipMemcpyAsync(next, cur, size_t(n) * sizeof(int), hipMemcpyDeviceToDevice, c.stream),
-                      "absorption snapshot");
                 hipLaunchKernelGGL(k_accept_absorption, grid1(coarse), dim3(kBlock), 0, c.stream, coarse, g.rp, g.ci,
                                    sizes, slots, proposal, scores, next);
                 check(hipGetLastError(), "accept absorption launch");
