@@ -24,9 +24,11 @@ namespace {
 
 constexpr int kSlotA = 40, kSlotB = 41, kSlotC = 42, kSlotD = 43, kSlotE = 44;
 // Levels up to this many rows compute residual and restriction in one
-// aggregate-parallel kernel (one thread per aggregate); measured slower than
-// the streaming residual kernel + restriction at every level of the 128^3
-// hierarchy (too little parallelism per level), so it is off by default.
+// aggregate-parallel kernel (four lanes per aggregate).  It saves a launch
+// but measured slower than the streaming residual kernel + restriction on
+// the latency-bound levels of the 128^3 hierarchy (20.58 -> 20.70 ms per FP32
+// solve with it on levels up to 16,384 rows, 20.80 ms up to 65,536): a
+// member row per lane is a long dependent chain.  Off by default.
 constexpr int kFusedResidualRows = 0;
 // Levels up to this many rows fuse prolongation into the first post-sweep
 // (saves a launch); on larger levels the extra map gather per nonzero costs

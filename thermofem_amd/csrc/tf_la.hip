@@ -175,34 +175,65 @@ __global__ void k_restrict(int nc, const int* __restrict__ off, const int* __res
     }
 }
 
-// Residual and restriction in one pass for one aggregate per thread:
-// coarse[c] = sum over members m (ascending) of (b[m] - sum_j a_mj x[j]),
-// each row sum sequential in stored order -- the same roundings as the
-// separate residual and restriction kernels.  Optional first coarse sweep
-// from zero as in k_restrict.
+// Residual and restriction in one pass, four lanes per aggregate (one
+// member row each, aggregates have at most four; more are taken four at a
+// time): coarse[c] = sum over members m (ascending) of (b[m] - sum_j a_mj
+// x[j]), each row sum sequential in stored order and the members' residuals
+// added in member order from 0 -- the same roundings as the separate
+// residual and restriction kernels.  Optional first coarse sweep from zero
+// as in k_restrict.
 template <class T>
-__global__ void k_residual_restrict(int nc, const int* __restrict__ off, const int* __restrict__ mem,
-                                    const int* __restrict__ rp, const int* __restrict__ ci, const T* __restrict__ v,
-                                    const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ cvec,
-                                    const T* __restrict__ inv, T w, T* __restrict__ xc) {
+__global__ __launch_bounds__(kBlock) void k_residual_restrict(int nc, const int* __restrict__ off,
+                                                              const int* __restrict__ mem,
+                                                              const int* __restrict__ rp, const int* __restrict__ ci,
+                                                              const T* __restrict__ v, const T* __restrict__ b,
+                                                              const T* __restrict__ x, T* __restrict__ cvec,
+                                                              const T* __restrict__ inv, T w, T* __restrict__ xc) {
 #pragma clang fp contract(off)
-    const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (c >= nc) return;
+    constexpr int G = 4;
+    const int c = int((int64_t(blockIdx.x) * kBlock + threadIdx.x) / G);
+    const int q = int(threadIdx.x) % G;
+    const bool live = c < nc;
+    const int cc = live ? c : nc - 1;  // (every lane of a group takes part in the shuffles)
+    const int k0 = off[cc], k1 = off[cc + 1];
     T s = 0;
-    for (int k = off[c]; k < off[c + 1]; ++k) {
-        const int m = mem[k];
-        T acc = 0;
-        for (int j = rp[m]; j < rp[m + 1]; ++j) {
-            const T prod = v[j] * x[ci[j]];
-            acc += prod;
+    for (int kb = k0; kb < k1; kb += G) {
+        const int k = kb + q;
+        T r = 0;
+        if (k < k1) {
+            const int m = mem[k];
+            const int e = rp[m + 1];
+            T acc = 0;
+            for (int j0 = rp[m]; j0 < e; j0 += 8) {
+                T pv[8], xv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int j = min(j0 + u, e - 1);
+                    pv[u] = v[j];
+                    xv[u] = x[ci[j]];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const T p = pv[u] * xv[u];
+                    const T t = acc + p;
+                    acc = j0 + u < e ? t : acc;
+                }
+            }
+            r = b[m] - acc;
         }
-        const T r = b[m] - acc;
-        s += r;
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const T ru = __shfl(r, u, G);
+            const T t = s + ru;
+            s = kb + u < k1 ? t : s;
+        }
     }
-    cvec[c] = s;
-    if (xc) {
-        const T wd = w * inv[c];
-        xc[c] = wd * s;
+    if (live && q == 0) {
+        cvec[c] = s;
+        if (xc) {
+            const T wd = w * inv[c];
+            xc[c] = wd * s;
+        }
     }
 }
 
@@ -359,8 +390,8 @@ template <class T>
 void residual_restrict(tf_ctx& c, int nc, const int* offsets, const int* members, const int* rp, const int* ci,
                        const T* v, const T* b, const T* x, T* coarse, const T* inv, T w, T* xc) {
     if (nc <= 0) return;
-    hipLaunchKernelGGL(k_residual_restrict<T>, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, nc,
-                       offsets, members, rp, ci, v, b, x, coarse, inv, w, xc);
+    hipLaunchKernelGGL(k_residual_restrict<T>, dim3(unsigned((int64_t(nc) * 4 + kBlock - 1) / kBlock)), dim3(kBlock),
+                       0, c.stream, nc, offsets, members, rp, ci, v, b, x, coarse, inv, w, xc);
     check(hipGetLastError(), "residual restrict launch");
 }
 template <class T>
