@@ -450,20 +450,33 @@ __device__ __forceinline__ double fold_partials(const double* __restrict__ parts
 }
 
 // rho = r.z (finaliser of the preconditioner's output step), then the CG
-// direction p = z (fresh) or z + beta p.
+// direction p = z (fresh) or z + beta p.  A true-residual refresh left
+// pending by the previous iteration's check (its |r|^2 partials in pend) is
+// finalised first, as its own finaliser would have done at the end of that
+// iteration: every block redoes the sum, block 0 writes the result back.
 // (z in FP64, or the preconditioner's level-0 iterate in level precision
 // Z, widened exactly; zero: z = 0)
 template <class Z>
 __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, const Z* __restrict__ z, int zero, double* __restrict__ p,
-                                                   const double* __restrict__ partsIn, int nIn, KCtl* ctl) {
+                                                   const double* __restrict__ partsIn, int nIn,
+                                                   const double* __restrict__ pend, int nPend, KCtl* ctl) {
     __shared__ double sBeta;
-    __shared__ int sRun, sFresh;
+    __shared__ int sRun, sFresh, sPend;
     if (stopped(ctl)) return;
+    if (threadIdx.x == 0) sPend = pend && ctl->check && !ctl->early;
+    __syncthreads();
+    const double trueSq = sPend ? fold_partials(pend, nPend) : 0.0;
     const double rho = fold_partials(partsIn, nIn);
     if (threadIdx.x == 0) {
         KCtl c = *ctl;
-        finalize(kFinRhoCg, rho, &c);
+        if (sPend) finalize(kFinTrue, trueSq, &c);  // (idempotent if block 0 already wrote it back)
+        if (c.status == kRunning) finalize(kFinRhoCg, rho, &c);
         if (blockIdx.x == 0) {
+            if (sPend) {
+                ctl->res = c.res;
+                ctl->check = c.check;
+                ctl->fresh = c.fresh;
+            }
             ctl->iter = c.iter;
             ctl->rho = c.rho;
             ctl->beta = c.beta;
@@ -1635,6 +1648,7 @@ void SolverEngine::launch_iteration(const R& rows, int nb) {
         // (producers keep to at most kFoldParts partials)
         const dim3 gf(std::min(int(grid.x), kFoldParts));
         const int nf = int(gf.x);
+        pendParts_ = nf;
         // the direction reads z where the preconditioner leaves it (its
         // level-0 iterate, or r itself without one): z is not written out
         if (amg_) {
@@ -1642,23 +1656,26 @@ void SolverEngine::launch_iteration(const R& rows, int nb) {
             amg_->apply_dot(r_, nullptr, r_, pslot(kSlotA), nv, ctl, &z);
             if (z.f32)
                 hipLaunchKernelGGL(k_cg_dir<float>, vb, dim3(kBlock), 0, c_->stream, n_,
-                                   static_cast<const float*>(z.x), int(z.zero), p_, pslot(kSlotA), nv, ctl);
+                                   static_cast<const float*>(z.x), int(z.zero), p_, pslot(kSlotA), nv, pslot(kSlotD),
+                                   nf, ctl);
             else
                 hipLaunchKernelGGL(k_cg_dir<double>, vb, dim3(kBlock), 0, c_->stream, n_,
-                                   static_cast<const double*>(z.x), int(z.zero), p_, pslot(kSlotA), nv, ctl);
+                                   static_cast<const double*>(z.x), int(z.zero), p_, pslot(kSlotA), nv,
+                                   pslot(kSlotD), nf, ctl);
         }
         else {
             hipLaunchKernelGGL(k_out_dot<double>, vb, dim3(kBlock), 0, c_->stream, n_, r_, 0, r_,
                                static_cast<double*>(nullptr), pslot(kSlotA), ctl);
             hipLaunchKernelGGL(k_cg_dir<double>, vb, dim3(kBlock), 0, c_->stream, n_, r_, 0, p_, pslot(kSlotA), nv,
-                               ctl);
+                               pslot(kSlotD), nf, ctl);
         }
         hipLaunchKernelGGL(k_kspmv_dot<R>, gf, dim3(kBlock), 0, c_->stream, rows, p_, q_, p_, pslot(kSlotB), ctl);
         hipLaunchKernelGGL(k_cg_step, vb, dim3(kBlock), 0, c_->stream, n_, p_, q_, x_, r_, pslot(kSlotB), nf,
                            pslot(kSlotC), ctl);
         hipLaunchKernelGGL(k_cg_check<R>, gf, dim3(kBlock), 0, c_->stream, rows, rhs_, x_, r_, pslot(kSlotC), nv,
                            pslot(kSlotD), ctl);
-        fin(kSlotD, nf, kFinTrue, 0);
+        // (a refresh's finaliser runs in the next iteration's direction
+        // kernel, or in settle() before the host reads the control block)
         (void)nu;
     }
     else {
@@ -1685,6 +1702,13 @@ void SolverEngine::launch_iteration(const R& rows, int nb) {
     }
 #undef TF_G
     check(hipGetLastError(), "Krylov iteration launch");
+}
+
+void SolverEngine::settle() {
+    if (kind_ != 0 || pendParts_ <= 0) return;
+    hipLaunchKernelGGL(k_kfinal, dim3(1), dim3(1024), 0, c_->stream, c_->partials + size_t(kSlotD) * kMaxReduceBlocks,
+                       pendParts_, static_cast<KCtl*>(ctl_), int(kFinTrue), 0);
+    check(hipGetLastError(), "refresh finaliser launch");
 }
 
 void SolverEngine::true_residual() {
@@ -1760,6 +1784,7 @@ tf_solve_result SolverEngine::apply(const double* b, double* x) {
             const uint64_t key = amg_ ? amg_->generation() : 0;
             for (int q = 0; q < batch; ++q) launch_captured(*c_, graph_, o_.graph, key, [&] { launch_part(); });
             k += batch;
+            settle();
             check(hipMemcpyAsync(hctl_, ctl_, sizeof(KCtl), hipMemcpyDeviceToHost, c_->stream), "control download");
             c_->sync();
             if (h->status != kRunning) {

@@ -310,6 +310,10 @@ public:
 private:
     void precondition(const double* in, double* out);
     void launch_part();
+    // finalises a true-residual refresh the last CG iteration left pending
+    // (its finaliser otherwise runs in the next iteration's direction kernel)
+    void settle();
+    int pendParts_ = 0;
     void launch_block_iteration(int k);
     struct Segments;
     void launch_segment_iteration();
