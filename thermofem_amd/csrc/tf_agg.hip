@@ -48,6 +48,9 @@ enum Slot {
     kStaging = 21,
     kRawLen = 22,
     kMaxPart = 23,
+    kSizes2 = 24,   // member table of the second pairing (the first's stays in kSizes / kSlots)
+    kSlots2 = 25,
+    kOldScan = 26,
 };
 
 inline dim3 grid1(int64_t n) { return dim3(unsigned((n + kBlock - 1) / kBlock > 0 ? (n + kBlock - 1) / kBlock : 1)); }
@@ -945,6 +948,51 @@ __device__ __forceinline__ void propose_vertex(int i, const int* __restrict__ rp
     proposal[i] = best;
 }
 
+// Sorts four ints ascending (a five-exchange network).
+__device__ __forceinline__ void sort4(int& a, int& b, int& c, int& d) {
+    auto cx = [](int& x, int& y) {
+        const int lo = min(x, y), hi = max(x, y);
+        x = lo;
+        y = hi;
+    };
+    cx(a, b);
+    cx(c, d);
+    cx(a, c);
+    cx(b, d);
+    cx(b, c);
+}
+
+// Member-table entry (sizes[a], slots[4a..]) of the aggregate a that root i
+// and its mate m (m < 0: unmatched) form, as the atomic slotting and
+// per-aggregate sort of GpuAggregation.cu:586-632 would leave it: with no
+// input table the members are {i} or {i, m} (i < m); with one (a pairing of
+// aggregates, each of at most two members) the sorted union of the member
+// lists of i and m.
+__device__ __forceinline__ void write_members(int i, int m, int a, const int* __restrict__ inSizes,
+                                              const int* __restrict__ inSlots, int* __restrict__ sizes,
+                                              int* __restrict__ slots) {
+    int* out = slots + size_t(a) * 4;
+    if (!inSizes) {
+        out[0] = i;
+        if (m >= 0) out[1] = m;
+        sizes[a] = m >= 0 ? 2 : 1;
+        return;
+    }
+    constexpr int kNone = 0x7fffffff;
+    const int na = inSizes[i], nb = m >= 0 ? inSizes[m] : 0;
+    const int* xs = inSlots + size_t(i) * 4;
+    const int* ys = inSlots + size_t(m >= 0 ? m : i) * 4;
+    int x0 = xs[0], x1 = na > 1 ? xs[1] : kNone;
+    int y0 = nb > 0 ? ys[0] : kNone, y1 = nb > 1 ? ys[1] : kNone;
+    sort4(x0, x1, y0, y1);
+    const int total = na + nb;
+    out[0] = x0;
+    if (total > 1) out[1] = x1;
+    if (total > 2) out[2] = y0;
+    if (total > 3) out[3] = y1;
+    sizes[a] = total;
+}
+
 // Mutual proposals become pairs; true when vertex i was matched.
 __device__ __forceinline__ bool accept_vertex(int i, const int* __restrict__ proposal, int* __restrict__ mate) {
     const int p = proposal[i];
@@ -982,7 +1030,10 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_block(int n, const int* _
                                                              const int* __restrict__ ci,
                                                              const double* __restrict__ w, int rounds,
                                                              int* __restrict__ mate, int* __restrict__ proposal,
-                                                             int* __restrict__ map, int* count) {
+                                                             int* __restrict__ map, int* count,
+                                                             const int* __restrict__ inSizes,
+                                                             const int* __restrict__ inSlots, int* __restrict__ sizes,
+                                                             int* __restrict__ slots) {
     __shared__ int prefix[kPairBlockRows + 1];
     __shared__ int wsum[kPairThreads / kWave];
     __shared__ int changed;
@@ -1027,8 +1078,16 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_block(int n, const int* _
         run += f[k];
     }
     __syncthreads();
-    for (int i = t; i < n; i += kPairThreads) map[i] = prefix[mate[i] < 0 ? i : min(i, mate[i])];
-    if (t == 0) *count = prefix[n];
+    for (int i = t; i < n; i += kPairThreads) {
+        const int m = mate[i];
+        const int root = m < 0 ? i : min(i, m);
+        map[i] = prefix[root];
+        if (sizes && root == i) write_members(i, m, prefix[root], inSizes, inSlots, sizes, slots);
+    }
+    if (t == 0) {
+        *count = prefix[n];
+        if (sizes) sizes[prefix[n]] = 0;
+    }
 }
 
 __global__ void k_roots(int n, const int* __restrict__ mate, int* __restrict__ flags) {
@@ -1036,9 +1095,23 @@ __global__ void k_roots(int n, const int* __restrict__ mate, int* __restrict__ f
     if (i <= n) flags[i] = (i < n && (mate[i] < 0 || i < mate[i])) ? 1 : 0;
 }
 
-__global__ void k_assign(int n, const int* __restrict__ mate, const int* __restrict__ prefix, int* __restrict__ map) {
+// Aggregate ids from the roots' prefix (launched over n + 1 threads); with a
+// member table the roots also write their aggregates' entries and thread n
+// the table's closing zero.
+__global__ void k_assign(int n, const int* __restrict__ mate, const int* __restrict__ prefix, int* __restrict__ map,
+                         const int* __restrict__ inSizes, const int* __restrict__ inSlots, int* __restrict__ sizes,
+                         int* __restrict__ slots) {
     const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i < n) map[i] = prefix[mate[i] < 0 ? i : min(i, mate[i])];
+    if (i > n) return;
+    if (i == n) {
+        if (sizes) sizes[prefix[n]] = 0;
+        return;
+    }
+    const int m = mate[i];
+    const int root = m < 0 ? i : min(i, m);
+    const int a = prefix[root];
+    map[i] = a;
+    if (sizes && root == i) write_members(i, m, a, inSizes, inSlots, sizes, slots);
 }
 
 __global__ void k_compose(int n, int* __restrict__ map, const int* __restrict__ second) {
@@ -1125,19 +1198,27 @@ __global__ void k_propose_absorption(int n, const int* __restrict__ rp, const in
     scores[i] = score;
 }
 
+// Aggregates of 2 or 3 members take their best singleton proposers (up to
+// 4 members).  The member table (sizes, slots) is brought up to date in
+// place, as rebuilding it from the new map would leave it: the aggregate's
+// sorted list gains the absorbed rows and each absorbed row's old
+// (singleton) aggregate, cur[row], becomes empty.  Only accepting aggregates
+// (sizes 2..3) are read here and only they and emptied singletons written.
 __global__ void k_accept_absorption(int nc, const int* __restrict__ rp, const int* __restrict__ ci,
-                                    const int* __restrict__ sizes, const int* __restrict__ slots,
+                                    const int* __restrict__ cur, int* __restrict__ sizes, int* __restrict__ slots,
                                     const int* __restrict__ proposal, const double* __restrict__ scores,
                                     int* __restrict__ nextMap) {
     const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (c >= nc || sizes[c] < 2 || sizes[c] >= 4) return;
+    if (c >= nc) return;
+    const int size = sizes[c];
+    if (size < 2 || size >= 4) return;
     int selected[2] = {-1, -1};
     double weights[2] = {-1, -1};
-    const int capacity = 4 - sizes[c];
+    const int capacity = 4 - size;
     // The chosen rows are the top `capacity` proposers under (score desc,
     // row asc), a total order, so entries can be visited in chunks of 8
     // whose loads overlap (indices clamped into the row, extra slots ignored).
-    for (int k = 0; k < sizes[c]; ++k) {
+    for (int k = 0; k < size; ++k) {
         const int row = slots[size_t(c) * 4 + k];
         const int e = rp[row + 1];
         for (int j0 = rp[row]; j0 < e; j0 += 8) {
@@ -1169,8 +1250,28 @@ __global__ void k_accept_absorption(int nc, const int* __restrict__ rp, const in
             }
         }
     }
-    for (int q = 0; q < capacity; ++q)
-        if (selected[q] >= 0) nextMap[selected[q]] = c;
+    constexpr int kNone = 0x7fffffff;
+    int* m = slots + size_t(c) * 4;
+    int m0 = m[0], m1 = m[1], m2 = size > 2 ? m[2] : kNone, m3 = kNone;
+    int added = 0;
+    for (int q = 0; q < capacity; ++q) {
+        const int row = selected[q];
+        if (row < 0) continue;
+        nextMap[row] = c;
+        sizes[cur[row]] = 0;
+        if (size + added == 2)
+            m2 = row;
+        else
+            m3 = row;
+        ++added;
+    }
+    if (!added) return;
+    sort4(m0, m1, m2, m3);
+    m[0] = m0;
+    m[1] = m1;
+    m[2] = m2;
+    if (size + added > 3) m[3] = m3;
+    sizes[c] = size + added;
 }
 
 __global__ void k_occupied(int nc, const int* __restrict__ sizes, int* __restrict__ flags) {
@@ -1193,6 +1294,36 @@ __global__ __launch_bounds__(kBlock) void k_pack_members(int nc, const int* __re
     }
     __syncthreads();
     if (histogram && threadIdx.x < 4 && h[threadIdx.x]) atomicAdd(histogram + threadIdx.x, h[threadIdx.x]);
+}
+
+// The compacted member list after absorption: old aggregate c (of `bound`)
+// that kept members becomes aggregate prefix[c]; its members go to
+// out[oldscan[c]...] (oldscan: exclusive scan of the old sizes, which is
+// also the compacted offsets), offsets[prefix[c]] = oldscan[c], and thread
+// `bound` closes the offsets.  The size histogram as k_pack_members.
+__global__ __launch_bounds__(kBlock) void k_pack_compact(int bound, const int* __restrict__ sizes,
+                                                         const int* __restrict__ slots,
+                                                         const int* __restrict__ prefix,
+                                                         const int* __restrict__ oldscan, int* __restrict__ offsets,
+                                                         int* __restrict__ out, int* histogram) {
+    __shared__ int h[4];
+    if (threadIdx.x < 4) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int c = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (c == bound) {
+        offsets[prefix[bound]] = oldscan[bound];
+    }
+    else if (c < bound) {
+        const int sz = sizes[c];
+        if (sz > 0) {
+            const int at = oldscan[c];
+            offsets[prefix[c]] = at;
+            for (int k = 0; k < sz; ++k) out[at + k] = slots[size_t(c) * 4 + k];
+            atomicAdd(&h[sz - 1], 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && h[threadIdx.x]) atomicAdd(histogram + threadIdx.x, h[threadIdx.x]);
 }
 
 // ---------------- host helpers ----------------
@@ -1279,13 +1410,17 @@ DevCsr gather_merge(tf_ctx& c, int nc, const int* offsets, const int* members, c
     c.clear_flags(3);
     {
         // Rows short on average: groups of 16 lanes (<= 12 entries on
-        // average) or 32 (<= 28), the few long rows queued for waves;
-        // otherwise a wave per row throughout.
-        const bool small = nnzIn <= int64_t(28) * nc;
+        // average), 32 (<= 28) or a wave with one key a lane (<= 60), the
+        // few longer rows queued for waves with up to 512 keys; otherwise
+        // the latter throughout.  (The one-key-a-lane wave keeps its LDS
+        // small, so many more rows are in flight than in the queue's
+        // kernel: level-1 Galerkin products average ~41 raw entries.)
         const bool tiny = nnzIn <= int64_t(12) * nc;
-        const int ng = kBlock / (tiny ? 16 : kMergeSmall);
+        const bool small = nnzIn <= int64_t(28) * nc;
+        const bool medium = nnzIn <= int64_t(60) * nc;
+        const int ng = kBlock / (tiny ? 16 : small ? kMergeSmall : kWave);
         const dim3 grid((nc + ng - 1) / ng), large(std::max(1, std::min((nc + 3) / 4, 4096)));
-        int* list = small ? wl : nullptr;
+        int* list = medium ? wl : nullptr;
 #define TF_MERGE(MX)                                                                                           \
     if (tiny) {                                                                                                \
         hipLaunchKernelGGL((k_merge_small<MX, 16>), grid, dim3(kBlock), 0, c.stream, nc, offsets, members, map, rp, \
@@ -1294,6 +1429,10 @@ DevCsr gather_merge(tf_ctx& c, int nc, const int* offsets, const int* members, c
     else if (small) {                                                                                          \
         hipLaunchKernelGGL((k_merge_small<MX, kMergeSmall>), grid, dim3(kBlock), 0, c.stream, nc, offsets, members, \
                            map, rp, ci, v, int(graph), raw, start, scol, sval, ucnt, wl, c.dflags + 1, c.dflags); \
+    }                                                                                                          \
+    else if (medium) {                                                                                         \
+        hipLaunchKernelGGL((k_merge_small<MX, kWave>), grid, dim3(kBlock), 0, c.stream, nc, offsets, members, map, \
+                           rp, ci, v, int(graph), raw, start, scol, sval, ucnt, wl, c.dflags + 1, c.dflags);     \
     }                                                                                                          \
     hipLaunchKernelGGL(k_merge_large<MX>, large, dim3(kBlock), 0, c.stream, nc, offsets, members, map, rp, ci, v, \
                        int(graph), raw, start, scol, sval, ucnt, list, c.dflags + 1, c.dflags, c.dflags + 2)
@@ -1539,14 +1678,15 @@ void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w
     match_round_flags(c, n, n > 0 ? c.read_int(rp + n) : 0, rp, ci, w, mate, proposal, nullptr, nullptr, false);
 }
 
-int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map, int64_t nnz) {
+int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map, int64_t nnz,
+         const int* inSizes, const int* inSlots, int* sizes, int* slots) {
     int* mate = c.wst<int>(kMate, size_t(n));
     int* proposal = c.wst<int>(kProposal, size_t(n));
     if (n >= 1 && n <= kPairBlockRows) {
         // one launch instead of two per round
         int* count = c.wst<int>(kChanged, 8);
         hipLaunchKernelGGL(k_pair_block, dim3(1), dim3(kPairThreads), 0, c.stream, n, rp, ci, w, rounds, mate,
-                           proposal, map, count);
+                           proposal, map, count, inSizes, inSlots, sizes, slots);
         check(hipGetLastError(), "small matching launch");
         return c.read_int(count);
     }
@@ -1561,15 +1701,17 @@ int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int ro
     check(hipGetLastError(), "roots launch");
     int* prefix = c.wst<int>(kPrefix, size_t(n) + 1);
     scan_excl(c, n + 1, flags, prefix);
-    hipLaunchKernelGGL(k_assign, grid1(n), dim3(kBlock), 0, c.stream, n, mate, prefix, map);
+    hipLaunchKernelGGL(k_assign, grid1(int64_t(n) + 1), dim3(kBlock), 0, c.stream, n, mate, prefix, map, inSizes,
+                       inSlots, sizes, slots);
     check(hipGetLastError(), "assign launch");
     return c.read_int(prefix + n);
 }
 
-DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc,
-                      int64_t nnz) {
-    int *sizes, *slots;
-    build_members(c, n, nc, map, &sizes, &slots);
+namespace {
+// The contracted graph from a member table (sizes: nc + 1 entries, the last
+// zero; slots: 4 sorted members an aggregate).
+DevCsr contract_members(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc,
+                        int64_t nnz, const int* sizes, const int* slots) {
     int* moff = c.wst<int>(kMemOffsets, size_t(nc) + 1);
     scan_excl(c, nc + 1, sizes, moff);
     int* mem = c.wst<int>(kMembers, size_t(n));
@@ -1577,6 +1719,14 @@ DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const doub
                        static_cast<int*>(nullptr));
     check(hipGetLastError(), "pack members launch");
     return gather_merge(c, nc, moff, mem, map, rp, ci, w, true, true, nnz);
+}
+}  // namespace
+
+DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc,
+                      int64_t nnz) {
+    int *sizes, *slots;
+    build_members(c, n, nc, map, &sizes, &slots);
+    return contract_members(c, n, rp, ci, w, map, nc, nnz, sizes, slots);
 }
 
 DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb, int* hostHist) {
@@ -1586,12 +1736,19 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb, int* hostH
     t.n = n;
     t.map = dev_alloc<int>(c, size_t(n));
     try {
-        const int first = pair(c, n, g.rp, g.ci, g.v, rounds, t.map, g.nnz);
-        DevCsr contracted = contract_graph(c, n, g.rp, g.ci, g.v, t.map, first, g.nnz);
+        // Member tables come from the pairings themselves (a root writes its
+        // aggregate's sorted members), and absorption updates them in place.
+        int* sizes1 = c.wst<int>(kSizes, size_t(n) + 1);
+        int* slots1 = c.wst<int>(kSlots, size_t(n) * 4 + 1);
+        const int first = pair(c, n, g.rp, g.ci, g.v, rounds, t.map, g.nnz, nullptr, nullptr, sizes1, slots1);
+        DevCsr contracted = contract_members(c, n, g.rp, g.ci, g.v, t.map, first, g.nnz, sizes1, slots1);
         int* second = c.wst<int>(kSecond, size_t(first));
+        int* sizes = c.wst<int>(kSizes2, size_t(first) + 1);
+        int* slots = c.wst<int>(kSlots2, size_t(first) * 4 + 1);
         int coarse = 0;
         try {
-            coarse = pair(c, first, contracted.rp, contracted.ci, contracted.v, rounds, second, contracted.nnz);
+            coarse = pair(c, first, contracted.rp, contracted.ci, contracted.v, rounds, second, contracted.nnz, sizes1,
+                          slots1, sizes, slots);
         }
         catch (...) {
             free_csr(c, contracted);
@@ -1600,8 +1757,7 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb, int* hostH
         free_csr(c, contracted);
         hipLaunchKernelGGL(k_compose, grid1(n), dim3(kBlock), 0, c.stream, n, t.map, second);
         check(hipGetLastError(), "compose launch");
-        int *sizes, *slots;
-        build_members(c, n, coarse, t.map, &sizes, &slots);
+        int* hist = c.wst<int>(kHist, 4);
         if (absorb) {
             int* proposal = c.wst<int>(kProposal, size_t(n));
             double* scores = c.wst<double>(kPropWeight, size_t(n));
@@ -1612,45 +1768,43 @@ DevAgg aggregate(tf_ctx& c, const DevCsr& g, int rounds, bool absorb, int* hostH
                                    sizes, proposal, scores, next);
                 check(hipGetLastError(), "propose absorption launch");
                 hipLaunchKernelGGL(k_accept_absorption, grid1(coarse), dim3(kBlock), 0, c.stream, coarse, g.rp, g.ci,
-                                   sizes, slots, proposal, scores, next);
+                                   cur, sizes, slots, proposal, scores, next);
                 check(hipGetLastError(), "accept absorption launch");
                 std::swap(cur, next);
-                build_members(c, n, coarse, cur, &sizes, &slots);
             }
             if (cur != t.map)
                 check(hipMemcpyAsync(t.map, cur, size_t(n) * sizeof(int), hipMemcpyDeviceToDevice, c.stream),
                       "absorbed map copy");
-            int* flags = c.wst<int>(kFlags, size_t(coarse) + 1);
-            hipLaunchKernelGGL(k_occupied, grid1(int64_t(coarse) + 1), dim3(kBlock), 0, c.stream, coarse, sizes,
-                               flags);
+            // compaction: occupied aggregates keep their order
+            const int bound = coarse;
+            int* flags = c.wst<int>(kFlags, size_t(bound) + 1);
+            hipLaunchKernelGGL(k_occupied, grid1(int64_t(bound) + 1), dim3(kBlock), 0, c.stream, bound, sizes, flags);
             check(hipGetLastError(), "occupied launch");
-            int* prefix = c.wst<int>(kPrefix, size_t(coarse) + 1);
-            scan_excl(c, coarse + 1, flags, prefix);
+            int* prefix = c.wst<int>(kPrefix, size_t(bound) + 1);
+            scan_excl(c, bound + 1, flags, prefix);
             hipLaunchKernelGGL(k_compose, grid1(n), dim3(kBlock), 0, c.stream, n, t.map, prefix);
             check(hipGetLastError(), "compaction launch");
-            // The compacted count comes back with the capacity flag; the
-            // members are gathered over the old count, a bound (aggregate
-            // ids past the new count stay empty).
-            const int bound = coarse;
-            check(hipMemcpyAsync(c.dflags + 1, prefix + bound, sizeof(int), hipMemcpyDeviceToDevice, c.stream),
-                  "aggregate count stage");
-            build_members(c, n, bound, t.map, &sizes, &slots);
-            c.read_flags(2);
-            coarse = c.hflags[1];
+            int* oldscan = c.wst<int>(kOldScan, size_t(bound) + 1);
+            scan_excl(c, bound + 1, sizes, oldscan);
+            fill_i32(c, 4, 0, hist);
+            coarse = c.read_int(prefix + bound);
+            t.nc = coarse;
+            t.offsets = dev_alloc<int>(c, size_t(coarse) + 1);
+            t.members = dev_alloc<int>(c, size_t(n));
+            hipLaunchKernelGGL(k_pack_compact, grid1(int64_t(bound) + 1), dim3(kBlock), 0, c.stream, bound, sizes,
+                               slots, prefix, oldscan, t.offsets, t.members, hist);
+            check(hipGetLastError(), "pack members launch");
         }
         else {
-            c.read_flags(1);
+            t.nc = coarse;
+            t.offsets = dev_alloc<int>(c, size_t(coarse) + 1);
+            t.members = dev_alloc<int>(c, size_t(n));
+            scan_excl(c, coarse + 1, sizes, t.offsets);
+            fill_i32(c, 4, 0, hist);
+            hipLaunchKernelGGL(k_pack_members, grid1(coarse), dim3(kBlock), 0, c.stream, coarse, sizes, slots,
+                               t.offsets, t.members, hist);
+            check(hipGetLastError(), "pack members launch");
         }
-        if (c.hflags[0]) throw Error(TF_E_INVALID, "Aggregate capacity exceeded");
-        t.nc = coarse;
-        t.offsets = dev_alloc<int>(c, size_t(coarse) + 1);
-        t.members = dev_alloc<int>(c, size_t(n));
-        scan_excl(c, coarse + 1, sizes, t.offsets);
-        int* hist = c.wst<int>(kHist, 4);
-        fill_i32(c, 4, 0, hist);
-        hipLaunchKernelGGL(k_pack_members, grid1(coarse), dim3(kBlock), 0, c.stream, coarse, sizes, slots, t.offsets,
-                           t.members, hist);
-        check(hipGetLastError(), "pack members launch");
         if (hostHist) {
             check(hipMemcpyAsync(hostHist, hist, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream),
                   "histogram download");

@@ -403,8 +403,14 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
                       const double* learned = nullptr, int deferred = 0);
 double* rowmax_workspace(tf_ctx& c, int n);
 void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal);
-// nnz < 0: read from rp[n].
-int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map, int64_t nnz = -1);
+// nnz < 0: read from rp[n].  With sizes / slots (capacity n + 1 and 4n + 1)
+// the member table of the pairs is written too: sizes[a] and the sorted
+// members in slots[4a..], sizes[count] = 0; with inSizes / inSlots (the
+// table of a previous pairing whose aggregates are this graph's vertices,
+// at most two members each) the members are the union of the paired
+// aggregates' members.
+int pair(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int rounds, int* map, int64_t nnz = -1,
+         const int* inSizes = nullptr, const int* inSlots = nullptr, int* sizes = nullptr, int* slots = nullptr);
 DevCsr contract_graph(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, const int* map, int nc,
                       int64_t nnz = -1);
 // With hostHist (pinned, 4 ints) the aggregate-size histogram is copied

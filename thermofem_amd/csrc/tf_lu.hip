@@ -22,6 +22,16 @@
 // algorithm; inside a trailing update the 16 products are accumulated by the
 // matrix core before the subtraction, so factors agree with the sequential
 // elimination to rounding.
+//
+// Orders up to kLuSmall (the coarsest levels of typical hierarchies) take a
+// latency-lean path instead: one 1024-thread workgroup holds the matrix and
+// the inverse in LDS, a group of kColLanes lanes owns each column, and the
+// unblocked elimination runs with one workgroup barrier per pivot (the pivot
+// column's group searches, swaps and divides; every other group then swaps
+// and updates its own column).  The inverse columns are solved by the same
+// groups with no barrier at all (a column's lanes share a wave).  Its
+// arithmetic is the sequential algorithm's operation for operation, so the
+// factors and inverse columns equal the unblocked elimination's bit for bit.
 #include "tf_device.hpp"
 #include "tf_internal.hpp"
 
@@ -35,6 +45,10 @@ constexpr int kFactorThreads = 512;
 constexpr int kInvThreads = 256;  // one workgroup per 16 inverse columns
 constexpr int kTouched = 2 * kNb; // rows a panel's permutation can move
 constexpr int kStrip = 8;         // trailing-update tiles per wave and round
+constexpr int kLuSmall = 96;      // orders factored and inverted in one workgroup's LDS (2 x 72 KB)
+constexpr int kSmallThreads = 1024;
+constexpr int kColLanes = 8;      // lanes of a column group (kSmallThreads / kColLanes >= kLuSmall)
+static_assert(kSmallThreads / kColLanes >= kLuSmall, "one lane group per column");
 
 // acc(16x16) += A(16x4) B(4x16): lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16] and holds acc[4 q + l / 16][l % 16] in acc[q] (lane
@@ -451,12 +465,178 @@ __global__ __launch_bounds__(kInvThreads) void k_lu_inverse(int n, const double*
     }
 }
 
+// Factor and (when inv is given) invert an order-n matrix, n <= kLuSmall,
+// from CSR in one workgroup.  Lane group g = tid / kColLanes owns column g;
+// its lanes take the rows i = sub, sub + kColLanes, ...  Step k: group k
+// finds the first maximal |a_ik| (i >= k) with the pivot rule above, swaps
+// a_kk and a_pk, and replaces a_ik (i > k) by a_ik / a_kk; after the
+// barrier every other group swaps rows k and p of its column and, right of
+// the pivot, subtracts l_i * a_kj from a_ij (rounded product, then the
+// difference).  A group's step-k update of its column precedes its own
+// pivot work at step k + 1 in program order, and no group touches another's
+// column, so one barrier per step orders everything.
+__global__ __launch_bounds__(kSmallThreads) void k_lu_small(int n, const int* __restrict__ rp,
+                                                            const int* __restrict__ ci,
+                                                            const double* __restrict__ v, double* __restrict__ lu,
+                                                            int* __restrict__ piv, double* __restrict__ inv,
+                                                            int* bad) {
+#pragma clang fp contract(off)
+    __shared__ double M[kLuSmall * kLuSmall];  // row-major, leading dimension n
+    __shared__ double X[kLuSmall * kLuSmall];  // inverse columns, row-major
+    __shared__ int pk[kLuSmall];
+    __shared__ int rowsrc[kLuSmall];
+    __shared__ double sv[kSmallThreads / kWave];
+    __shared__ int failed;
+    const int tid = int(threadIdx.x);
+    const int lane = tid & (kWave - 1), w = tid / kWave;
+    const int g = tid / kColLanes, sub = tid % kColLanes;
+    for (int i = tid; i < n * n; i += kSmallThreads) M[i] = 0.0;
+    if (tid == 0) failed = 0;
+    __syncthreads();
+    double mx = 0;
+    for (int r = tid; r < n; r += kSmallThreads)
+        for (int j = rp[r]; j < rp[r + 1]; ++j) {
+            M[r * n + ci[j]] = v[j];
+            mx = fmax(mx, fabs(v[j]));
+        }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, kWave));
+    if (lane == 0) sv[w] = mx;
+    __syncthreads();
+    double scale = 0;
+#pragma unroll
+    for (int i = 0; i < kSmallThreads / kWave; ++i) scale = fmax(scale, sv[i]);
+    const double thr = 64 * 2.2204460492503131e-16 * scale;
+
+    for (int k = 0; k < n; ++k) {
+        if (g == k) {
+            double bv = -1.0;
+            int bi = -1;
+            for (int i = k + sub; i < n; i += kColLanes) {
+                const double a = fabs(M[i * n + k]);
+                const bool take = a > bv;  // strict: NaN never wins, the first maximum stays
+                bv = take ? a : bv;
+                bi = take ? i : bi;
+            }
+#pragma unroll
+            for (int o = 1; o < kColLanes; o <<= 1) {
+                const double ov = __shfl_xor(bv, o, kWave);
+                const int oi = __shfl_xor(bi, o, kWave);
+                const bool take = (oi >= 0) & ((bi < 0) | (ov > bv) | ((ov == bv) & (oi < bi)));
+                bv = take ? ov : bv;
+                bi = take ? oi : bi;
+            }
+            const int p = (bi < 0 || isnan(M[k * n + k])) ? k : bi;
+            const double pv = M[p * n + k];
+            if (!isfinite(pv) || fabs(pv) <= thr) {
+                if (sub == 0) failed = 1;
+            }
+            else {
+                const double akk = M[k * n + k];
+                wave_sync();
+                if (sub == 0) {
+                    pk[k] = p;
+                    M[p * n + k] = akk;
+                    M[k * n + k] = pv;
+                }
+                wave_sync();
+                for (int i = k + 1 + sub; i < n; i += kColLanes) M[i * n + k] = M[i * n + k] / pv;
+            }
+        }
+        __syncthreads();
+        if (failed) {
+            if (tid == 0) *bad = 1;
+            return;
+        }
+        if (g < n && g != k) {
+            const int p = pk[k];
+            if (p != k && sub == 0) {
+                const double t = M[k * n + g];
+                M[k * n + g] = M[p * n + g];
+                M[p * n + g] = t;
+            }
+            wave_sync();
+            if (g > k) {
+                const double u = M[k * n + g];
+                for (int i = k + 1 + sub; i < n; i += kColLanes) {
+                    const double prod = M[i * n + k] * u;
+                    M[i * n + g] = M[i * n + g] - prod;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    int nonfinite = 0;
+    for (int i = tid; i < n * n; i += kSmallThreads) {
+        const double x = M[i];
+        nonfinite |= isfinite(x) ? 0 : 1;
+        lu[i] = x;
+    }
+    for (int i = tid; i < n; i += kSmallThreads) piv[i] = pk[i];
+    if (__syncthreads_or(nonfinite)) {
+        if (tid == 0) *bad = 1;
+        return;
+    }
+    if (!inv) return;
+
+    // X = P I (the row swaps in pivot order), then L Y = X and U Z = Y by
+    // columns, each as the sequential substitution: forward updates in row
+    // order, backward the division then the updates above.
+    if (tid == 0) {
+        for (int i = 0; i < n; ++i) rowsrc[i] = i;
+        for (int k = 0; k < n; ++k) {
+            const int p = pk[k], t = rowsrc[k];
+            rowsrc[k] = rowsrc[p];
+            rowsrc[p] = t;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < n * n; i += kSmallThreads) {
+        const int r = i / n, c = i - r * n;
+        X[i] = rowsrc[r] == c ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (g < n) {
+        for (int k = 0; k < n; ++k) {
+            wave_sync();
+            const double xk = X[k * n + g];
+            for (int i = k + 1 + sub; i < n; i += kColLanes) {
+                const double prod = M[i * n + k] * xk;
+                X[i * n + g] = X[i * n + g] - prod;
+            }
+        }
+        for (int k = n - 1; k >= 0; --k) {
+            wave_sync();
+            const double xk = X[k * n + g] / M[k * n + k];
+            wave_sync();
+            if (k % kColLanes == sub) X[k * n + g] = xk;
+            for (int i = sub; i < k; i += kColLanes) {
+                const double prod = M[i * n + k] * xk;
+                X[i * n + g] = X[i * n + g] - prod;
+            }
+        }
+    }
+    __syncthreads();
+    // column-major output: column j is contiguous
+    for (int i = tid; i < n * n; i += kSmallThreads) {
+        const int j = i / n, r = i - j * n;
+        inv[i] = X[r * n + j];
+    }
+}
+
 }  // namespace
 
 bool coarse_lu_factor(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* lu, int* piv,
                       double* inverse) {
     require(n >= 1 && n <= kCoarseMax, "GPU LU requires order 1..1024");
     c.clear_flags(1);
+    if (n <= kLuSmall) {
+        hipLaunchKernelGGL(k_lu_small, dim3(1), dim3(kSmallThreads), 0, c.stream, n, rp, ci, v, lu, piv, inverse,
+                           c.dflags);
+        check(hipGetLastError(), "small lu launch");
+        c.read_flags(1);
+        return c.hflags[0] == 0;
+    }
     hipLaunchKernelGGL(k_dense_fill, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, n, rp, ci, v, lu);
     check(hipGetLastError(), "dense fill launch");
     hipLaunchKernelGGL(k_lu_factor, dim3(1), dim3(kFactorThreads), 0, c.stream, n, lu, piv, c.dflags);
