@@ -150,6 +150,45 @@ __global__ void k_validate(int n, const int* __restrict__ rp, const int* __restr
     if (rowmax) rowmax[i] = mx;
 }
 
+// k_validate fused with the level's inverse diagonal in its precision T
+// (canonical rows: the diagonal is one entry, inv = 1 / T(a_ii)), raising
+// bad2 as k_inverse_diagonal would.
+template <class T>
+__global__ void k_validate_inv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                               const double* __restrict__ a, int* bad, double* __restrict__ rowmax,
+                               T* __restrict__ inv, int* bad2) {
+    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (i >= n) return;
+    double diag = 0, mx = 0;
+    bool invalid = false;
+    const int e = rp[i + 1];
+    for (int j0 = rp[i]; j0 < e; j0 += 8) {
+        int cc[8];
+        double aa[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = min(j0 + u, e - 1);
+            cc[u] = ci[j];
+            aa[u] = a[j];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (j0 + u >= e) break;
+            invalid |= !isfinite(aa[u]);
+            if (cc[u] == i)
+                diag = aa[u];
+            else
+                mx = fmax(mx, fabs(aa[u]));
+        }
+    }
+    if (invalid || diag == 0 || !isfinite(diag) || !isfinite(1.0 / diag)) atomicOr(bad, 1);
+    rowmax[i] = mx;
+    const T d = T(diag);
+    const T r = T(1) / d;
+    if (!isfinite(d) || d == T(0) || !isfinite(r)) atomicOr(bad2, 1);
+    inv[i] = r;
+}
+
 // Strength edges (GpuAggregation.cu:339-364): w = |a_ij| / rowmax_i for
 // j != i when rowmax_i > 0, w > 0 and w >= theta; both directions.
 __global__ void k_strength_count(int n, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -1562,6 +1601,18 @@ void validate_rowmax_async(tf_ctx& c, int n, const int* rp, const int* ci, const
     hipLaunchKernelGGL(k_validate, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, v, flag, rowmax);
     check(hipGetLastError(), "validate launch");
 }
+
+template <class T>
+void validate_inverse_async(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax, T* inv,
+                            int* flag, int* flagInv) {
+    hipLaunchKernelGGL(k_validate_inv<T>, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, v, flag, rowmax, inv,
+                       flagInv);
+    check(hipGetLastError(), "validate launch");
+}
+template void validate_inverse_async<float>(tf_ctx&, int, const int*, const int*, const double*, double*, float*,
+                                            int*, int*);
+template void validate_inverse_async<double>(tf_ctx&, int, const int*, const int*, const double*, double*, double*,
+                                             int*, int*);
 
 bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax) {
     c.clear_flags(1);

@@ -65,6 +65,7 @@ void destroy_ctx(tf_ctx* c) {
     if (!c) return;
     DeviceScope scope(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     c->pool.trim();
     for (int i = 0; i < 32; ++i)
         if (c->ws_ptr[i]) (void)hipFree(c->ws_ptr[i]);
@@ -74,6 +75,12 @@ void destroy_ctx(tf_ctx* c) {
     if (c->hscalars) (void)hipHostFree(c->hscalars);
     if (c->dflags) (void)hipFree(c->dflags);
     if (c->hflags) (void)hipHostFree(c->hflags);
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipStreamDestroy(c->side);
+        (void)hipEventDestroy(c->ev_fork);
+        (void)hipEventDestroy(c->ev_join);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -794,6 +794,7 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
     bool factored = false, singular = false;
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(c_->stream);
+        if (c_->side) (void)hipStreamSynchronize(c_->side);  // (a coarse LU may still run there)
         for (auto& l : next) free_level(*c_, l, f32);
         dev_free(*c_, lu);
         dev_free(*c_, piv);
@@ -813,8 +814,6 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
             // (diagonal); they come back with the strength graph's size, or
             // in a read of their own on the coarsest level.
             c_->clear_flags(3);
-            validate_rowmax_async(*c_, ln, native.rp, native.ci, native.v, rowmax_workspace(*c_, ln),
-                                  c_->dflags + 1);
             auto level_checks = [&]() {
                 if (c_->hflags[1])
                     throw Error(TF_E_NUMERIC,
@@ -833,12 +832,14 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
             L.x = dev_alloc<char>(*c_, size_t(ln) * es);
             L.x2 = dev_alloc<char>(*c_, size_t(ln) * es);
             L.r = dev_alloc<char>(*c_, size_t(ln) * es);
+            // row checks, row maxima and the inverse diagonal in one pass
+            // (the level's rows are canonical: one diagonal entry each)
             if (f32)
-                inverse_diagonal<float>(*c_, ln, native.rp, native.ci, static_cast<const float*>(L.vals),
-                                        static_cast<float*>(L.inv), c_->dflags + 2);
+                validate_inverse_async<float>(*c_, ln, native.rp, native.ci, native.v, rowmax_workspace(*c_, ln),
+                                              static_cast<float*>(L.inv), c_->dflags + 1, c_->dflags + 2);
             else
-                inverse_diagonal<double>(*c_, ln, native.rp, native.ci, native.v, static_cast<double*>(L.inv),
-                                         c_->dflags + 2);
+                validate_inverse_async<double>(*c_, ln, native.rp, native.ci, native.v, rowmax_workspace(*c_, ln),
+                                               static_cast<double*>(L.inv), c_->dflags + 1, c_->dflags + 2);
             if (ln <= opt_.coarseRows || depth + 1 == opt_.maxLevels) {
                 c_->read_flags(3);
                 level_checks();
@@ -894,17 +895,19 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
             for (int k = 0; k < 4; ++k) L.hist[k] = t.hist[k];  // (filled below when copied asynchronously)
             native = galerkin(*c_, L.native, t.map, t.nc, t.offsets, t.members);
         }
+        // The coarse LU runs on the side stream while the levels' sweep
+        // storage is built here (it reads only the coarsest matrix).
         const DevCsr& last = next.back().native;
-        if (last.n <= std::max(64, opt_.coarseRows)) {
+        const bool luPending = last.n <= std::max(64, opt_.coarseRows);
+        if (luPending) {
             lu = dev_alloc<double>(*c_, size_t(last.n) * size_t(last.n));
             inv = dev_alloc<double>(*c_, size_t(last.n) * size_t(last.n));
             piv = dev_alloc<int>(*c_, size_t(last.n));
-            factored = coarse_lu_factor(*c_, last.n, last.rp, last.ci, last.v, lu, piv, inv);
-            singular = !factored;
+            coarse_lu_launch(*c_, last.n, last.rp, last.ci, last.v, lu, piv, inv);
         }
         for (size_t d = 0; d < next.size(); ++d) {
             AmgLevel& L = next[d];
-            if (L.native.n < kSellMinRows || (factored && d + 1 == next.size())) continue;
+            if (L.native.n < kSellMinRows || (luPending && d + 1 == next.size())) continue;
             // The finest operator of a structured grid sweeps in the stencil
             // storage (coarsened levels never have one offset list).
             double* st64 = nullptr;
@@ -950,6 +953,10 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
                 dev_free(*c_, L.svals);
                 L.svals = nullptr;
             }
+        }
+        if (luPending) {
+            factored = coarse_lu_join(*c_);
+            singular = !factored;
         }
         c_->sync();
         for (size_t d = 0; d + 1 < next.size() && d < size_t(kMaxHistLevels); ++d)

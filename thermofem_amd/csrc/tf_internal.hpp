@@ -128,6 +128,19 @@ struct tf_ctx {
     void* ws_ptr[32] = {};     // grow-only setup workspace slots
     size_t ws_bytes[32] = {};
 
+    // Side stream for setup work that overlaps the main stream (default
+    // priority, created on first use) and the events that fork and join it.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t side_stream() {
+        if (!side) {
+            tf::check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreate side");
+            tf::check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreate fork");
+            tf::check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreate join");
+        }
+        return side;
+    }
+
     void sync() { tf::check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
     // Grow-only scratch for scans.
     int* scan_scratch(size_t ints) {
@@ -273,6 +286,14 @@ template <class T>
 void coarse_lu_solve(tf_ctx& c, int n, const double* lu, const int* piv, const T* b, T* x);
 bool coarse_lu_factor(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* lu, int* piv,
                       double* inverse);
+// coarse_lu_factor queued on the context's side stream behind the work
+// queued so far on its main stream, so later main-stream work overlaps it;
+// coarse_lu_join makes the main stream wait for it and returns whether the
+// factorisation succeeded.  Nothing the factorisation reads or writes may be
+// freed or changed in between.
+void coarse_lu_launch(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* lu, int* piv,
+                      double* inverse);
+bool coarse_lu_join(tf_ctx& c);
 
 // ---- aggregation setup (tf_agg.hip) ----
 // Device CSR owned by the caller (hipMalloc'd arrays, released by free_csr).
@@ -393,6 +414,11 @@ bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const doubl
 // becomes nonzero when a row fails.
 void validate_rowmax_async(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax,
                            int* flag);
+// validate_rowmax_async fused with the inverse diagonal of a canonical matrix
+// in precision T (flagInv raised as inverse_diagonal raises its flag).
+template <class T>
+void validate_inverse_async(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax, T* inv,
+                            int* flag, int* flagInv);
 // symmetric: -1 unknown (checked here), 0/1 known pattern symmetry.
 // prevalidated: the caller ran validate_rowmax with rowmax_workspace(c, n).
 // With deferred = k > 0 the graph's size comes back through

@@ -24,12 +24,13 @@
 // elimination to rounding.
 //
 // Orders up to kLuSmall (the coarsest levels of typical hierarchies) take a
-// latency-lean path instead: one 1024-thread workgroup holds the matrix and
-// the inverse in LDS, a group of kColLanes lanes owns each column, and the
-// unblocked elimination runs with one workgroup barrier per pivot (the pivot
-// column's group searches, swaps and divides; every other group then swaps
-// and updates its own column).  The inverse columns are solved by the same
-// groups with no barrier at all (a column's lanes share a wave).  Its
+// latency-lean path instead: one 768-thread workgroup, a group of kColLanes
+// lanes holding each column in registers, and the unblocked elimination with
+// one workgroup barrier per pivot (the pivot column's group searches, swaps
+// and divides, publishing the multipliers in LDS; every other group then
+// swaps and updates its own column).  The inverse columns are solved by the
+// same groups in registers, reading the factors from LDS, with no barrier at
+// all (a column's lanes share a wave).  Its
 // arithmetic is the sequential algorithm's operation for operation, so the
 // factors and inverse columns equal the unblocked elimination's bit for bit.
 #include "tf_device.hpp"
@@ -45,9 +46,10 @@ constexpr int kFactorThreads = 512;
 constexpr int kInvThreads = 256;  // one workgroup per 16 inverse columns
 constexpr int kTouched = 2 * kNb; // rows a panel's permutation can move
 constexpr int kStrip = 8;         // trailing-update tiles per wave and round
-constexpr int kLuSmall = 96;      // orders factored and inverted in one workgroup's LDS (2 x 72 KB)
-constexpr int kSmallThreads = 1024;
+constexpr int kLuSmall = 96;      // orders factored and inverted by one workgroup (factors in LDS, 72 KB)
+constexpr int kSmallThreads = 768;  // 12 waves: one lane group per column of order kLuSmall
 constexpr int kColLanes = 8;      // lanes of a column group (kSmallThreads / kColLanes >= kLuSmall)
+constexpr int kSlots = kLuSmall / kColLanes;  // rows of a column per lane
 static_assert(kSmallThreads / kColLanes >= kLuSmall, "one lane group per column");
 
 // acc(16x16) += A(16x4) B(4x16): lane l supplies A[l % 16][l / 16] and
@@ -321,7 +323,9 @@ __global__ __launch_bounds__(kFactorThreads) void k_lu_factor(int n, double* __r
 // ascending below / descending above, each update then the division); the
 // blocks off the diagonal are applied as 16x16x16 matrix-core products.
 __global__ __launch_bounds__(kInvThreads) void k_lu_inverse(int n, const double* __restrict__ lu,
-                                                            const int* __restrict__ piv, double* __restrict__ inv) {
+                                                            const int* __restrict__ piv, double* __restrict__ inv,
+                                                            const int* failed) {
+    if (failed && *failed) return;  // (queued without a host check: no valid factors)
     __shared__ double X[kCoarseMax * kNb];  // row-major, 16 columns
     __shared__ int pv[kCoarseMax];
     const int tid = int(threadIdx.x);
@@ -465,16 +469,33 @@ __global__ __launch_bounds__(kInvThreads) void k_lu_inverse(int n, const double*
     }
 }
 
+// Value of row r of a column held by a lane group (row r in register r / 8
+// of lane r % 8), returned to every lane of the group.
+__device__ __forceinline__ double group_row(const double (&x)[kSlots], int r, int base) {
+    double mine = x[0];
+#pragma unroll
+    for (int q = 1; q < kSlots; ++q) mine = q == r / kColLanes ? x[q] : mine;
+    return __shfl(mine, base + r % kColLanes, kWave);
+}
+
+// Sets row r of the group's column to val (in the lane holding it).
+__device__ __forceinline__ void group_set(double (&x)[kSlots], int r, int sub, double val) {
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) x[q] = (q == r / kColLanes && sub == r % kColLanes) ? val : x[q];
+}
+
 // Factor and (when inv is given) invert an order-n matrix, n <= kLuSmall,
-// from CSR in one workgroup.  Lane group g = tid / kColLanes owns column g;
-// its lanes take the rows i = sub, sub + kColLanes, ...  Step k: group k
-// finds the first maximal |a_ik| (i >= k) with the pivot rule above, swaps
-// a_kk and a_pk, and replaces a_ik (i > k) by a_ik / a_kk; after the
-// barrier every other group swaps rows k and p of its column and, right of
-// the pivot, subtracts l_i * a_kj from a_ij (rounded product, then the
-// difference).  A group's step-k update of its column precedes its own
-// pivot work at step k + 1 in program order, and no group touches another's
-// column, so one barrier per step orders everything.
+// from CSR in one workgroup.  Lane group g = tid / kColLanes owns column g
+// in registers: lane sub holds rows sub + kColLanes q (q < kSlots).  Step k:
+// group k finds the first maximal |a_ik| (i >= k) with the pivot rule above,
+// swaps rows k and p of its column, divides the rows below by the pivot and
+// publishes those multipliers and p through LDS; after the barrier every
+// other group swaps rows k and p of its column and, right of the pivot,
+// subtracts l_i * a_kj from a_ij (rounded product, then the difference).  A
+// group's step-k update precedes its own pivot work at step k + 1 in program
+// order, and the multipliers alternate between two buffers, so one barrier
+// per step orders everything.  The inverse columns are solved in registers
+// by the same groups, reading L and U from LDS, with no barrier.
 __global__ __launch_bounds__(kSmallThreads) void k_lu_small(int n, const int* __restrict__ rp,
                                                             const int* __restrict__ ci,
                                                             const double* __restrict__ v, double* __restrict__ lu,
@@ -482,14 +503,14 @@ __global__ __launch_bounds__(kSmallThreads) void k_lu_small(int n, const int* __
                                                             int* bad) {
 #pragma clang fp contract(off)
     __shared__ double M[kLuSmall * kLuSmall];  // row-major, leading dimension n
-    __shared__ double X[kLuSmall * kLuSmall];  // inverse columns, row-major
+    __shared__ double lcol[2][kLuSmall + 1];   // multipliers of the step's pivot column (+ a discard slot)
     __shared__ int pk[kLuSmall];
     __shared__ int rowsrc[kLuSmall];
     __shared__ double sv[kSmallThreads / kWave];
     __shared__ int failed;
     const int tid = int(threadIdx.x);
     const int lane = tid & (kWave - 1), w = tid / kWave;
-    const int g = tid / kColLanes, sub = tid % kColLanes;
+    const int g = tid / kColLanes, sub = tid % kColLanes, base = lane - sub;
     for (int i = tid; i < n * n; i += kSmallThreads) M[i] = 0.0;
     if (tid == 0) failed = 0;
     __syncthreads();
@@ -507,40 +528,50 @@ __global__ __launch_bounds__(kSmallThreads) void k_lu_small(int n, const int* __
 #pragma unroll
     for (int i = 0; i < kSmallThreads / kWave; ++i) scale = fmax(scale, sv[i]);
     const double thr = 64 * 2.2204460492503131e-16 * scale;
+    const int gc = min(g, n - 1);  // (groups past n keep a copy of the last column; never stored)
+    double x[kSlots];
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) x[q] = M[min(sub + kColLanes * q, n - 1) * n + gc];
 
     for (int k = 0; k < n; ++k) {
+        double* lk = lcol[k & 1];
         if (g == k) {
             double bv = -1.0;
             int bi = -1;
-            for (int i = k + sub; i < n; i += kColLanes) {
-                const double a = fabs(M[i * n + k]);
-                const bool take = a > bv;  // strict: NaN never wins, the first maximum stays
+#pragma unroll
+            for (int q = 0; q < kSlots; ++q) {
+                const int i = sub + kColLanes * q;
+                const double a = fabs(x[q]);
+                const bool take = i >= k && i < n && a > bv;  // strict: NaN never wins, the first maximum stays
                 bv = take ? a : bv;
                 bi = take ? i : bi;
             }
-#pragma unroll
-            for (int o = 1; o < kColLanes; o <<= 1) {
-                const double ov = __shfl_xor(bv, o, kWave);
-                const int oi = __shfl_xor(bi, o, kWave);
-                const bool take = (oi >= 0) & ((bi < 0) | (ov > bv) | ((ov == bv) & (oi < bi)));
-                bv = take ? ov : bv;
-                bi = take ? oi : bi;
-            }
-            const int p = (bi < 0 || isnan(M[k * n + k])) ? k : bi;
-            const double pv = M[p * n + k];
+            // the group's 8 lanes: quad exchanges, then the half-row mirror
+            argmax_dpp<0xB1, 0xF>(bv, bi);
+            argmax_dpp<0x4E, 0xF>(bv, bi);
+            argmax_dpp<0x141, 0xF>(bv, bi);
+            const double akk = group_row(x, k, base);
+            const int p = (bi < 0 || isnan(akk)) ? k : bi;
+            const double pv = group_row(x, p, base);
             if (!isfinite(pv) || fabs(pv) <= thr) {
                 if (sub == 0) failed = 1;
             }
             else {
-                const double akk = M[k * n + k];
-                wave_sync();
-                if (sub == 0) {
-                    pk[k] = p;
-                    M[p * n + k] = akk;
-                    M[k * n + k] = pv;
+                group_set(x, p, sub, akk);
+                group_set(x, k, sub, pv);
+                // every slot divides (the quotients go out together); rows
+                // outside (k, n) keep their value and store to the discard slot
+                double l[kSlots];
+#pragma unroll
+                for (int q = 0; q < kSlots; ++q) l[q] = x[q] / pv;
+#pragma unroll
+                for (int q = 0; q < kSlots; ++q) {
+                    const int i = sub + kColLanes * q;
+                    const bool below = unsigned(i - k - 1) < unsigned(n - k - 1);
+                    x[q] = below ? l[q] : x[q];
+                    lk[below ? i : kLuSmall] = l[q];
                 }
-                wave_sync();
-                for (int i = k + 1 + sub; i < n; i += kColLanes) M[i * n + k] = M[i * n + k] / pv;
+                if (sub == 0) pk[k] = p;
             }
         }
         __syncthreads();
@@ -550,27 +581,34 @@ __global__ __launch_bounds__(kSmallThreads) void k_lu_small(int n, const int* __
         }
         if (g < n && g != k) {
             const int p = pk[k];
-            if (p != k && sub == 0) {
-                const double t = M[k * n + g];
-                M[k * n + g] = M[p * n + g];
-                M[p * n + g] = t;
-            }
-            wave_sync();
+            const double vk = group_row(x, k, base), vp = group_row(x, p, base);
+            group_set(x, p, sub, vk);
+            group_set(x, k, sub, vp);
             if (g > k) {
-                const double u = M[k * n + g];
-                for (int i = k + 1 + sub; i < n; i += kColLanes) {
-                    const double prod = M[i * n + k] * u;
-                    M[i * n + g] = M[i * n + g] - prod;
+                double l[kSlots];
+#pragma unroll
+                for (int q = 0; q < kSlots; ++q) l[q] = lk[min(sub + kColLanes * q, n - 1)];
+#pragma unroll
+                for (int q = 0; q < kSlots; ++q) {
+                    const int i = sub + kColLanes * q;
+                    const double prod = l[q] * vp;
+                    const double y = x[q] - prod;
+                    x[q] = unsigned(i - k - 1) < unsigned(n - k - 1) ? y : x[q];
                 }
             }
         }
     }
-    __syncthreads();
     int nonfinite = 0;
-    for (int i = tid; i < n * n; i += kSmallThreads) {
-        const double x = M[i];
-        nonfinite |= isfinite(x) ? 0 : 1;
-        lu[i] = x;
+    if (g < n) {
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) {
+            const int i = sub + kColLanes * q;
+            if (i < n) {
+                nonfinite |= isfinite(x[q]) ? 0 : 1;
+                M[i * n + g] = x[q];
+                lu[size_t(i) * n + g] = x[q];
+            }
+        }
     }
     for (int i = tid; i < n; i += kSmallThreads) piv[i] = pk[i];
     if (__syncthreads_or(nonfinite)) {
@@ -591,36 +629,45 @@ __global__ __launch_bounds__(kSmallThreads) void k_lu_small(int n, const int* __
         }
     }
     __syncthreads();
-    for (int i = tid; i < n * n; i += kSmallThreads) {
-        const int r = i / n, c = i - r * n;
-        X[i] = rowsrc[r] == c ? 1.0 : 0.0;
-    }
-    __syncthreads();
     if (g < n) {
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) {
+            const int i = sub + kColLanes * q;
+            x[q] = i < n && rowsrc[i] == g ? 1.0 : 0.0;
+        }
         for (int k = 0; k < n; ++k) {
-            wave_sync();
-            const double xk = X[k * n + g];
-            for (int i = k + 1 + sub; i < n; i += kColLanes) {
-                const double prod = M[i * n + k] * xk;
-                X[i * n + g] = X[i * n + g] - prod;
+            double l[kSlots];
+#pragma unroll
+            for (int q = 0; q < kSlots; ++q) l[q] = M[min(sub + kColLanes * q, n - 1) * n + k];
+            const double xk = group_row(x, k, base);
+#pragma unroll
+            for (int q = 0; q < kSlots; ++q) {
+                const int i = sub + kColLanes * q;
+                const double prod = l[q] * xk;
+                const double y = x[q] - prod;
+                x[q] = unsigned(i - k - 1) < unsigned(n - k - 1) ? y : x[q];
             }
         }
         for (int k = n - 1; k >= 0; --k) {
-            wave_sync();
-            const double xk = X[k * n + g] / M[k * n + k];
-            wave_sync();
-            if (k % kColLanes == sub) X[k * n + g] = xk;
-            for (int i = sub; i < k; i += kColLanes) {
-                const double prod = M[i * n + k] * xk;
-                X[i * n + g] = X[i * n + g] - prod;
+            double u[kSlots];
+#pragma unroll
+            for (int q = 0; q < kSlots; ++q) u[q] = M[min(sub + kColLanes * q, n - 1) * n + k];
+            const double xk = group_row(x, k, base) / M[k * n + k];
+            group_set(x, k, sub, xk);
+#pragma unroll
+            for (int q = 0; q < kSlots; ++q) {
+                const int i = sub + kColLanes * q;
+                const double prod = u[q] * xk;
+                const double y = x[q] - prod;
+                x[q] = i < k ? y : x[q];
             }
         }
-    }
-    __syncthreads();
-    // column-major output: column j is contiguous
-    for (int i = tid; i < n * n; i += kSmallThreads) {
-        const int j = i / n, r = i - j * n;
-        inv[i] = X[r * n + j];
+        // column-major output: column g is contiguous
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) {
+            const int i = sub + kColLanes * q;
+            if (i < n) inv[size_t(g) * n + i] = x[q];
+        }
     }
 }
 
@@ -645,10 +692,44 @@ bool coarse_lu_factor(tf_ctx& c, int n, const int* rp, const int* ci, const doub
     const bool ok = c.hflags[0] == 0;
     if (ok && inverse) {
         hipLaunchKernelGGL(k_lu_inverse, dim3((n + kNb - 1) / kNb), dim3(kInvThreads), 0, c.stream, n, lu, piv,
-                           inverse);
+                           inverse, static_cast<const int*>(nullptr));
         check(hipGetLastError(), "lu inverse launch");
     }
     return ok;
+}
+
+// The factorisation's verdict slot in the context's flags (apart from the
+// low slots the main stream's setup kernels use meanwhile).
+constexpr int kLuFlagSlot = 48;
+
+void coarse_lu_launch(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* lu, int* piv,
+                      double* inverse) {
+    require(n >= 1 && n <= kCoarseMax, "GPU LU requires order 1..1024");
+    hipStream_t s = c.side_stream();
+    int* flag = c.dflags + kLuFlagSlot;
+    check(hipMemsetAsync(flag, 0, sizeof(int), c.stream), "lu flag clear");
+    check(hipEventRecord(c.ev_fork, c.stream), "fork record");
+    check(hipStreamWaitEvent(s, c.ev_fork, 0), "fork wait");
+    if (n <= kLuSmall) {
+        hipLaunchKernelGGL(k_lu_small, dim3(1), dim3(kSmallThreads), 0, s, n, rp, ci, v, lu, piv, inverse, flag);
+    }
+    else {
+        hipLaunchKernelGGL(k_dense_fill, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, rp, ci, v, lu);
+        hipLaunchKernelGGL(k_lu_factor, dim3(1), dim3(kFactorThreads), 0, s, n, lu, piv, flag);
+        if (inverse)
+            hipLaunchKernelGGL(k_lu_inverse, dim3((n + kNb - 1) / kNb), dim3(kInvThreads), 0, s, n, lu, piv, inverse,
+                               static_cast<const int*>(flag));
+    }
+    check(hipGetLastError(), "lu launch");
+    check(hipEventRecord(c.ev_join, s), "join record");
+}
+
+bool coarse_lu_join(tf_ctx& c) {
+    check(hipStreamWaitEvent(c.stream, c.ev_join, 0), "join wait");
+    check(hipMemcpyAsync(c.hflags + kLuFlagSlot, c.dflags + kLuFlagSlot, sizeof(int), hipMemcpyDeviceToHost, c.stream),
+          "lu flag download");
+    c.sync();
+    return c.hflags[kLuFlagSlot] == 0;
 }
 
 }  // namespace tf
