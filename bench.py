@@ -461,8 +461,10 @@ def fem_block(args):
 def batch(args):
     """32 independent log-normal heat problems, cube edges ~ U{16..96} (seed
     3), as one block-diagonal matrix: one AMG setup + segmented CG (FP32
-    V(2,2) cycle, FP64 CG, to 1e-8 per system) against 32 separate
-    setup + solve runs of the same solver configuration."""
+    V(2,2) cycle, FP64 CG, to 1e-8 per system), the solve as per-kernel
+    launches (`value`: captured iterations replayed between control-block
+    reads) and as one persistent launch, against 32 separate setup + solve
+    runs of the same solver configuration."""
     import numpy as np
     from thermofem_amd import problems as P
     from thermofem_amd.device import DeviceArray, DeviceCsr
@@ -476,25 +478,37 @@ def batch(args):
     dx = DeviceArray(n, np.float64)
     amg = AmgOptions(coarse_rows=256, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0, level_precision=32)
     s = Cg(KrylovOptions(relative_tolerance=1e-8, use_graph=not args.no_graph), amg)
-    times, res = [], None
-    for it in range(args.warmup + args.steps):
-        dx.upload(np.zeros(n))
-        t0 = time.perf_counter()
-        s.setup_device(n, nnz, *A.args())
-        res = s.apply_segmented_device(off, db, dx)
-        t1 = time.perf_counter()
-        if it >= args.warmup:
-            times.append(t1 - t0)
-    x = dx.get()
-    worst = 0.0
-    for k in range(len(edges)):
-        lo, hi = off[k], off[k + 1]
-        rs = b[lo:hi] - csr_matvec(rp[lo:hi + 1] - rp[lo], ci[rp[lo]:rp[hi]] - lo, v[rp[lo]:rp[hi]], x[lo:hi])
-        worst = max(worst, float(np.linalg.norm(rs) / np.linalg.norm(b[lo:hi])))
-    its = [r.iterations for r in res]
-    seg = {"s": round(statistics.median(times), 6), "iterations_min_max": [min(its), max(its)],
-           "all_converged": all(r.status == SolveStatus.Converged for r in res), "worst_true_rel_residual": worst}
-    log(f"segmented: {seg}")
+
+    def worst_of(x):
+        worst = 0.0
+        for k in range(len(edges)):
+            lo, hi = off[k], off[k + 1]
+            rs = b[lo:hi] - csr_matvec(rp[lo:hi + 1] - rp[lo], ci[rp[lo]:rp[hi]] - lo, v[rp[lo]:rp[hi]], x[lo:hi])
+            worst = max(worst, float(np.linalg.norm(rs) / np.linalg.norm(b[lo:hi])))
+        return worst
+
+    # per-kernel launches (captured iterations replayed between control reads)
+    # and the whole solve as one persistent launch, each timed as setup + solve
+    results = {}
+    for mode in ("launched", "persistent"):
+        times, solves, res = [], [], None
+        for it in range(args.warmup + args.steps):
+            dx.upload(np.zeros(n))
+            t0 = time.perf_counter()
+            s.setup_device(n, nnz, *A.args())
+            t1 = time.perf_counter()
+            res = s.apply_segmented_device(off, db, dx, persistent=mode == "persistent")
+            t2 = time.perf_counter()
+            if it >= args.warmup:
+                times.append(t2 - t0)
+                solves.append(t2 - t1)
+        its = [r.iterations for r in res]
+        results[mode] = {"s": round(statistics.median(times), 6), "solve_s": round(statistics.median(solves), 6),
+                         "iterations_min_max": [min(its), max(its)],
+                         "all_converged": all(r.status == SolveStatus.Converged for r in res),
+                         "worst_true_rel_residual": worst_of(dx.get())}
+        log(f"segmented ({mode}): {results[mode]}")
+    seg = results["launched"]
     del s
     # the same systems one at a time (setup + solve each)
     one = Cg(KrylovOptions(relative_tolerance=1e-8, use_graph=not args.no_graph), amg)
@@ -519,7 +533,8 @@ def batch(args):
                       "warmup": args.warmup, "dtype": "f64+f32", "data": "synthetic",
                       "config": {"workload": "32 log-normal heat3d cubes, edges U{16..96} seed 3, segmented CG + "
                                              "AMG FP32 V(2,2)", "n": n, "nnz": nnz, "edges": edges.tolist()},
-                      "value": seg["s"], "segmented": seg, "separate": single,
+                      "value": seg["s"], "segmented": seg, "segmented_persistent": results["persistent"],
+                      "separate": single,
                       "speedup": round(single["s"] / seg["s"], 3)}), flush=True)
 
 

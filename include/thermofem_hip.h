@@ -336,6 +336,15 @@ int tf_amg_convergence_estimate(tf_amg* amg, double* rho);
 
 int tf_solver_apply_segmented(tf_solver* solver, int segments, const int* host_offsets, const double* b, double* x,
                               tf_solve_result* results);
+/* The same segmented CG in one persistent launch: the initial residuals,
+ * every iteration with its whole AMG cycle, each segment's stopping test and
+ * the final residuals run as phases of one cooperative grid separated by
+ * grid-wide barriers, with no host round trip until the solve ends
+ * (Cg.cuh:123-189 per segment).  Results equal tf_solver_apply_segmented's.
+ * TF_E_INVALID when the device cannot launch cooperatively or the factored
+ * coarsest level has more than 1024 rows. */
+int tf_solver_apply_segmented_persistent(tf_solver* solver, int segments, const int* host_offsets, const double* b,
+                                         double* x, tf_solve_result* results);
 /* Seconds of the last setup / apply measured with HIP events on the solver stream. */
 int tf_solver_timings(tf_solver* solver, double* setup_seconds, double* apply_seconds);
 

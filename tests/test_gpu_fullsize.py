@@ -13,7 +13,8 @@
   block CG solve reaches 1e-8 in every column, two columns against the
   oracle's CG.
 - 32 heat systems with edges U{16..96} (9,195,298 rows): the segmented solve
-  reaches 1e-8 in every segment's host-recomputed FP64 residual.
+  reaches 1e-8 in every segment's host-recomputed FP64 residual, launched
+  per kernel and as one persistent launch (same iterations and iterates).
 """
 import numpy as np
 import pytest
@@ -177,3 +178,10 @@ def test_c4_batch_32_systems_full_size():
     log(f"batch of 32 (rows {off[-1]}): iterations {min(q.iterations for q in res)}.."
         f"{max(q.iterations for q in res)}, worst host rel {worst:.3e}")
     assert worst <= 1.0001e-8
+    # the whole solve as one persistent launch: the same iterations and
+    # iterates as the per-kernel path
+    xp, rp = s.apply_segmented(off, b, persistent=True)
+    log(f"persistent launch: iterations {[q.iterations for q in rp]}")
+    for k in range(32):
+        assert rp[k].status == SolveStatus.Converged and rp[k].iterations == res[k].iterations, k
+    assert np.linalg.norm(xp - x) <= 1e-12 * np.linalg.norm(x)

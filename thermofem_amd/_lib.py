@@ -157,6 +157,7 @@ _SIGS = {
     "tf_solver_apply": (i32, [vp, vp, vp, i32, C.POINTER(SolveResultC)]),
     "tf_solver_apply_block": (i32, [vp, i32, vp, vp, C.POINTER(SolveResultC)]),
     "tf_solver_apply_segmented": (i32, [vp, i32, vp, vp, vp, C.POINTER(SolveResultC)]),
+    "tf_solver_apply_segmented_persistent": (i32, [vp, i32, vp, vp, vp, C.POINTER(SolveResultC)]),
     "tf_gnn_param_count": (i32, []),
     "tf_gnn_create": (i32, [i32, C.POINTER(vp)]),
     "tf_gnn_destroy": (i32, [vp]),

@@ -982,6 +982,22 @@ int tf_solver_apply_segmented(tf_solver* s, int segments, const int* host_offset
     });
 }
 
+int tf_solver_apply_segmented_persistent(tf_solver* s, int segments, const int* host_offsets, const double* b,
+                                         double* x, tf_solve_result* results) {
+    return guard([&] {
+        require(s != nullptr && results != nullptr && host_offsets != nullptr, "null handle");
+        require(b != nullptr && x != nullptr && static_cast<const void*>(b) != x,
+                "segmented solve needs distinct device vectors");
+        DeviceScope scope(s->ctx->device);
+        Events ev;
+        check(hipEventRecord(ev.a, s->ctx->stream), "event record");
+        s->eng->apply_segmented_persistent(segments, host_offsets, b, x, results);
+        check(hipEventRecord(ev.b, s->ctx->stream), "event record");
+        check(hipEventSynchronize(ev.b), "event sync");
+        s->eng->applySeconds = elapsed(ev.a, ev.b);
+    });
+}
+
 // ------------------------------------------------------- learned affinities
 
 int tf_gnn_param_count(void) { return GnnModel::param_count(); }

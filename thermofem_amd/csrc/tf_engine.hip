@@ -1114,6 +1114,20 @@ struct TailGen {
 };
 }  // namespace
 
+// The whole cycle of apply() as one operation list, from level 0 with its
+// first sweep applied at entry when apply() fuses it there; curOut / zeroOut
+// receive the level-0 iterate's buffer and whether it is still zero.
+std::vector<TailOp> AmgEngine::cycle_program(bool& firstAtEntry, int& curOut, bool& zeroOut) const {
+    TailGen g{levels_, opt_, factored_, std::vector<char>(levels_.size(), 0), std::vector<char>(levels_.size(), 0),
+              std::vector<int>(levels_.size(), 0), std::vector<int64_t>(levels_.size(), 0), {}};
+    firstAtEntry = opt_.cycle != 2 && levels_.size() > 1 && opt_.pre >= 1;
+    g.pre[0] = firstAtEntry;
+    g.cycle(opt_.cycle, 0, true);
+    curOut = g.cur[0];
+    zeroOut = g.zero[0] != 0;
+    return g.ops;
+}
+
 // Chooses the tail's first level and records its programs for every entry
 // state (cycle kind, zero start, first sweep already applied by the parent's
 // restriction).

@@ -181,6 +181,12 @@ public:
     // captured graph of the cycle is valid only for the generation it was
     // captured at.
     uint64_t generation() const { return generation_; }
+    // apply()'s whole cycle as one operation list (tail-program form, from
+    // level 0): firstAtEntry says whether level 0's first sweep from zero is
+    // applied when the rhs enters; curOut / zeroOut give the buffer that holds
+    // the level-0 iterate at the end and whether it is still zero.
+    std::vector<TailOp> cycle_program(bool& firstAtEntry, int& curOut, bool& zeroOut) const;
+    bool factored() const { return factored_; }
 
     std::vector<AmgLevel> levels_;
     std::vector<int64_t> visits_;
@@ -303,6 +309,10 @@ public:
     // Independent systems on the diagonal blocks of the operator (CG only):
     // segment s = rows [off[s], off[s+1]) (host offsets); out[segments].
     void apply_segmented(int segments, const int* off, const double* b, double* x, tf_solve_result* out);
+    // The same solve as apply_segmented, every iteration and the whole cycle
+    // in one cooperative launch with grid-wide barriers between phases.
+    void apply_segmented_persistent(int segments, const int* off, const double* b, double* x,
+                                    tf_solve_result* out);
     AmgEngine* amg() { return amg_; }
     int n() const { return n_; }
     double setupSeconds = 0, applySeconds = 0;
@@ -317,6 +327,12 @@ private:
     void launch_block_iteration(int k);
     struct Segments;
     void launch_segment_iteration();
+    // Checks the offsets and (re)makes the tile table and work buffers of a
+    // segment layout.
+    void prepare_segments(int segments, const int* off);
+    // Results from the control blocks and the final residuals' squares.
+    void segment_results(const std::vector<double>& normb, const std::vector<double>& tol,
+                         const std::vector<double>& rr, tf_solve_result* out);
     template <class R>
     void launch_iteration(const R& rows, int nb);
     tf_solve_result finish(int status, int iterations);

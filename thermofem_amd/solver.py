@@ -303,21 +303,23 @@ class _Krylov:
             _raise(e)
         return [SolveResult(SolveStatus(q.status), q.iterations, q.residual_norm, q.relative_residual) for q in r]
 
-    def apply_segmented(self, offsets, b, x0=None):
+    def apply_segmented(self, offsets, b, x0=None, persistent=False):
         """CG on the independent diagonal blocks [offsets[s], offsets[s+1]) of
-        the set-up matrix at once (host arrays): (x, [SolveResult per segment])."""
+        the set-up matrix at once (host arrays): (x, [SolveResult per segment]).
+        persistent: the whole solve in one cooperative launch."""
         b = np.ascontiguousarray(b, np.float64)
         x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64)
         db, dx = DeviceArray.from_host(b), DeviceArray.from_host(x)
-        res = self.apply_segmented_device(offsets, db, dx)
+        res = self.apply_segmented_device(offsets, db, dx, persistent)
         return dx.get(), res
 
-    def apply_segmented_device(self, offsets, b, x):
+    def apply_segmented_device(self, offsets, b, x, persistent=False):
         off = np.ascontiguousarray(offsets, np.int32)
         S = len(off) - 1
         r = (_lib.SolveResultC * max(S, 1))()
+        fn = lib().tf_solver_apply_segmented_persistent if persistent else lib().tf_solver_apply_segmented
         try:
-            check(lib().tf_solver_apply_segmented(self.handle, S, off.ctypes.data_as(C.c_void_p), b.ptr, x.ptr, r))
+            check(fn(self.handle, S, off.ctypes.data_as(C.c_void_p), b.ptr, x.ptr, r))
         except TfError as e:
             _raise(e)
         return [SolveResult(SolveStatus(q.status), q.iterations, q.residual_norm, q.relative_residual)
