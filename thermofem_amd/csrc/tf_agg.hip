@@ -152,41 +152,49 @@ __global__ void k_validate(int n, const int* __restrict__ rp, const int* __restr
 
 // k_validate fused with the level's inverse diagonal in its precision T
 // (canonical rows: the diagonal is one entry, inv = 1 / T(a_ii)), raising
-// bad2 as k_inverse_diagonal would.
-template <class T>
+// bad2 as k_inverse_diagonal would.  G lanes per row, one entry per lane
+// (coalesced loads); the row maximum is a max (exact in any order) and the
+// diagonal comes from the lane that holds it.
+template <class T, int G>
 __global__ void k_validate_inv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                const double* __restrict__ a, int* bad, double* __restrict__ rowmax,
                                T* __restrict__ inv, int* bad2) {
-    const int i = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (i >= n) return;
-    double diag = 0, mx = 0;
-    bool invalid = false;
-    const int e = rp[i + 1];
-    for (int j0 = rp[i]; j0 < e; j0 += 8) {
-        int cc[8];
-        double aa[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int j = min(j0 + u, e - 1);
-            cc[u] = ci[j];
-            aa[u] = a[j];
+    const int lane = int(threadIdx.x) % G;
+    const int groups = int(gridDim.x) * (kBlock / G);
+    for (int i = int((int64_t(blockIdx.x) * kBlock + threadIdx.x) / G); i < n; i += groups) {
+        double diag = 0, mx = 0;
+        int invalid = 0, found = 0;
+        const int e = rp[i + 1];
+        for (int j = rp[i] + lane; j < e; j += G) {
+            const int c = ci[j];
+            const double av = a[j];
+            invalid |= isfinite(av) ? 0 : 1;
+            if (c == i) {
+                diag = av;
+                found = 1;
+            }
+            else {
+                mx = fmax(mx, fabs(av));
+            }
         }
+        int who = found ? lane : G;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            if (j0 + u >= e) break;
-            invalid |= !isfinite(aa[u]);
-            if (cc[u] == i)
-                diag = aa[u];
-            else
-                mx = fmax(mx, fabs(aa[u]));
+        for (int o = G / 2; o > 0; o >>= 1) {
+            mx = fmax(mx, __shfl_xor(mx, o, G));
+            invalid |= __shfl_xor(invalid, o, G);
+            who = min(who, __shfl_xor(who, o, G));
         }
+        const int base = (int(threadIdx.x) & (kWave - 1)) / G * G;
+        diag = __shfl(diag, base + (who < G ? who : 0), kWave);
+        if (who == G) diag = 0;
+        if (lane != 0) continue;
+        if (invalid || diag == 0 || !isfinite(diag) || !isfinite(1.0 / diag)) atomicOr(bad, 1);
+        rowmax[i] = mx;
+        const T d = T(diag);
+        const T r = T(1) / d;
+        if (!isfinite(d) || d == T(0) || !isfinite(r)) atomicOr(bad2, 1);
+        inv[i] = r;
     }
-    if (invalid || diag == 0 || !isfinite(diag) || !isfinite(1.0 / diag)) atomicOr(bad, 1);
-    rowmax[i] = mx;
-    const T d = T(diag);
-    const T r = T(1) / d;
-    if (!isfinite(d) || d == T(0) || !isfinite(r)) atomicOr(bad2, 1);
-    inv[i] = r;
 }
 
 // Strength edges (GpuAggregation.cu:339-364): w = |a_ij| / rowmax_i for
@@ -234,19 +242,6 @@ __global__ void k_strength_fill(int n, const int* __restrict__ rp, const int* __
     }
 }
 
-// Position of column `col` in the sorted row [b, e), or -1.
-__device__ __forceinline__ int find_col(const int* __restrict__ ci, int b, int e, int col) {
-    while (b < e) {
-        const int m = (b + e) >> 1;
-        const int c = ci[m];
-        if (c == col) return m;
-        if (c < col)
-            b = m + 1;
-        else
-            e = m;
-    }
-    return -1;
-}
 
 // Flags a canonical matrix whose pattern is not symmetric.
 __global__ void k_struct_sym(int n, const int* __restrict__ rp, const int* __restrict__ ci, int* flag) {
@@ -254,7 +249,7 @@ __global__ void k_struct_sym(int n, const int* __restrict__ rp, const int* __res
     if (i >= n) return;
     for (int j = rp[i]; j < rp[i + 1]; ++j) {
         const int c = ci[j];
-        if (c != i && find_col(ci, rp[c], rp[c + 1], i) < 0) {
+        if (c != i && find_sorted_col(ci, rp[c], rp[c + 1], i) < 0) {
             atomicOr(flag, 1);
             return;
         }
@@ -268,41 +263,99 @@ __global__ void k_struct_sym(int n, const int* __restrict__ rp, const int* __res
 // are dropped.  This is exactly the max-merge of both emitted directions.
 // One pass writes the weights in the layout of A (column -1 = dropped) and
 // the kept count per row; k_compact_rows packs them.
-template <int G>
+// VS: the values are symmetric bit for bit, so a_ji is read as a_ij and the
+// transpose is not looked up.
+template <int G, bool VS = false>
 __global__ void k_strength_sym(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                const double* __restrict__ a, const double* __restrict__ rowmax, double theta,
                                int* __restrict__ scol, double* __restrict__ sw, int* __restrict__ cnt,
                                int* nonsym) {
-    // G lanes per row, one entry per lane: the loads of a row are coalesced and
-    // the binary searches in the neighbours' rows run side by side.  With
-    // nonsym, an entry without its transpose raises the flag (the pattern
-    // was not known to be symmetric: the result is then discarded).
-    const int i = int((int64_t(blockIdx.x) * kBlock + threadIdx.x) / G);
+    // G lanes per row, one entry per lane (coalesced loads).  With VS each
+    // group takes 4 rows at once (rows g, g + groups, ...), whose loads are
+    // all issued before the first is used (level 0 of the 128^3 heat
+    // operator: 135 -> 112 us); the transpose lookups' registers leave room
+    // for one row a group otherwise.  The transpose of entry (i, c) is
+    // looked up in row c.  With nonsym, an entry without
+    // its transpose raises the flag (the pattern was not known to be
+    // symmetric: the result is then discarded).
+    constexpr int R = VS ? 4 : 1;
     const int lane = int(threadIdx.x) % G;
-    if (i == n && lane == 0) cnt[n] = 0;
-    if (i >= n) return;  // whole groups leave together
-    const double mi = rowmax[i];
-    const int s = rp[i], e = rp[i + 1];
-    int kept = 0;
-    for (int j = s + lane; j < e; j += G) {
-        const int c = ci[j];
-        double w = 0.0;
-        if (c != i) {
-            const double wi = mi > 0 ? fabs(a[j]) / mi : 0.0;
-            const double qi = (wi >= theta && wi > 0) ? wi : 0.0;
-            const double mc = rowmax[c];
-            const int k = find_col(ci, rp[c], rp[c + 1], i);
-            if (k < 0 && nonsym) atomicOr(nonsym, 1);
-            const double wc = (mc > 0 && k >= 0) ? fabs(a[k]) / mc : 0.0;
-            const double qc = (wc >= theta && wc > 0) ? wc : 0.0;
-            w = qi > qc ? qi : qc;
+    const int groups = int(gridDim.x) * (kBlock / G);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[n] = 0;
+    for (int i0 = int((int64_t(blockIdx.x) * kBlock + threadIdx.x) / G); i0 < n; i0 += groups * R) {
+        int row[R], s[R], e[R], kept[R];
+        double mi[R];
+        int len = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            row[k] = i0 + k * groups;
+            const bool valid = row[k] < n;
+            const int ii = valid ? row[k] : i0;
+            s[k] = rp[ii];
+            e[k] = valid ? rp[ii + 1] : s[k];
+            mi[k] = rowmax[ii];
+            kept[k] = 0;
+            len = max(len, e[k] - s[k]);
         }
-        scol[j] = w > 0 ? c : -1;
-        sw[j] = w;
-        kept += w > 0 ? 1 : 0;
+        for (int t0 = 0; t0 < len; t0 += G) {
+            int c[R], jj[R];
+            double av[R];
+            bool live[R];
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const int j = s[k] + t0 + lane;
+                live[k] = j < e[k];
+                jj[k] = live[k] ? j : 0;
+                c[k] = ci[jj[k]];
+                av[k] = a[jj[k]];
+            }
+            double mc[R], at[R];
+            bool has[R];
+#pragma unroll
+            for (int k = 0; k < R; ++k) mc[k] = rowmax[live[k] ? c[k] : 0];
+            if constexpr (VS) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    at[k] = av[k];
+                    has[k] = true;
+                }
+            }
+            else {
+                int pos[R];
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const int ck = live[k] ? c[k] : 0;
+                    pos[k] = live[k] && c[k] != row[k] ? find_sorted_col(ci, rp[ck], rp[ck + 1], row[k]) : -1;
+                }
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    has[k] = pos[k] >= 0;
+                    at[k] = a[has[k] ? pos[k] : 0];
+                    if (live[k] && c[k] != row[k] && !has[k] && nonsym) atomicOr(nonsym, 1);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if (!live[k]) continue;
+                double w = 0.0;
+                if (c[k] != row[k]) {
+                    const double wi = mi[k] > 0 ? fabs(av[k]) / mi[k] : 0.0;
+                    const double qi = (wi >= theta && wi > 0) ? wi : 0.0;
+                    const double wc = (mc[k] > 0 && has[k]) ? fabs(at[k]) / mc[k] : 0.0;
+                    const double qc = (wc >= theta && wc > 0) ? wc : 0.0;
+                    w = qi > qc ? qi : qc;
+                }
+                scol[jj[k]] = w > 0 ? c[k] : -1;
+                sw[jj[k]] = w;
+                kept[k] += w > 0 ? 1 : 0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int kt = group_sum<G>(kept[k]);
+            if (lane == 0 && row[k] < n) cnt[row[k]] = kt;
+        }
     }
-    kept = group_sum<G>(kept);
-    if (lane == 0) cnt[i] = kept;
 }
 
 // Learned weights in place of the strength measure: every off-diagonal
@@ -366,6 +419,14 @@ inline int lanes_per_row(int n, int64_t nnz) {
 inline dim3 grid_rows(int64_t rows, int g) {
     const int64_t threads = rows * g;
     return dim3(unsigned((threads + kBlock - 1) / kBlock > 0 ? (threads + kBlock - 1) / kBlock : 1));
+}
+// The same for grid-stride kernels: at most kRowGroupBlocks blocks (two
+// rounds of resident 256-thread blocks on 256 CUs), each group of lanes
+// taking rows i, i + groups, ...
+constexpr unsigned kRowGroupBlocks = 4096;
+inline dim3 grid_rows_capped(int64_t rows, int g) {
+    const dim3 d = grid_rows(rows, g);
+    return dim3(d.x < kRowGroupBlocks ? d.x : kRowGroupBlocks);
 }
 
 // Output row I gathers the rows of its members (all rows of `offsets`
@@ -1603,15 +1664,23 @@ void validate_rowmax_async(tf_ctx& c, int n, const int* rp, const int* ci, const
 }
 
 template <class T>
-void validate_inverse_async(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax, T* inv,
-                            int* flag, int* flagInv) {
-    hipLaunchKernelGGL(k_validate_inv<T>, grid1(n), dim3(kBlock), 0, c.stream, n, rp, ci, v, flag, rowmax, inv,
-                       flagInv);
+void validate_inverse_async(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v,
+                            double* rowmax, T* inv, int* flag, int* flagInv) {
+    if (n <= 0) return;
+    switch (lanes_per_row(n, nnz)) {
+#define TF_VALIDATE(G)                                                                                           \
+    case G:                                                                                                      \
+        hipLaunchKernelGGL((k_validate_inv<T, G>), grid_rows_capped(n, G), dim3(kBlock), 0, c.stream, n, rp, ci, v, \
+                           flag, rowmax, inv, flagInv);                                                          \
+        break;
+        TF_VALIDATE(4) TF_VALIDATE(8) TF_VALIDATE(16) TF_VALIDATE(32)
+#undef TF_VALIDATE
+    }
     check(hipGetLastError(), "validate launch");
 }
-template void validate_inverse_async<float>(tf_ctx&, int, const int*, const int*, const double*, double*, float*,
+template void validate_inverse_async<float>(tf_ctx&, int, int64_t, const int*, const int*, const double*, double*, float*,
                                             int*, int*);
-template void validate_inverse_async<double>(tf_ctx&, int, const int*, const int*, const double*, double*, double*,
+template void validate_inverse_async<double>(tf_ctx&, int, int64_t, const int*, const int*, const double*, double*, double*,
                                              int*, int*);
 
 bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax) {
@@ -1625,7 +1694,9 @@ bool validate_rowmax(tf_ctx& c, int n, const int* rp, const int* ci, const doubl
 double* rowmax_workspace(tf_ctx& c, int n) { return c.wst<double>(kRowmax, size_t(n)); }
 
 DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta,
-                      int symmetric, bool prevalidated, int* symmetricOut, const double* learned, int deferred) {
+                      int symmetric, bool prevalidated, int* symmetricOut, const double* learned, int deferred,
+                      bool valueSymmetric) {
+    if (valueSymmetric) symmetric = 1;
     require(std::isfinite(theta) && theta >= 0 && theta <= 1, "Invalid strength threshold");
     if (nnz > int64_t(0x3fffffff)) throw Error(TF_E_CAPACITY, "Native AMG: strength graph exceeds 32-bit capacity");
     double* rowmax = rowmax_workspace(c, n);
@@ -1659,10 +1730,14 @@ DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* c
                                cnt);
         }
         else {
-#define TF_STRENGTH(G)                                                                                               \
-    case G:                                                                                                          \
-        hipLaunchKernelGGL(k_strength_sym<G>, grid_rows(int64_t(n) + 1, G), dim3(kBlock), 0, c.stream, n, rp, ci, v, \
-                           rowmax, theta, scol, sw, cnt, nonsym);                                                    \
+#define TF_STRENGTH(G)                                                                                          \
+    case G:                                                                                                     \
+        if (valueSymmetric)                                                                                     \
+            hipLaunchKernelGGL((k_strength_sym<G, true>), grid_rows_capped(n, G), dim3(kBlock), 0, c.stream,     \
+                               n, rp, ci, v, rowmax, theta, scol, sw, cnt, nonsym);                             \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_strength_sym<G, false>), grid_rows(int64_t(n) + 1, G), dim3(kBlock), 0,         \
+                               c.stream, n, rp, ci, v, rowmax, theta, scol, sw, cnt, nonsym);                   \
         break;
             switch (lanes) {
                 TF_STRENGTH(4) TF_STRENGTH(8) TF_STRENGTH(16) TF_STRENGTH(32)

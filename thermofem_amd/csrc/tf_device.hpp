@@ -67,6 +67,35 @@ __device__ __forceinline__ int xcd_block(int limit) {
 
 inline int round_xcd(int blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
 
+// Position of column `col` in the sorted row [b, e) (unique columns), or
+// -1.  A row of up to 8 entries is read as one chunk whose loads are issued
+// together (indices clamped into the row; a clamped slot repeats the last
+// column, and the first matching slot is the real position), so it costs one
+// round of loads instead of a chain of dependent probes; longer rows are
+// binary-searched.
+__device__ __forceinline__ int find_sorted_col(const int* __restrict__ ci, int b, int e, int col) {
+    if (e - b <= 8) {
+        if (e <= b) return -1;
+        int cc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) cc[u] = ci[min(b + u, e - 1)];
+        int k = -1;
+#pragma unroll
+        for (int u = 7; u >= 0; --u) k = cc[u] == col ? b + u : k;
+        return k;
+    }
+    while (b < e) {
+        const int m = (b + e) >> 1;
+        const int c = ci[m];
+        if (c == col) return m;
+        if (c < col)
+            b = m + 1;
+        else
+            e = m;
+    }
+    return -1;
+}
+
 // Loads of matrix entries (values, column indices) that a launch reads
 // once.  Plain loads: on gfx950 the nontemporal hint measured slower for the
 // level-0 SpMV (38.5 against 28.4 us with the matrix coming from HBM, 35.8

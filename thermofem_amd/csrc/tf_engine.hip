@@ -835,10 +835,10 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
             // row checks, row maxima and the inverse diagonal in one pass
             // (the level's rows are canonical: one diagonal entry each)
             if (f32)
-                validate_inverse_async<float>(*c_, ln, native.rp, native.ci, native.v, rowmax_workspace(*c_, ln),
+                validate_inverse_async<float>(*c_, ln, native.nnz, native.rp, native.ci, native.v, rowmax_workspace(*c_, ln),
                                               static_cast<float*>(L.inv), c_->dflags + 1, c_->dflags + 2);
             else
-                validate_inverse_async<double>(*c_, ln, native.rp, native.ci, native.v, rowmax_workspace(*c_, ln),
+                validate_inverse_async<double>(*c_, ln, native.nnz, native.rp, native.ci, native.v, rowmax_workspace(*c_, ln),
                                                static_cast<double*>(L.inv), c_->dflags + 1, c_->dflags + 2);
             if (ln <= opt_.coarseRows || depth + 1 == opt_.maxLevels) {
                 c_->read_flags(3);
@@ -858,8 +858,11 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
             }
             DevCsr g;
             try {
+                // a finest level whose stencil storage was built mirrored has
+                // bit-symmetric values (its fill compared every transpose)
+                const bool valueSym = depth == 0 && fineStencil_ && fineStencil_->n == ln && fineStencil_->mirrored;
                 g = strength_graph(*c_, ln, native.nnz, native.rp, native.ci, native.v, opt_.theta, symmetric, true,
-                                   &symmetric, lw, 3);
+                                   &symmetric, lw, 3, valueSym);
                 level_checks();
             }
             catch (...) {

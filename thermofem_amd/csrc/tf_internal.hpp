@@ -417,16 +417,18 @@ void validate_rowmax_async(tf_ctx& c, int n, const int* rp, const int* ci, const
 // validate_rowmax_async fused with the inverse diagonal of a canonical matrix
 // in precision T (flagInv raised as inverse_diagonal raises its flag).
 template <class T>
-void validate_inverse_async(tf_ctx& c, int n, const int* rp, const int* ci, const double* v, double* rowmax, T* inv,
-                            int* flag, int* flagInv);
-// symmetric: -1 unknown (checked here), 0/1 known pattern symmetry.
+void validate_inverse_async(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v,
+                            double* rowmax, T* inv, int* flag, int* flagInv);
+// symmetric: -1 unknown (checked here), 0/1 known pattern symmetry;
+// valueSymmetric: the values are known to be symmetric bit for bit (then the
+// pattern is too, and a_ji is read as a_ij).
 // prevalidated: the caller ran validate_rowmax with rowmax_workspace(c, n).
 // With deferred = k > 0 the graph's size comes back through
 // read_int_with_flags(..., k): flag slots 1..k-1 (slot 0 is the graph's own
 // symmetry check) are in c.hflags when it returns.
 DevCsr strength_graph(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, const double* v, double theta,
                       int symmetric = -1, bool prevalidated = false, int* symmetricOut = nullptr,
-                      const double* learned = nullptr, int deferred = 0);
+                      const double* learned = nullptr, int deferred = 0, bool valueSymmetric = false);
 double* rowmax_workspace(tf_ctx& c, int n);
 void match_round(tf_ctx& c, int n, const int* rp, const int* ci, const double* w, int* mate, int* proposal);
 // nnz < 0: read from rp[n].  With sizes / slots (capacity n + 1 and 4n + 1)

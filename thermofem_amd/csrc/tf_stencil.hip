@@ -120,15 +120,9 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fill(StencilTable tb, const 
             vals[stencil_index(i, tb.own[t], tb.tile)] = v[j];
         }
         else {
-            // row c < i holds the transpose (c, i) among its last entries
-            bool found = false;
-            for (int q = rp[c + 1] - 1; q >= rp[c] && ci[q] >= i; --q) {
-                if (ci[q] == i) {
-                    found = true;
-                    differs |= __double_as_longlong(v[q]) != __double_as_longlong(v[j]);
-                }
-            }
-            differs |= !found;
+            // the transpose (c, i) in row c < i
+            const int q = find_sorted_col(ci, rp[c], rp[c + 1], i);
+            differs |= q < 0 || __double_as_longlong(v[q]) != __double_as_longlong(v[j]);
         }
     }
     mask[i] = static_cast<unsigned short>(m);
