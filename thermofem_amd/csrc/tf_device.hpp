@@ -198,13 +198,15 @@ __device__ __forceinline__ void epilogue(int row, T acc, const T* __restrict__ x
 }
 
 // Read-only view of a sliced-ELL matrix (slices of 64 row positions, entries
-// stored slice-major; C = short: column offsets from the row, int: columns).
+// stored slice-major in groups of 4 per lane, see sell_slot; C = short:
+// column offsets from the row, int: columns).
 // With perm, rows are ordered by decreasing length inside windows of 256
 // rows so that the rows sharing a slice have similar lengths.
 template <class C>
 struct SellView {
     int n;
     int slices;
+    int packed;                // layout (sell_slot)
     const int* base;           // entry offset of each slice (slices + 1)
     const unsigned char* len;  // row length per position
     const C* col;
@@ -266,12 +268,57 @@ __device__ __forceinline__ T stencil_row_sum(const StencilView& m, const T* __re
 __device__ __forceinline__ int sell_col(short c, int row) { return row + int(c); }
 __device__ __forceinline__ int sell_col(int c, int) { return c; }
 
-// Sum of one row of a sliced-ELL matrix for the calling lane: vp and cp
-// point at the row's first entry (stride 64 between its entries), len >= 0.
-// Chunks of 8 entries issue all their loads and gathers before the adds;
-// slots past the row's end re-read its last entry (no branch, no padding
-// read) and are not added.  Products are rounded and added one by one in
-// stored order, as csr_stream_rows does.
+// Sliced-ELL slot of entry k of the row at lane l of its slice, relative to
+// the slice's base.  Plain layout: entry k of every lane, then entry k + 1
+// (64 k + l).  Packed layout: groups of 4 per lane (group g of every lane's
+// row, then group g + 1), so a lane's 4 consecutive entries are contiguous
+// and one vector load brings them in.
+__host__ __device__ __forceinline__ int sell_slot(int k, int l, bool packed) {
+    return packed ? 256 * (k >> 2) + 4 * l + (k & 3) : kWave * k + l;
+}
+
+// Four consecutive entries (16-byte aligned for values and 32-bit columns,
+// 8-byte aligned for 16-bit columns).
+typedef float SellF4 __attribute__((ext_vector_type(4)));
+typedef double SellD2 __attribute__((ext_vector_type(2)));
+typedef short SellS4 __attribute__((ext_vector_type(4)));
+typedef int SellI4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void sell_load4(const float* p, float (&o)[4]) {
+    const SellF4 q = *reinterpret_cast<const SellF4*>(p);
+    o[0] = q.x;
+    o[1] = q.y;
+    o[2] = q.z;
+    o[3] = q.w;
+}
+__device__ __forceinline__ void sell_load4(const double* p, double (&o)[4]) {
+    const SellD2 a = *reinterpret_cast<const SellD2*>(p);
+    const SellD2 b = *reinterpret_cast<const SellD2*>(p + 2);
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = b.x;
+    o[3] = b.y;
+}
+__device__ __forceinline__ void sell_load4(const short* p, short (&o)[4]) {
+    const SellS4 q = *reinterpret_cast<const SellS4*>(p);
+    o[0] = q.x;
+    o[1] = q.y;
+    o[2] = q.z;
+    o[3] = q.w;
+}
+__device__ __forceinline__ void sell_load4(const int* p, int (&o)[4]) {
+    const SellI4 q = *reinterpret_cast<const SellI4*>(p);
+    o[0] = q.x;
+    o[1] = q.y;
+    o[2] = q.z;
+    o[3] = q.w;
+}
+
+// Sum of one row of a sliced-ELL matrix for the calling lane (plain
+// layout): vp and cp point at the row's first entry (base + lane, stride 64
+// between its entries), len >= 0.  Chunks of 8 entries issue all their loads
+// and gathers before the adds; slots past the row's end re-read its last
+// entry (no branch, no padding read) and are not added.  Products are
+// rounded and added one by one in stored order, as csr_stream_rows does.
 template <class T, class C, class XF>
 __device__ __forceinline__ T sell_row_sum(int row, int len, const T* __restrict__ vp, const C* __restrict__ cp,
                                           XF&& xval) {
@@ -293,6 +340,46 @@ __device__ __forceinline__ T sell_row_sum(int row, int len, const T* __restrict_
             const T p = pv[u] * xv[u];
             const T s = acc + p;
             acc = k0 + u < len ? s : acc;
+        }
+    }
+    return acc;
+}
+
+// The same for the packed layout: vp and cp point at the lane's first group
+// (base + 4 lane).  Two groups (8 entries) go per round, each one vector
+// load of values and one of columns, all gathers of the round issued before
+// the adds; a round past the row's last group re-reads that group and adds
+// nothing.  The slots of the last group past the row's end hold value 0 and
+// the row's own column, and are not added.  The same arithmetic.
+template <class T, class C, class XF>
+__device__ __forceinline__ T sell_row_sum4(int row, int len, const T* __restrict__ vp, const C* __restrict__ cp,
+                                           XF&& xval) {
+#pragma clang fp contract(off)
+    T acc = 0;
+    const int groups = (len + 3) >> 2;
+    for (int g0 = 0; g0 < groups; g0 += 2) {
+        T pv[8], xv[8];
+        C cv[8];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int g = min(g0 + h, groups - 1);
+            T pq[4];
+            C cq[4];
+            sell_load4(vp + 256 * g, pq);
+            sell_load4(cp + 256 * g, cq);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                pv[4 * h + u] = pq[u];
+                cv[4 * h + u] = cq[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = xval(sell_col(cv[u], row));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const T p = pv[u] * xv[u];
+            const T s = acc + p;
+            acc = 4 * g0 + u < len ? s : acc;
         }
     }
     return acc;

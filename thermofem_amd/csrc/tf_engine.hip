@@ -202,9 +202,10 @@ __device__ __forceinline__ void for_sell_rows(const SellView<C>& m, const double
         const int slice = lb * spb + int(threadIdx.x) / kWave;
         const int pos = slice * kWave + lane;
         if (pos < m.n) {
-            const int off = m.base[slice] + lane;
             const int row = m.perm ? m.perm[pos] : pos;
-            f(row, sell_row_sum<double>(row, int(m.len[pos]), v + off, m.col + off, xval));
+            const int b = m.base[slice];
+            f(row, m.packed ? sell_row_sum4<double>(row, int(m.len[pos]), v + b + 4 * lane, m.col + b + 4 * lane, xval)
+                            : sell_row_sum<double>(row, int(m.len[pos]), v + b + lane, m.col + b + lane, xval));
         }
     }
 }

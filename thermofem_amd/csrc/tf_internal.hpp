@@ -326,15 +326,19 @@ struct DevSell {
     void* col = nullptr;
     int* perm = nullptr;       // row at each position, nullptr when rows keep their order
     int lanes = 1;             // lanes per row of the product kernels (1, 2 or 4; same results)
+    bool packed = false;       // entries in groups of 4 per lane (sell_slot)
     template <class C>
     SellView<C> view() const {
-        return SellView<C>{n, slices, base, len, static_cast<const C*>(col), perm};
+        return SellView<C>{n, slices, packed ? 1 : 0, base, len, static_cast<const C*>(col), perm};
     }
     explicit operator bool() const { return base != nullptr; }
 };
 // Builds the structure; false (nothing kept) if a row has more than 255
 // entries or padding would exceed maxPadding * nnz.
 bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, DevSell& s, double maxPadding);
+// The same with the layout given (pack: groups of 4 entries per lane).
+bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, DevSell& s, double maxPadding,
+                bool pack);
 template <class A, class T>
 void sell_values(tf_ctx& c, const DevSell& s, const int* rp, const A* v, T* out);
 void free_sell(tf_ctx& c, DevSell& s);

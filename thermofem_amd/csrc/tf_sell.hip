@@ -2,9 +2,19 @@
 //
 // Rows are grouped in slices of 64 consecutive rows (one wave).  Slice s
 // stores its entries slice-major: entry k of row 64s+l sits at
-// base[s] + 64k + l, for k < width(s) = the longest row of the slice, so the
-// wave's k-th loads are 64 consecutive elements (one or two cache lines per
-// array) and no LDS staging or block barrier is needed.  Columns are kept as
+// base[s] + 64k + l (plain layout), for k < width(s) = the longest row of
+// the slice, so the wave's k-th loads are 64 consecutive elements (one or
+// two cache lines per array) and no LDS staging or block barrier is needed.
+// Matrices with rows of 10 or more entries on average (aggregation-coarsened
+// levels, FEM operators) are stored packed instead: entries in groups of 4
+// per lane, entry k at base[s] + 256 (k / 4) + 4 l + k % 4 with widths
+// rounded up to a multiple of 4, so a lane's 4 consecutive entries are one
+// 16-byte value load (FP32; two for FP64) and one 8-byte column load (on
+// the 598k-row level 1 of the 128^3 heat hierarchy an FP32 sweep took
+// 15.4 us against 18.5 us with one load per entry); the slots of a row's
+// last group past its end hold value 0 and the row's own column.  Short rows
+// (5- and 7-point stencils) keep the plain layout, where rounding the widths
+// up would add most of the padding.  Columns are kept as
 // 16-bit offsets from the row whenever every offset of the matrix fits
 // (stencil and aggregation-coarsened matrices), which cuts the index bytes
 // per entry from 4 to 2; otherwise as 32-bit columns.  Row lengths are one
@@ -15,8 +25,7 @@
 // sigma = 256) and a position -> row table is kept; results are written to
 // the original rows, so the reordering is invisible.  Each lane adds its row's rounded
 // products sequentially in stored (ascending column) order, so the row sums
-// equal the CSR kernels' and the oracle's bit for bit.  Padding slots are
-// never read.
+// equal the CSR kernels' and the oracle's bit for bit.
 #include <climits>
 #include <vector>
 
@@ -32,7 +41,7 @@ constexpr int kSortWindow = 256;
 
 __global__ __launch_bounds__(kBlock) void k_sell_width(int n, int slices, const int* __restrict__ rp,
                                                        const int* __restrict__ ci, int* __restrict__ width64,
-                                                       int* __restrict__ flags) {
+                                                       int* __restrict__ flags, int pack) {
     const int slice = int(blockIdx.x) * kSlicesPerBlock + int(threadIdx.x) / kWave;
     const int lane = int(threadIdx.x) & (kWave - 1);
     const int row = slice * kSliceRows + lane;
@@ -51,14 +60,15 @@ __global__ __launch_bounds__(kBlock) void k_sell_width(int n, int slices, const 
     int m = len;
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
-    if (slice < slices && lane == 0) width64[slice] = m * kSliceRows;
+    if (slice < slices && lane == 0) width64[slice] = (pack ? (m + 3) / 4 * 4 : m) * kSliceRows;
     if (slice == slices - 1 && lane == 0) width64[slices] = 0;
 }
 
 // Rows of each 256-row window ranked by (length descending, row ascending);
 // writes the position -> row table and the width of every slice.
 __global__ __launch_bounds__(kSortWindow) void k_sell_sort(int n, int slices, const int* __restrict__ rp,
-                                                           int* __restrict__ perm, int* __restrict__ width64) {
+                                                           int* __restrict__ perm, int* __restrict__ width64,
+                                                           int pack) {
     __shared__ int lens[kSortWindow];
     const int w0 = int(blockIdx.x) * kSortWindow;
     const int i = int(threadIdx.x);
@@ -74,38 +84,41 @@ __global__ __launch_bounds__(kSortWindow) void k_sell_sort(int n, int slices, co
     }
     const int pos = w0 + rank;
     perm[pos] = row;
-    if ((pos & (kSliceRows - 1)) == 0) width64[pos / kSliceRows] = len * kSliceRows;
+    if ((pos & (kSliceRows - 1)) == 0) width64[pos / kSliceRows] = (pack ? (len + 3) / 4 * 4 : len) * kSliceRows;
     if (pos / kSliceRows == slices - 1 && (pos & (kSliceRows - 1)) == 0) width64[slices] = 0;
 }
 
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                                       const int* __restrict__ perm, const int* __restrict__ base,
-                                                      unsigned char* __restrict__ len, C* __restrict__ col) {
+                                                      unsigned char* __restrict__ len, C* __restrict__ col, int pack) {
     const int pos = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (pos >= n) return;
     const int row = perm ? perm[pos] : pos;
     const int s = rp[row], e = rp[row + 1];
     len[pos] = static_cast<unsigned char>(e - s);
-    const int off = base[pos / kSliceRows] + (pos & (kSliceRows - 1));
-    for (int j = s; j < e; ++j) {
+    const int b = base[pos / kSliceRows], l = pos & (kSliceRows - 1);
+    const int filled = pack ? (e - s + 3) / 4 * 4 : e - s;  // packed: the last group's spare slots too
+    for (int k = 0; k < filled; ++k) {
+        const int j = s + k;
         if constexpr (sizeof(C) == 2)
-            col[off + (j - s) * kSliceRows] = static_cast<C>(ci[j] - row);
+            col[b + sell_slot(k, l, pack)] = static_cast<C>(j < e ? ci[j] - row : 0);
         else
-            col[off + (j - s) * kSliceRows] = ci[j];
+            col[b + sell_slot(k, l, pack)] = j < e ? ci[j] : row;
     }
 }
 
 template <class A, class T>
 __global__ __launch_bounds__(kBlock) void k_sell_values(int n, const int* __restrict__ rp, const int* __restrict__ perm,
                                                         const int* __restrict__ base, const A* __restrict__ v,
-                                                        T* __restrict__ out) {
+                                                        T* __restrict__ out, int pack) {
     const int pos = int(blockIdx.x) * kBlock + int(threadIdx.x);
     if (pos >= n) return;
     const int row = perm ? perm[pos] : pos;
     const int s = rp[row], e = rp[row + 1];
-    const int off = base[pos / kSliceRows] + (pos & (kSliceRows - 1));
-    for (int j = s; j < e; ++j) out[off + (j - s) * kSliceRows] = static_cast<T>(v[j]);
+    const int b = base[pos / kSliceRows], l = pos & (kSliceRows - 1);
+    const int filled = pack ? (e - s + 3) / 4 * 4 : e - s;
+    for (int k = 0; k < filled; ++k) out[b + sell_slot(k, l, pack)] = s + k < e ? static_cast<T>(v[s + k]) : T(0);
 }
 
 template <class T, class C, int E>
@@ -118,9 +131,12 @@ __global__ __launch_bounds__(kBlock) void k_sell(SellView<C> m, const T* __restr
     const int slice = blk * kSlicesPerBlock + int(threadIdx.x) / kWave;
     const int pos = slice * kSliceRows + lane;
     if (pos >= m.n) return;
-    const int off = m.base[slice] + lane;
     const int row = m.perm ? m.perm[pos] : pos;
-    const T acc = sell_row_sum<T>(row, int(m.len[pos]), v + off, m.col + off, [&](int c) { return x[c]; });
+    auto xf = [&](int c) { return x[c]; };
+    const T acc = m.packed
+                      ? sell_row_sum4<T>(row, int(m.len[pos]), v + m.base[slice] + 4 * lane,
+                                         m.col + m.base[slice] + 4 * lane, xf)
+                      : sell_row_sum<T>(row, int(m.len[pos]), v + m.base[slice] + lane, m.col + m.base[slice] + lane, xf);
     epilogue<T, E>(row, acc, x, y, b, inv, w);
 }
 
@@ -157,8 +173,8 @@ __global__ __launch_bounds__(kBlock) void k_sell_split(SellView<C> m, const T* _
     const int k0 = q * kseg, k1 = min(len, k0 + kseg);
     const int cnt = k1 > k0 ? k1 - k0 : 0;
     const int row = valid ? (m.perm ? m.perm[pos] : pos) : 0;
-    const T* vp = v + base + rl;
-    const C* cp = m.col + base + rl;
+    const T* vp = v + base;
+    const C* cp = m.col + base;
     T prod[kSplitRegs];
     if (cnt > 0) {
         T pv[kSplitRegs], xv[kSplitRegs];
@@ -166,8 +182,8 @@ __global__ __launch_bounds__(kBlock) void k_sell_split(SellView<C> m, const T* _
 #pragma unroll
         for (int u = 0; u < kSplitRegs; ++u) {
             const int k = min(k0 + u, k1 - 1);
-            pv[u] = ld_entry(vp + k * kWave);
-            pc[u] = sell_col(ld_entry(cp + k * kWave), row);
+            pv[u] = ld_entry(vp + sell_slot(k, rl, m.packed));
+            pc[u] = sell_col(ld_entry(cp + sell_slot(k, rl, m.packed)), row);
         }
 #pragma unroll
         for (int u = 0; u < kSplitRegs; ++u) xv[u] = x[pc[u]];
@@ -186,7 +202,8 @@ __global__ __launch_bounds__(kBlock) void k_sell_split(SellView<C> m, const T* _
                 acc = u < cnt ? sum : acc;
             }
             for (int k = k0 + kSplitRegs; k < k1; ++k) {  // rows wider than R * kSplitRegs
-                const T p = ld_entry(vp + k * kWave) * x[sell_col(ld_entry(cp + k * kWave), row)];
+                const T p = ld_entry(vp + sell_slot(k, rl, m.packed)) *
+                            x[sell_col(ld_entry(cp + sell_slot(k, rl, m.packed)), row)];
                 acc = acc + p;
             }
         }
@@ -201,7 +218,7 @@ namespace {
 // pass the int range or the padding limit.  Sets s.base and s.stored.
 bool sell_offsets(tf_ctx& c, int slices, int64_t nnz, const int* width64, double maxPadding, DevSell& s,
                   int flags = 0) {
-    if (int64_t(slices) * kSliceRows * 255 > int64_t(INT32_MAX)) {
+    if (int64_t(slices) * kSliceRows * 256 > int64_t(INT32_MAX)) {
         // The offsets are scanned in int: check the exact total first when it could overflow.
         std::vector<int> w(static_cast<size_t>(slices));
         check(hipMemcpyAsync(w.data(), width64, size_t(slices) * sizeof(int), hipMemcpyDeviceToHost, c.stream),
@@ -228,13 +245,22 @@ bool sell_offsets(tf_ctx& c, int slices, int64_t nnz, const int* width64, double
 }  // namespace
 
 bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, DevSell& s, double maxPadding) {
+    // packed when rows hold 10 or more entries on average (the rounding of
+    // the widths to groups of 4 is then a small share of the padding); a
+    // packed build that would pass the padding limit is made plain instead
+    if (nnz >= 10 * int64_t(n) && sell_build(c, n, nnz, rp, ci, s, maxPadding, true)) return true;
+    return sell_build(c, n, nnz, rp, ci, s, maxPadding, false);
+}
+
+bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, DevSell& s, double maxPadding,
+                bool pack) {
     s = DevSell{};
     if (n <= 0) return false;
     const int slices = (n + kSliceRows - 1) / kSliceRows;
     int* width64 = c.wst<int>(kSellWsSlot, size_t(slices) + 1);
     c.clear_flags(2);
     hipLaunchKernelGGL(k_sell_width, dim3((slices + kSlicesPerBlock - 1) / kSlicesPerBlock), dim3(kBlock), 0,
-                       c.stream, n, slices, rp, ci, width64, c.dflags);
+                       c.stream, n, slices, rp, ci, width64, c.dflags, pack ? 1 : 0);
     check(hipGetLastError(), "sell width launch");
     // The width flags (a row over 255 entries, a column offset past 16 bits)
     // come back with the first offsets scan's total.
@@ -248,7 +274,7 @@ bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, Dev
         // Rows of one slice differ too much in length: order rows by length in 256-row windows.
         int* perm = dev_alloc<int>(c, size_t(n));
         hipLaunchKernelGGL(k_sell_sort, dim3((n + kSortWindow - 1) / kSortWindow), dim3(kSortWindow), 0, c.stream, n,
-                           slices, rp, perm, width64);
+                           slices, rp, perm, width64, pack ? 1 : 0);
         check(hipGetLastError(), "sell sort launch");
         if (!sell_offsets(c, slices, nnz, width64, maxPadding, s)) {
             dev_free(c, perm);
@@ -259,16 +285,17 @@ bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, Dev
     s.n = n;
     s.slices = slices;
     s.narrow = !wide;
+    s.packed = pack;
     s.len = dev_alloc<unsigned char>(c, size_t(n));
     s.col = s.narrow ? static_cast<void*>(dev_alloc<short>(c, size_t(s.stored)))
                      : static_cast<void*>(dev_alloc<int>(c, size_t(s.stored)));
     const dim3 grid((n + kBlock - 1) / kBlock);
     if (s.narrow)
         hipLaunchKernelGGL(k_sell_fill<short>, grid, dim3(kBlock), 0, c.stream, n, rp, ci, s.perm, s.base, s.len,
-                           static_cast<short*>(s.col));
+                           static_cast<short*>(s.col), pack ? 1 : 0);
     else
         hipLaunchKernelGGL(k_sell_fill<int>, grid, dim3(kBlock), 0, c.stream, n, rp, ci, s.perm, s.base, s.len,
-                           static_cast<int*>(s.col));
+                           static_cast<int*>(s.col), pack ? 1 : 0);
     check(hipGetLastError(), "sell fill launch");
     return true;
 }
@@ -277,7 +304,7 @@ template <class A, class T>
 void sell_values(tf_ctx& c, const DevSell& s, const int* rp, const A* v, T* out) {
     if (s.n <= 0) return;
     hipLaunchKernelGGL((k_sell_values<A, T>), dim3((s.n + kBlock - 1) / kBlock), dim3(kBlock), 0, c.stream, s.n, rp,
-                       s.perm, s.base, v, out);
+                       s.perm, s.base, v, out, s.packed ? 1 : 0);
     check(hipGetLastError(), "sell values launch");
 }
 
