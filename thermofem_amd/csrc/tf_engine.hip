@@ -38,6 +38,8 @@ constexpr int kFusedProlongRows = 65536;
 // padding (relative to nnz) and the level is not the LU-solved coarsest one:
 // one lane per row, 16-bit column offsets, no LDS staging (tf_sell.hip).
 constexpr double kSellMaxPadding = 0.3;
+// the packed layout (10+ entries a row) in row order: padding costs memory only
+constexpr double kSellMaxPackedPadding = 0.6;
 constexpr int kSellMinRows = 256;
 constexpr int kStencilMinRows = 4096;  // below this the probe and fill cost more than they save
 // Levels with at least this many nonzeros are tuned: sliced ELL and the CSR
@@ -938,7 +940,8 @@ void AmgEngine::setup_hierarchy(int n, int64_t nnz, const int* rp, const int* ci
                 }
                 continue;
             }
-            if (!sell_build(*c_, L.native.n, L.native.nnz, L.native.rp, L.native.ci, L.sell, kSellMaxPadding))
+            if (!sell_build(*c_, L.native.n, L.native.nnz, L.native.rp, L.native.ci, L.sell, kSellMaxPadding,
+                            kSellMaxPackedPadding))
                 continue;
             bool keep;
             if (f32) {
@@ -1574,7 +1577,8 @@ void SolverEngine::setup(int n, int64_t nnz, const int* rp, const int* ci, const
     if (n >= kStencilMinRows && stencil_build(*c_, A_.n, A_.nnz, A_.rp, A_.ci, A_.v, Ast_, &Astv_)) {
         // structured-grid operator: the iteration's products read the stencil arrays
     }
-    else if (n >= kSellMinRows && sell_build(*c_, A_.n, A_.nnz, A_.rp, A_.ci, As_, kSellMaxPadding)) {
+    else if (n >= kSellMinRows &&
+             sell_build(*c_, A_.n, A_.nnz, A_.rp, A_.ci, As_, kSellMaxPadding, kSellMaxPackedPadding)) {
         Asv_ = dev_alloc<double>(*c_, size_t(As_.stored));
         sell_values<double, double>(*c_, As_, A_.rp, A_.v, Asv_);
         if (!sell_spmv_pays()) {

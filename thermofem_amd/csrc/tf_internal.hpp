@@ -335,7 +335,10 @@ struct DevSell {
 };
 // Builds the structure; false (nothing kept) if a row has more than 255
 // entries or padding would exceed maxPadding * nnz.
-bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, DevSell& s, double maxPadding);
+// maxPackedPadding (< 0: maxPadding) bounds the padding of the packed
+// layout, which matrices of 10+ entries a row on average take first.
+bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, DevSell& s, double maxPadding,
+                double maxPackedPadding = -1);
 // The same with the layout given (pack: groups of 4 entries per lane).
 bool sell_build(tf_ctx& c, int n, int64_t nnz, const int* rp, const int* ci, DevSell& s, double maxPadding,
                 bool pack);
