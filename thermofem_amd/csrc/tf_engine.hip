@@ -224,7 +224,28 @@ __device__ __forceinline__ void for_stencil_rows(const StencilView& m, const dou
     const int end = min(nblocks, (xcd + 1) * chunk);
     for (int lb = xcd * chunk + idx; lb < end; lb += per) {
         const int row = lb * kBlock + int(threadIdx.x);
-        if (row < m.n) f(row, stencil_row_sum<double, NC>(m, v, row, xval));
+        if (row < m.n) f(row, stencil_row_sum<double, NC == 0 ? 1 : NC>(m, v, row, xval));
+    }
+}
+// The same with x an array: the mirrored 7-point stencil (NC == 0) takes its
+// -1 / +1 neighbours from the wave's lanes.
+template <int NC, class F>
+__device__ __forceinline__ void for_stencil_rows_x(const StencilView& m, const double* __restrict__ v,
+                                                   const double* x, F&& f) {
+    if constexpr (NC != 0) {
+        for_stencil_rows<NC>(m, v, [&](int c) { return x[c]; }, f);
+    }
+    else {
+        const int nblocks = (m.n + kBlock - 1) / kBlock;
+        const int per = int(gridDim.x) / kXcds;
+        const int xcd = int(blockIdx.x % kXcds), idx = int(blockIdx.x / kXcds);
+        const int chunk = (nblocks + kXcds - 1) / kXcds;
+        const int end = min(nblocks, (xcd + 1) * chunk);
+        for (int lb = xcd * chunk + idx; lb < end; lb += per) {
+            const int row = lb * kBlock + int(threadIdx.x);
+            const double acc = stencil7_row_sum<double>(m, v, x, row);
+            if (row < m.n) f(row, acc);
+        }
     }
 }
 
@@ -264,7 +285,7 @@ struct StencilRows {
     const double* v;
     template <class F>
     __device__ __forceinline__ void each(const double* x, F&& f) const {
-        for_stencil_rows<NC>(m, v, [&](int c) { return x[c]; }, f);
+        for_stencil_rows_x<NC>(m, v, x, f);
     }
     template <class XF, class F>
     __device__ __forceinline__ void each_fn(XF&& xval, F&& f) const {
@@ -1640,7 +1661,9 @@ double SolverEngine::norm_host(const double* x) {
 void SolverEngine::launch_part() {
     if (Ast_) {
         const int nb = (Ast_.n + kBlock - 1) / kBlock;
-        if (Ast_.chunks() == 1)
+        if (Ast_.chunks() == 0)
+            launch_iteration(StencilRows<0>{Ast_.view(), Astv_}, nb);
+        else if (Ast_.chunks() == 1)
             launch_iteration(StencilRows<1>{Ast_.view(), Astv_}, nb);
         else
             launch_iteration(StencilRows<2>{Ast_.view(), Astv_}, nb);

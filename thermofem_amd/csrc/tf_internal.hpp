@@ -372,7 +372,16 @@ struct DevStencil {
         }
         return v;
     }
-    int chunks() const { return terms > 8 ? 2 : 1; }
+    // Row-sum variant of the product kernels: 0 the mirrored 7-point stencil
+    // (stencil7_row_sum), else 1 or 2 chunks of 8 terms.
+    int chunks() const {
+        const bool seven = terms == 7 && mirrored && off[2] == -1 && off[3] == 0 && off[4] == 1 &&
+                           off[1] == -off[5] && off[0] == -off[6] && off[5] > 1 && off[6] > off[5] &&
+                           arrays == 4 && src[3] == 0 && src[4] == 64 && src[5] == 128 && src[6] == 192 &&
+                           src[2] == 64 && src[1] == 128 && src[0] == 192 && shift[2] == -1 &&
+                           shift[1] == off[1] && shift[0] == off[0] && shift[3] == 0 && shift[4] == 0;
+        return seven ? 0 : (terms > 8 ? 2 : 1);
+    }
     explicit operator bool() const { return mask != nullptr; }
 };
 // Builds the storage when every row's columns come from one list of at most
