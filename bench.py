@@ -215,8 +215,10 @@ def main():
     # the operator has one offset list (it does here): the roofline kernel.
     if st64 is not None:
         moved_st = n * (8 * arrays + 2 + 16)
-        roof_t, roof_kernel = st64, (f"k_stencil<double,0,1> level-0 FP64 SpMV, structured-stencil storage "
-                                     f"({terms} offsets, {arrays} value arrays)")
+        variant = 0 if (terms == 7 and arrays == 4) else (1 if terms <= 8 else 2)
+        roof_t, roof_kernel = st64, (f"k_stencil<double,0,{variant}> level-0 FP64 SpMV, structured-stencil storage "
+                                     f"({terms} offsets, {arrays} value arrays"
+                                     f"{', -1/+1 neighbours from adjacent lanes' if variant == 0 else ''})")
         log(f"spmv stencil f64 {st64 * 1e6:.1f} us {bytes64 / st64 / 1e9:.0f} GB/s; f32 {st32 * 1e6:.1f} us")
     else:
         roof_t, roof_kernel = t64, "k_sell<double,short,0> level-0 FP64 SpMV"
@@ -673,7 +675,8 @@ def traffic_bytes(nx, stencil=True):
 
 def spmv_only(args, n, nnz, A, v, np, P, check, lib, Context, DeviceArray):
     """Level-0 FP64 SpMV in sliced ELL (k_sell<double,short,0>) and in the
-    structured-stencil storage (k_stencil<double,0,1>) x reps each, and a plain copy kernel
+    structured-stencil storage (k_stencil<double,0,0> for the mirrored 7-point
+    operator) x reps each, and a plain copy kernel
     (k_convert<double,double>, 2*8*n bytes per launch) x reps, for counter runs."""
     ctx = Context(0)
     xv = DeviceArray.from_host(P.uniform(11, n))
