@@ -238,23 +238,45 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(int n, int k, const int* __res
 // shuffles.  Every column is computed with k_bspmm's arithmetic (products
 // rounded and added in stored order, the same epilogues), so the results are
 // bit-identical; the wider loads halve the gathers' load instructions (P1
-// tets on 64^3 cells, k = 64, FP64: 171 -> 131 us).
-template <class T, int GL, int E>
+// tets on 64^3 cells, k = 64, FP64: 171 -> 131 us).  FP64 launches may fuse
+// a column reduction D as k_bspmm does (partials[col * kBParts + block],
+// each block's rows added in wave, then row-in-wave order).
+template <class T, int GL, int E, int D = kBNone>
 __global__ __launch_bounds__(kBlock) void k_bspmm_vec(int n, int k, const int* __restrict__ rp,
                                                       const int* __restrict__ ci, const T* __restrict__ v,
                                                       const T* __restrict__ X, T* __restrict__ Y,
-                                                      const T* __restrict__ B, const T* __restrict__ inv, T w) {
+                                                      const T* __restrict__ B, const T* __restrict__ inv, T w,
+                                                      double* __restrict__ partials, const KCtl* __restrict__ ctl) {
 #pragma clang fp contract(off)
     constexpr int V = 16 / int(sizeof(T));
+    static_assert(D == kBNone || sizeof(T) == 8, "fused column reductions are FP64");
     typedef T Vec __attribute__((ext_vector_type(V)));
     constexpr int RW = kWave / GL;  // rows per wave
     constexpr int R = kBlock / GL;  // rows per block step
+    __shared__ double red[D != kBNone ? kBlock * V : 1];
     const int lane = int(threadIdx.x) & (kWave - 1);
     const int sub = lane / GL, l = lane % GL;
     const int wave = int(threadIdx.x) / kWave;
     const int col = int(blockIdx.y) * GL * V + l * V;
     const bool live = col < k;
     const int cc = live ? col : 0;
+    // columns this lane writes (kBTrue: only those due for a refresh)
+    bool act[V];
+#pragma unroll
+    for (int t = 0; t < V; ++t) act[t] = live;
+    if constexpr (D == kBTrue) {
+        bool any = false;
+#pragma unroll
+        for (int t = 0; t < V; ++t) {
+            const KCtl* c = ctl + cc + t;
+            act[t] = live && !stopped(c) && c->check && !c->early;
+            any = any || act[t];
+        }
+        if (!__syncthreads_or(any ? 1 : 0)) return;  // (their partials are never read)
+    }
+    double contrib[V];
+#pragma unroll
+    for (int t = 0; t < V; ++t) contrib[t] = 0;
     int first, step, end;
     block_groups((n + R - 1) / R, first, step, end);
     for (int g = first; g < end; g += step) {
@@ -321,7 +343,41 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_vec(int n, int k, const int* _
                 y[t] = xo[t] + upd;
             }
         }
-        *reinterpret_cast<Vec*>(Y + at) = y;
+        if constexpr (D == kBTrue) {
+#pragma unroll
+            for (int t = 0; t < V; ++t)
+                if (act[t]) {
+                    Y[at + size_t(t)] = y[t];
+                    contrib[t] += double(y[t]) * double(y[t]);
+                }
+        }
+        else {
+            *reinterpret_cast<Vec*>(Y + at) = y;
+            if constexpr (D == kBDotX) {
+                const Vec xo = *reinterpret_cast<const Vec*>(X + at);
+#pragma unroll
+                for (int t = 0; t < V; ++t) contrib[t] += double(xo[t]) * double(y[t]);
+            }
+            if constexpr (D == kBSquares) {
+#pragma unroll
+                for (int t = 0; t < V; ++t) contrib[t] += double(y[t]) * double(y[t]);
+            }
+        }
+    }
+    if constexpr (D != kBNone) {
+#pragma unroll
+        for (int t = 0; t < V; ++t) red[t * kBlock + int(threadIdx.x)] = contrib[t];
+        __syncthreads();
+        const int q = int(threadIdx.x);  // column of this pass
+        const int qcol = int(blockIdx.y) * GL * V + q;
+        if (q < GL * V && qcol < k) {
+            const int ql = q / V, qt = q % V;
+            double s = 0;
+            for (int wv = 0; wv < kBlock / kWave; ++wv)
+#pragma unroll
+                for (int sb = 0; sb < RW; ++sb) s += red[qt * kBlock + wv * kWave + sb * GL + ql];
+            partials[size_t(qcol) * kBParts + blockIdx.x] = s;
+        }
     }
 }
 
@@ -347,27 +403,25 @@ __global__ __launch_bounds__(kBlock) void k_bdot(int n, int k, const double* __r
     block_partials(s, G, col, k, red, partials);
 }
 
-// Finaliser: per column, the partials added in block order, then the
-// single-vector recurrence on that column's control block (mode >= 0), or
-// the plain sums written to out (mode < 0).
-__global__ __launch_bounds__(1024) void k_bfinal(int k, const double* __restrict__ partials, int nparts, KCtl* ctl,
-                                                 int mode, double* __restrict__ out) {
-    const int lane = int(threadIdx.x) & (kWave - 1), w = int(threadIdx.x) / kWave;
-    for (int col = w; col < k; col += 1024 / kWave) {
-        if (mode >= 0) {
-            const KCtl* c = ctl + col;
-            if (stopped(c)) continue;
-            if (mode == kFinTrue && (!c->check || c->early)) continue;
-        }
-        double s = 0;
-        for (int i = lane; i < nparts; i += kWave) s += partials[size_t(col) * kBParts + i];
-        s = wave_sum(s);
-        if (lane == 0) {
-            if (mode >= 0)
-                finalize(mode, s, ctl + col);
-            else
-                out[col] = s;
-        }
+// Finaliser, one block per column: the column's partials added by the
+// block (sum_partials), then the single-vector recurrence on that column's
+// control block (mode >= 0), or the plain sum written to out (mode < 0).
+// (One 1024-thread block walking all columns took ~32 us at k = 64 with
+// 2048 partials a column; the column blocks run side by side.)
+__global__ __launch_bounds__(kBlock) void k_bfinal(const double* __restrict__ partials, int nparts, KCtl* ctl,
+                                                   int mode, double* __restrict__ out) {
+    const int col = int(blockIdx.x);
+    if (mode >= 0) {
+        const KCtl* c = ctl + col;
+        if (stopped(c)) return;
+        if (mode == kFinTrue && (!c->check || c->early)) return;
+    }
+    const double s = sum_partials(partials + size_t(col) * kBParts, nparts);
+    if (threadIdx.x == 0) {
+        if (mode >= 0)
+            finalize(mode, s, ctl + col);
+        else
+            out[col] = s;
     }
 }
 
@@ -503,20 +557,27 @@ void bspmm(tf_ctx& c, int n, int k, const int* rp, const int* ci, const T* v, co
     if (G <= 0) G = lanes_for(k);
     constexpr int V = 16 / int(sizeof(T));
     auto aligned = [](const void* p) { return p == nullptr || reinterpret_cast<uintptr_t>(p) % 16 == 0; };
-    if (D == kBNone && G == lanes_for(k) && k % V == 0 && aligned(X) && aligned(Y) && aligned(B)) {
-        // whole-width passes: 16-byte column loads (k_bspmm_vec)
-        const int GL = lanes_for(k / V);
-        const int RV = kBlock / GL;
-        const dim3 gv(row_grid((n + RV - 1) / RV, false), (k + GL * V - 1) / (GL * V));
-#define TF_BV(GG)                                                                                             \
-    case GG:                                                                                                  \
-        hipLaunchKernelGGL((k_bspmm_vec<T, GG, E>), gv, dim3(kBlock), 0, c.stream, n, k, rp, ci, v, X, Y, B, \
-                           inv, w);                                                                           \
+    const bool aligned16 = aligned(X) && aligned(Y) && aligned(B);
+    if constexpr (D == kBNone || sizeof(T) == 8) {
+        // whole-width passes: 16-byte column loads (k_bspmm_vec).  A reduced
+        // launch always takes this form when it applies, so that the number
+        // of partials it leaves is spmm_parts's; the solver's blocks are
+        // 16-byte aligned whenever k is a multiple of V.
+        if (G == lanes_for(k) && k % V == 0 && (aligned16 || D != kBNone)) {
+            require(aligned16, "block SpMM with a reduction needs 16-byte aligned blocks", TF_E_INVALID);
+            const int GL = lanes_for(k / V);
+            const int RV = kBlock / GL;
+            const dim3 gv(row_grid((n + RV - 1) / RV, D != kBNone), (k + GL * V - 1) / (GL * V));
+#define TF_BV(GG)                                                                                                \
+    case GG:                                                                                                     \
+        hipLaunchKernelGGL((k_bspmm_vec<T, GG, E, D>), gv, dim3(kBlock), 0, c.stream, n, k, rp, ci, v, X, Y, B, \
+                           inv, w, partials, ctl);                                                               \
         break;
-        switch (GL) { TF_BV(1) TF_BV(2) TF_BV(4) TF_BV(8) TF_BV(16) TF_BV(32) TF_BV(64) }
+            switch (GL) { TF_BV(1) TF_BV(2) TF_BV(4) TF_BV(8) TF_BV(16) TF_BV(32) TF_BV(64) }
 #undef TF_BV
-        check(hipGetLastError(), "block spmm launch");
-        return;
+            check(hipGetLastError(), "block spmm launch");
+            return;
+        }
     }
     const int R = kBlock / G;
     const dim3 grid(row_grid((n + R - 1) / R, D != kBNone), (k + G - 1) / G);
@@ -530,9 +591,20 @@ void bspmm(tf_ctx& c, int n, int k, const int* rp, const int* ci, const T* v, co
     check(hipGetLastError(), "block spmm launch");
 }
 
-// Number of partials a row-parallel launch over n rows writes per column.
-int row_parts(int n, int k, int G = 0) {
-    const int R = kBlock / (G > 0 ? G : lanes_for(k));
+// Number of partials a row-parallel launch over n rows writes per column
+// (k_bdot, k_bcg_update: one column a lane).
+int row_parts(int n, int k) {
+    const int R = kBlock / lanes_for(k);
+    return row_grid((n + R - 1) / R);
+}
+
+// Partials per column of bspmm's reduced launch over n rows with G lanes a
+// row: the 16-byte column form where bspmm takes it, else one column a lane.
+template <class T>
+int spmm_parts(int n, int k, int G) {
+    if (G <= 0) G = lanes_for(k);
+    constexpr int V = 16 / int(sizeof(T));
+    const int R = G == lanes_for(k) && k % V == 0 ? kBlock / lanes_for(k / V) : kBlock / G;
     return row_grid((n + R - 1) / R);
 }
 
@@ -565,7 +637,8 @@ void bcg_update(tf_ctx& c, int n, int k, const double* p, const double* q, doubl
 }
 
 void bfinal(tf_ctx& c, int k, const double* partials, int nparts, KCtl* ctl, int mode, double* out) {
-    hipLaunchKernelGGL(k_bfinal, dim3(1), dim3(1024), 0, c.stream, k, partials, nparts, ctl, mode, out);
+    if (k <= 0) return;
+    hipLaunchKernelGGL(k_bfinal, dim3(unsigned(k)), dim3(kBlock), 0, c.stream, partials, nparts, ctl, mode, out);
     check(hipGetLastError(), "block finaliser launch");
 }
 
@@ -890,7 +963,7 @@ void SolverEngine::launch_block_iteration(int k) {
     double* bP = bvec_ + 4 * nk;
     double* bQ = bvec_ + 5 * nk;
     KCtl* ctl = static_cast<KCtl*>(bctl_);
-    const int np = row_parts(n_, k), npA = row_parts(n_, k, bLanes_);
+    const int np = row_parts(n_, k), npA = spmm_parts<double>(n_, k, bLanes_);
     if (amg_)
         amg_->block().apply(k, bR, bZ);
     else
@@ -937,7 +1010,7 @@ void SolverEngine::apply_block(int k, const double* B, double* X, tf_solve_resul
                                          nullptr, G);
         });
     }
-    const int np = row_parts(n_, k), npA = row_parts(n_, k, bLanes_);
+    const int np = row_parts(n_, k), npA = spmm_parts<double>(n_, k, bLanes_);
     const unsigned fb = flat_blocks(nk);
     hipLaunchKernelGGL((k_bconvert<double, double>), dim3(fb), dim3(kBlock), 0, c_->stream, nk, B, bB);
     hipLaunchKernelGGL((k_bconvert<double, double>), dim3(fb), dim3(kBlock), 0, c_->stream, nk, X, bX);
