@@ -381,10 +381,11 @@ __global__ __launch_bounds__(kBlock) void k_bspmm_vec(int n, int k, const int* _
     }
 }
 
-// Column dot products sum X[r][j] * Y[r][j].
-template <int G>
+// Column dot products sum X[r][j] * Y[r][j] (Y widened exactly to FP64;
+// zero: Y = 0).
+template <int G, class YT>
 __global__ __launch_bounds__(kBlock) void k_bdot(int n, int k, const double* __restrict__ X,
-                                                 const double* __restrict__ Y, double* __restrict__ partials) {
+                                                 const YT* __restrict__ Y, int zero, double* __restrict__ partials) {
     __shared__ double red[kBlock];
     constexpr int R = kBlock / G;
     const int sub = int(threadIdx.x) / G, lane = int(threadIdx.x) % G;
@@ -397,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_bdot(int n, int k, const double* __r
             const int row = g * R + sub;
             if (row < n) {
                 const size_t at = size_t(row) * size_t(k) + size_t(col);
-                s += X[at] * Y[at];
+                s += X[at] * (zero ? 0.0 : double(Y[at]));
             }
         }
     block_partials(s, G, col, k, red, partials);
@@ -426,12 +427,16 @@ __global__ __launch_bounds__(kBlock) void k_bfinal(const double* __restrict__ pa
 }
 
 // CG direction per column: p = z (fresh) or p = z + beta p.
-__global__ void k_bcg_p(size_t nk, int k, const double* __restrict__ z, double* __restrict__ p,
+// (z in FP64 or the cycle's level-0 iterate in level precision, widened
+// exactly; zero: z = 0)
+template <class Z>
+__global__ void k_bcg_p(size_t nk, int k, const Z* __restrict__ z, int zero, double* __restrict__ p,
                         const KCtl* __restrict__ ctl) {
     for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
         const KCtl* c = ctl + int(i % size_t(k));
         if (stopped(c)) continue;
-        p[i] = c->fresh ? z[i] : z[i] + c->beta * p[i];
+        const double zi = zero ? 0.0 : double(z[i]);
+        p[i] = c->fresh ? zi : zi + c->beta * p[i];
     }
 }
 
@@ -476,10 +481,13 @@ __global__ void k_bjacobi_zero(int n, int k, const T* __restrict__ inv, T w, con
     }
 }
 
-// coarse[c][j] = sum over the members m of aggregate c (ascending) of f[m][j].
+// coarse[c][j] = sum over the members m of aggregate c (ascending) of f[m][j];
+// sweep: also the coarse level's first Jacobi sweep from zero,
+// x[c][j] = (w * inv[c]) * coarse[c][j] (k_bjacobi_zero's arithmetic).
 template <class T>
 __global__ void k_brestrict(int nc, int k, const int* __restrict__ off, const int* __restrict__ mem,
-                            const T* __restrict__ f, T* __restrict__ coarse) {
+                            const T* __restrict__ f, T* __restrict__ coarse, const T* __restrict__ inv, T w,
+                            T* __restrict__ x, int sweep) {
 #pragma clang fp contract(off)
     const size_t nk = size_t(nc) * size_t(k);
     for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
@@ -499,6 +507,26 @@ __global__ void k_brestrict(int nc, int k, const int* __restrict__ off, const in
             for (int q = q0 + 4; q < q1; ++q) s += f[size_t(mem[q]) * size_t(k) + size_t(j)];
         }
         coarse[i] = s;
+        if (sweep) {
+            const T wd = w * inv[c];
+            x[i] = wd * s;
+        }
+    }
+}
+
+// Cycle entry: b = R in level precision; sweep: also the first Jacobi sweep
+// from zero, x = (w * inv) * b (k_bjacobi_zero's arithmetic).
+template <class T>
+__global__ void k_benter(size_t nk, int k, const double* __restrict__ R, const T* __restrict__ inv, T w,
+                         T* __restrict__ b, T* __restrict__ x, int sweep) {
+#pragma clang fp contract(off)
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
+        const T bi = T(R[i]);
+        b[i] = bi;
+        if (sweep) {
+            const T wd = w * inv[i / size_t(k)];
+            x[i] = wd * bi;
+        }
     }
 }
 
@@ -608,13 +636,14 @@ int spmm_parts(int n, int k, int G) {
     return row_grid((n + R - 1) / R);
 }
 
-void bdot(tf_ctx& c, int n, int k, const double* X, const double* Y, double* partials) {
+template <class YT>
+void bdot(tf_ctx& c, int n, int k, const double* X, const YT* Y, int zero, double* partials) {
     const int G = lanes_for(k);
     const int R = kBlock / G;
     const dim3 grid(row_grid((n + R - 1) / R), (k + G - 1) / G);
 #define TF_BD(GG)                                                                                              \
     case GG:                                                                                                   \
-        hipLaunchKernelGGL((k_bdot<GG>), grid, dim3(kBlock), 0, c.stream, n, k, X, Y, partials);               \
+        hipLaunchKernelGGL((k_bdot<GG, YT>), grid, dim3(kBlock), 0, c.stream, n, k, X, Y, zero, partials);     \
         break;
     switch (G) { TF_BD(1) TF_BD(2) TF_BD(4) TF_BD(8) TF_BD(16) TF_BD(32) TF_BD(64) }
 #undef TF_BD
@@ -807,7 +836,7 @@ void BlockAmg::smooth(size_t d, int sweeps) {
 }
 
 template <class L>
-void BlockAmg::restrict_to(size_t d, bool ofResidual) {
+void BlockAmg::restrict_to(size_t d, bool ofResidual, bool sweepNext) {
     const AmgLevel& a = amg_->levels_[d];
     Lvl& l = lv_[d];
     const DevAgg& t = a.transfer;
@@ -819,9 +848,12 @@ void BlockAmg::restrict_to(size_t d, bool ofResidual) {
         f = static_cast<const L*>(l.r);
     }
     const size_t nk = size_t(t.nc) * size_t(k_);
+    Lvl& lc = lv_[d + 1];
     hipLaunchKernelGGL(k_brestrict<L>, dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, t.nc, k_, t.offsets,
-                       t.members, f, static_cast<L*>(lv_[d + 1].b));
+                       t.members, f, static_cast<L*>(lc.b), static_cast<const L*>(amg_->levels_[d + 1].inv),
+                       L(amg_->opt_.omega), static_cast<L*>(lc.x), int(sweepNext));
     check(hipGetLastError(), "block restrict launch");
+    lc.swept = sweepNext;
 }
 
 template <class L>
@@ -859,52 +891,83 @@ void BlockAmg::cycle(int kind, size_t d, bool zero) {
     ++visits[d];
     Lvl& l = lv_[d];
     if (zero || kind == 2) l.zero = true;
+    // a zero-start first sweep already done by the kernel that wrote b
+    const int done = l.swept ? 1 : 0;
+    if (l.swept) {
+        l.zero = false;
+        l.swept = false;
+    }
     if (d + 1 == lv_.size()) {
         coarse<L>(d);
         return;
     }
     const AmgOptionsC& o = amg_->opt_;
     if (kind == 2) {
-        restrict_to<L>(d, false);
+        restrict_to<L>(d, false, false);
         cycle<L>(2, d + 1, true);
         prolong<L>(d);
         cycle<L>(0, d, false);
         return;
     }
-    smooth<L>(d, o.pre);
-    restrict_to<L>(d, true);
+    smooth<L>(d, o.pre - done);
+    // the coarser level starts with a zero-start sweep unless it is the
+    // coarsest one
+    restrict_to<L>(d, true, o.pre >= 1 && d + 2 < lv_.size());
     cycle<L>(kind, d + 1, true);
     if (kind == 1) cycle<L>(kind, d + 1, false);
     prolong<L>(d);
     smooth<L>(d, o.post);
 }
 
+// The cycle on R, its result left in lv_[0].x (or lv_[0].zero).  The entry
+// kernel also does the finest level's first (zero-start) sweep when the
+// cycle begins with one.
 template <class L>
-void BlockAmg::apply_typed(const double* R, double* Z) {
+void BlockAmg::run_typed(const double* R) {
+    const AmgOptionsC& o = amg_->opt_;
     const int n = amg_->levels_[0].native.n;
     const size_t nk = size_t(n) * size_t(k_);
-    hipLaunchKernelGGL((k_bconvert<double, L>), dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk, R,
-                       static_cast<L*>(lv_[0].b));
+    for (auto& l : lv_) {
+        l.zero = false;
+        l.swept = false;
+    }
+    const bool sweep = o.cycle != 2 && o.pre >= 1 && lv_.size() > 1;
+    hipLaunchKernelGGL((k_benter<L>), dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk, k_, R,
+                       static_cast<const L*>(amg_->levels_[0].inv), L(o.omega), static_cast<L*>(lv_[0].b),
+                       static_cast<L*>(lv_[0].x), int(sweep));
     check(hipGetLastError(), "block enter launch");
-    for (auto& l : lv_) l.zero = false;
+    lv_[0].swept = sweep;
     std::fill(visits.begin(), visits.end(), 0);
-    cycle<L>(amg_->opt_.cycle, 0, true);
-    if (lv_[0].zero)
-        fill_zero(*c_, Z, nk * sizeof(double));
-    else
-        hipLaunchKernelGGL((k_bconvert<L, double>), dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk,
-                           static_cast<const L*>(lv_[0].x), Z);
-    check(hipGetLastError(), "block exit launch");
+    cycle<L>(o.cycle, 0, true);
 }
 
 void BlockAmg::apply(int k, const double* R, double* Z) {
+    int bits = 0;
+    bool zero = false;
+    const void* x = apply_in_place(k, R, bits, zero);
+    const size_t nk = size_t(amg_->levels_[0].native.n) * size_t(k);
+    if (zero)
+        fill_zero(*c_, Z, nk * sizeof(double));
+    else if (bits == 32)
+        hipLaunchKernelGGL((k_bconvert<float, double>), dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk,
+                           static_cast<const float*>(x), Z);
+    else
+        hipLaunchKernelGGL((k_bconvert<double, double>), dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk,
+                           static_cast<const double*>(x), Z);
+    check(hipGetLastError(), "block exit launch");
+}
+
+const void* BlockAmg::apply_in_place(int k, const double* R, int& bits, bool& zero) {
     require(amg_->ready(), "AMG requires setup", TF_E_INVALID);
     require(k >= 1, "block apply needs at least one column");
     reserve(k);
-    if (amg_->opt_.precision == 32)
-        apply_typed<float>(R, Z);
+    bits = amg_->opt_.precision == 32 ? 32 : 64;
+    if (bits == 32)
+        run_typed<float>(R);
     else
-        apply_typed<double>(R, Z);
+        run_typed<double>(R);
+    zero = lv_[0].zero;
+    return lv_[0].x;
 }
 
 // ---------------------------------------------------------- block Krylov
@@ -913,7 +976,7 @@ void SolverEngine::reserve_block(int k) {
     if (k == bk_ && bvec_) return;
     release_block();
     const size_t nk = size_t(n_) * size_t(k);
-    bvec_ = dev_alloc<double>(*c_, 6 * nk);
+    bvec_ = dev_alloc<double>(*c_, 5 * nk);  // B, X, R, P, Q
     bctl_ = dev_alloc<KCtl>(*c_, size_t(k));
     bparts_ = dev_alloc<double>(*c_, size_t(kBParts) * size_t(k));
     bsums_ = dev_alloc<double>(*c_, size_t(k));
@@ -959,19 +1022,29 @@ void SolverEngine::launch_block_iteration(int k) {
     double* bB = bvec_;
     double* bX = bvec_ + nk;
     double* bR = bvec_ + 2 * nk;
-    double* bZ = bvec_ + 3 * nk;
-    double* bP = bvec_ + 4 * nk;
-    double* bQ = bvec_ + 5 * nk;
+    double* bP = bvec_ + 3 * nk;
+    double* bQ = bvec_ + 4 * nk;
     KCtl* ctl = static_cast<KCtl*>(bctl_);
     const int np = row_parts(n_, k), npA = spmm_parts<double>(n_, k, bLanes_);
-    if (amg_)
-        amg_->block().apply(k, bR, bZ);
-    else
-        hipLaunchKernelGGL((k_bconvert<double, double>), dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk, bR,
-                           bZ);
-    bdot(*c_, n_, k, bR, bZ, bparts_);
-    bfinal(*c_, k, bparts_, np, ctl, kFinRhoCg, nullptr);
-    hipLaunchKernelGGL(k_bcg_p, dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk, k, bZ, bP, ctl);
+    // z = M^-1 r read where the cycle left it (level precision; z = r
+    // without a preconditioner)
+    const void* z = bR;
+    int bits = 64;
+    bool zzero = false;
+    if (amg_) z = amg_->block().apply_in_place(k, bR, bits, zzero);
+    const unsigned fb = flat_blocks(nk);
+    if (bits == 32) {
+        bdot(*c_, n_, k, bR, static_cast<const float*>(z), int(zzero), bparts_);
+        bfinal(*c_, k, bparts_, np, ctl, kFinRhoCg, nullptr);
+        hipLaunchKernelGGL(k_bcg_p<float>, dim3(fb), dim3(kBlock), 0, c_->stream, nk, k, static_cast<const float*>(z),
+                           int(zzero), bP, ctl);
+    }
+    else {
+        bdot(*c_, n_, k, bR, static_cast<const double*>(z), int(zzero), bparts_);
+        bfinal(*c_, k, bparts_, np, ctl, kFinRhoCg, nullptr);
+        hipLaunchKernelGGL(k_bcg_p<double>, dim3(fb), dim3(kBlock), 0, c_->stream, nk, k,
+                           static_cast<const double*>(z), int(zzero), bP, ctl);
+    }
     bspmm<double, kSpmv, kBDotX>(*c_, n_, k, A_.rp, A_.ci, A_.v, bP, bQ, nullptr, nullptr, 0.0, bparts_, ctl,
                                  bLanes_);
     bfinal(*c_, k, bparts_, npA, ctl, kFinCurvCg, nullptr);
@@ -998,8 +1071,8 @@ void SolverEngine::apply_block(int k, const double* B, double* X, tf_solve_resul
     double* bB = bvec_;
     double* bX = bvec_ + nk;
     double* bR = bvec_ + 2 * nk;
-    double* bP = bvec_ + 4 * nk;
-    double* bQ = bvec_ + 5 * nk;
+    double* bP = bvec_ + 3 * nk;
+    double* bQ = bvec_ + 4 * nk;
     KCtl* ctl = static_cast<KCtl*>(bctl_);
     KCtl* h = static_cast<KCtl*>(hbctl_);
     double* hs = static_cast<double*>(hbsums_);
@@ -1023,7 +1096,7 @@ void SolverEngine::apply_block(int k, const double* B, double* X, tf_solve_resul
         res.resize(size_t(k));
         for (int j = 0; j < k; ++j) res[size_t(j)] = std::sqrt(hs[j]);
     };
-    bdot(*c_, n_, k, bB, bB, bparts_);
+    bdot(*c_, n_, k, bB, static_cast<const double*>(bB), 0, bparts_);
     column_sums(*c_, k, bparts_, np, bsums_, hs);
     std::vector<double> normb(static_cast<size_t>(k)), tol(static_cast<size_t>(k)), res;
     for (int j = 0; j < k; ++j) {

@@ -255,6 +255,10 @@ public:
     BlockAmg(tf_ctx* ctx, AmgEngine* amg);
     ~BlockAmg();
     void apply(int k, const double* R, double* Z);  // Z = M^-1 R per column; R, Z distinct
+    // The same cycle with the result left in the level-0 iterate: returns it
+    // (float or double as `bits` says, valid until the next cycle) and
+    // whether it is known zero.
+    const void* apply_in_place(int k, const double* R, int& bits, bool& zero);
     // Makes the buffers (and tunes the launch widths) for k columns, so that
     // apply(k, ...) allocates nothing and can be captured.
     void reserve(int k);
@@ -264,15 +268,16 @@ private:
     struct Lvl {
         void *b = nullptr, *x = nullptr, *x2 = nullptr, *r = nullptr;
         bool zero = false;
+        bool swept = false;  // the zero-start sweep was fused into the kernel that wrote b
     };
     template <class L>
-    void apply_typed(const double* R, double* Z);
+    void run_typed(const double* R);
     template <class L>
     void cycle(int kind, size_t d, bool zero);
     template <class L>
     void smooth(size_t d, int sweeps);
     template <class L>
-    void restrict_to(size_t d, bool ofResidual);
+    void restrict_to(size_t d, bool ofResidual, bool sweepNext);
     template <class L>
     void prolong(size_t d);
     template <class L>
