@@ -426,17 +426,56 @@ __global__ __launch_bounds__(kBlock) void k_bfinal(const double* __restrict__ pa
     }
 }
 
+// Element-wise block kernels walk rows without dividing flat indices:
+// 2^lg lanes a row (lanes_for(k)), lane l taking the row's columns l,
+// l + 2^lg, ...; rows grid-strided.  `each` hands out one element at a
+// time; the streaming kernels below take kWalkRows rows a step instead, so
+// that several rows' loads are in flight (flat loops with one element a
+// thread ran the 64-column prolongation and CG direction at 2.8-3.6 TB/s).
+constexpr int kWalkRows = 4;
+struct RowWalk {
+    int n, k, lg;
+    template <class F>
+    __device__ __forceinline__ void each(F&& f) const {
+        const int G = 1 << lg, R = kBlock >> lg;
+        const int sub = int(threadIdx.x) >> lg, l = int(threadIdx.x) & (G - 1);
+        for (int row = int(blockIdx.x) * R + sub; row < n; row += int(gridDim.x) * R) {
+            const size_t base = size_t(row) * size_t(k);
+            for (int j = l; j < k; j += G) f(row, j, base + size_t(j));
+        }
+    }
+    // lane's first column, step between its columns, first row, row stride
+    __device__ __forceinline__ int lane() const { return int(threadIdx.x) & ((1 << lg) - 1); }
+    __device__ __forceinline__ int step() const { return 1 << lg; }
+    __device__ __forceinline__ int row0() const { return int(blockIdx.x) * (kBlock >> lg) + (int(threadIdx.x) >> lg); }
+    __device__ __forceinline__ int stride() const { return int(gridDim.x) * (kBlock >> lg); }
+};
+
 // CG direction per column: p = z (fresh) or p = z + beta p.
 // (z in FP64 or the cycle's level-0 iterate in level precision, widened
 // exactly; zero: z = 0)
 template <class Z>
-__global__ void k_bcg_p(size_t nk, int k, const Z* __restrict__ z, int zero, double* __restrict__ p,
+__global__ void k_bcg_p(RowWalk rw, const Z* __restrict__ z, int zero, double* __restrict__ p,
                         const KCtl* __restrict__ ctl) {
-    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
-        const KCtl* c = ctl + int(i % size_t(k));
+    const int n = rw.n, k = rw.k, stride = rw.stride();
+    for (int j = rw.lane(); j < k; j += rw.step()) {
+        const KCtl* c = ctl + j;
         if (stopped(c)) continue;
-        const double zi = zero ? 0.0 : double(z[i]);
-        p[i] = c->fresh ? zi : zi + c->beta * p[i];
+        const bool fresh = c->fresh != 0;
+        const double beta = c->beta;
+        for (int r0 = rw.row0(); r0 < n; r0 += stride * kWalkRows) {
+            size_t at[kWalkRows];
+            double zi[kWalkRows], pi[kWalkRows];
+#pragma unroll
+            for (int u = 0; u < kWalkRows; ++u) {
+                at[u] = size_t(min(r0 + u * stride, n - 1)) * size_t(k) + size_t(j);
+                zi[u] = zero ? 0.0 : double(z[at[u]]);
+                pi[u] = fresh ? 0.0 : p[at[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kWalkRows; ++u)
+                if (r0 + u * stride < n) p[at[u]] = fresh ? zi[u] : zi[u] + beta * pi[u];
+        }
     }
 }
 
@@ -471,27 +510,25 @@ __global__ __launch_bounds__(kBlock) void k_bcg_update(int n, int k, const doubl
 
 // First Jacobi sweep from a zero iterate: x = (w * inv) * b.
 template <class T>
-__global__ void k_bjacobi_zero(int n, int k, const T* __restrict__ inv, T w, const T* __restrict__ b,
+__global__ void k_bjacobi_zero(RowWalk rw, const T* __restrict__ inv, T w, const T* __restrict__ b,
                                T* __restrict__ x) {
 #pragma clang fp contract(off)
-    const size_t nk = size_t(n) * size_t(k);
-    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
-        const T wd = w * inv[i / size_t(k)];
+    rw.each([&](int row, int, size_t i) {
+        const T wd = w * inv[row];
         x[i] = wd * b[i];
-    }
+    });
 }
 
 // coarse[c][j] = sum over the members m of aggregate c (ascending) of f[m][j];
 // sweep: also the coarse level's first Jacobi sweep from zero,
 // x[c][j] = (w * inv[c]) * coarse[c][j] (k_bjacobi_zero's arithmetic).
 template <class T>
-__global__ void k_brestrict(int nc, int k, const int* __restrict__ off, const int* __restrict__ mem,
+__global__ void k_brestrict(RowWalk rw, const int* __restrict__ off, const int* __restrict__ mem,
                             const T* __restrict__ f, T* __restrict__ coarse, const T* __restrict__ inv, T w,
                             T* __restrict__ x, int sweep) {
 #pragma clang fp contract(off)
-    const size_t nk = size_t(nc) * size_t(k);
-    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
-        const int c = int(i / size_t(k)), j = int(i % size_t(k));
+    const int k = rw.k;
+    rw.each([&](int c, int j, size_t i) {
         const int q0 = off[c], q1 = off[c + 1];
         T s = 0;
         if (q1 > q0) {
@@ -511,34 +548,62 @@ __global__ void k_brestrict(int nc, int k, const int* __restrict__ off, const in
             const T wd = w * inv[c];
             x[i] = wd * s;
         }
-    }
+    });
 }
 
 // Cycle entry: b = R in level precision; sweep: also the first Jacobi sweep
 // from zero, x = (w * inv) * b (k_bjacobi_zero's arithmetic).
 template <class T>
-__global__ void k_benter(size_t nk, int k, const double* __restrict__ R, const T* __restrict__ inv, T w,
+__global__ void k_benter(RowWalk rw, const double* __restrict__ R, const T* __restrict__ inv, T w,
                          T* __restrict__ b, T* __restrict__ x, int sweep) {
 #pragma clang fp contract(off)
-    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
-        const T bi = T(R[i]);
-        b[i] = bi;
-        if (sweep) {
-            const T wd = w * inv[i / size_t(k)];
-            x[i] = wd * bi;
+    const int n = rw.n, k = rw.k, stride = rw.stride();
+    for (int j = rw.lane(); j < k; j += rw.step())
+        for (int r0 = rw.row0(); r0 < n; r0 += stride * kWalkRows) {
+            size_t at[kWalkRows];
+            T bi[kWalkRows], iv[kWalkRows];
+#pragma unroll
+            for (int u = 0; u < kWalkRows; ++u) {
+                const int row = min(r0 + u * stride, n - 1);
+                at[u] = size_t(row) * size_t(k) + size_t(j);
+                bi[u] = T(R[at[u]]);
+                iv[u] = sweep ? inv[row] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < kWalkRows; ++u)
+                if (r0 + u * stride < n) {
+                    b[at[u]] = bi[u];
+                    if (sweep) {
+                        const T wd = w * iv[u];
+                        x[at[u]] = wd * bi[u];
+                    }
+                }
         }
-    }
 }
 
 // fine += P coarse, or fine = P coarse when the fine iterate is known zero.
 template <class T>
-__global__ void k_bprolong(int n, int k, const int* __restrict__ map, const T* __restrict__ coarse,
+__global__ void k_bprolong(RowWalk rw, const int* __restrict__ map, const T* __restrict__ coarse,
                            T* __restrict__ fine, int set) {
-    const size_t nk = size_t(n) * size_t(k);
-    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nk; i += size_t(gridDim.x) * kBlock) {
-        const T c = coarse[size_t(map[i / size_t(k)]) * size_t(k) + i % size_t(k)];
-        fine[i] = set ? c : fine[i] + c;
-    }
+    const int n = rw.n, k = rw.k, stride = rw.stride();
+    for (int j = rw.lane(); j < k; j += rw.step())
+        for (int r0 = rw.row0(); r0 < n; r0 += stride * kWalkRows) {
+            size_t at[kWalkRows];
+            int m[kWalkRows];
+            T cv[kWalkRows], fv[kWalkRows];
+#pragma unroll
+            for (int u = 0; u < kWalkRows; ++u) {
+                const int row = min(r0 + u * stride, n - 1);
+                at[u] = size_t(row) * size_t(k) + size_t(j);
+                m[u] = map[row];
+                fv[u] = set ? T(0) : fine[at[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kWalkRows; ++u) cv[u] = coarse[size_t(m[u]) * size_t(k) + size_t(j)];
+#pragma unroll
+            for (int u = 0; u < kWalkRows; ++u)
+                if (r0 + u * stride < n) fine[at[u]] = set ? cv[u] : fv[u] + cv[u];
+        }
 }
 
 template <class S, class D>
@@ -575,6 +640,17 @@ __global__ __launch_bounds__(kBlock) void k_bcoarse(int n, int k, const double* 
 
 unsigned flat_blocks(size_t count) {
     return unsigned(std::max<size_t>(1, std::min<size_t>((count + kBlock - 1) / kBlock, 8192)));
+}
+
+// Row walk over n rows of k columns and its grid (at most 8192 blocks).
+RowWalk row_walk(int n, int k) {
+    int lg = 0;
+    while ((1 << lg) < lanes_for(k)) ++lg;
+    return RowWalk{n, k, lg};
+}
+unsigned walk_blocks(int n, int k) {
+    const int R = kBlock / lanes_for(k);
+    return unsigned(std::max(1, std::min((n + R - 1) / R, 8192)));
 }
 
 // G: lanes per row = columns per pass (0: lanes_for(k)).
@@ -820,9 +896,9 @@ void BlockAmg::smooth(size_t d, int sweeps) {
     const L w = L(amg_->opt_.omega);
     for (int s = 0; s < sweeps; ++s) {
         if (l.zero) {
-            const size_t nk = size_t(A.n) * size_t(k_);
-            hipLaunchKernelGGL(k_bjacobi_zero<L>, dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, A.n, k_,
-                               static_cast<const L*>(a.inv), w, static_cast<const L*>(l.b), static_cast<L*>(l.x));
+            hipLaunchKernelGGL(k_bjacobi_zero<L>, dim3(walk_blocks(A.n, k_)), dim3(kBlock), 0, c_->stream,
+                               row_walk(A.n, k_), static_cast<const L*>(a.inv), w, static_cast<const L*>(l.b),
+                               static_cast<L*>(l.x));
             check(hipGetLastError(), "block Jacobi launch");
             l.zero = false;
         }
@@ -847,10 +923,9 @@ void BlockAmg::restrict_to(size_t d, bool ofResidual, bool sweepNext) {
                                     nullptr, L(0), nullptr, nullptr, lanes_[d]);
         f = static_cast<const L*>(l.r);
     }
-    const size_t nk = size_t(t.nc) * size_t(k_);
     Lvl& lc = lv_[d + 1];
-    hipLaunchKernelGGL(k_brestrict<L>, dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, t.nc, k_, t.offsets,
-                       t.members, f, static_cast<L*>(lc.b), static_cast<const L*>(amg_->levels_[d + 1].inv),
+    hipLaunchKernelGGL(k_brestrict<L>, dim3(walk_blocks(t.nc, k_)), dim3(kBlock), 0, c_->stream,
+                       row_walk(t.nc, k_), t.offsets, t.members, f, static_cast<L*>(lc.b), static_cast<const L*>(amg_->levels_[d + 1].inv),
                        L(amg_->opt_.omega), static_cast<L*>(lc.x), int(sweepNext));
     check(hipGetLastError(), "block restrict launch");
     lc.swept = sweepNext;
@@ -860,9 +935,8 @@ template <class L>
 void BlockAmg::prolong(size_t d) {
     const AmgLevel& a = amg_->levels_[d];
     Lvl& l = lv_[d];
-    const size_t nk = size_t(a.native.n) * size_t(k_);
-    hipLaunchKernelGGL(k_bprolong<L>, dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, a.native.n, k_,
-                       a.transfer.map, static_cast<const L*>(lv_[d + 1].x), static_cast<L*>(l.x), int(l.zero));
+    hipLaunchKernelGGL(k_bprolong<L>, dim3(walk_blocks(a.native.n, k_)), dim3(kBlock), 0, c_->stream,
+                       row_walk(a.native.n, k_), a.transfer.map, static_cast<const L*>(lv_[d + 1].x), static_cast<L*>(l.x), int(l.zero));
     check(hipGetLastError(), "block prolong launch");
     l.zero = false;
 }
@@ -926,13 +1000,12 @@ template <class L>
 void BlockAmg::run_typed(const double* R) {
     const AmgOptionsC& o = amg_->opt_;
     const int n = amg_->levels_[0].native.n;
-    const size_t nk = size_t(n) * size_t(k_);
     for (auto& l : lv_) {
         l.zero = false;
         l.swept = false;
     }
     const bool sweep = o.cycle != 2 && o.pre >= 1 && lv_.size() > 1;
-    hipLaunchKernelGGL((k_benter<L>), dim3(flat_blocks(nk)), dim3(kBlock), 0, c_->stream, nk, k_, R,
+    hipLaunchKernelGGL((k_benter<L>), dim3(walk_blocks(n, k_)), dim3(kBlock), 0, c_->stream, row_walk(n, k_), R,
                        static_cast<const L*>(amg_->levels_[0].inv), L(o.omega), static_cast<L*>(lv_[0].b),
                        static_cast<L*>(lv_[0].x), int(sweep));
     check(hipGetLastError(), "block enter launch");
@@ -1032,17 +1105,18 @@ void SolverEngine::launch_block_iteration(int k) {
     int bits = 64;
     bool zzero = false;
     if (amg_) z = amg_->block().apply_in_place(k, bR, bits, zzero);
-    const unsigned fb = flat_blocks(nk);
+    const RowWalk rw = row_walk(n_, k);
+    const unsigned wb = walk_blocks(n_, k);
     if (bits == 32) {
         bdot(*c_, n_, k, bR, static_cast<const float*>(z), int(zzero), bparts_);
         bfinal(*c_, k, bparts_, np, ctl, kFinRhoCg, nullptr);
-        hipLaunchKernelGGL(k_bcg_p<float>, dim3(fb), dim3(kBlock), 0, c_->stream, nk, k, static_cast<const float*>(z),
+        hipLaunchKernelGGL(k_bcg_p<float>, dim3(wb), dim3(kBlock), 0, c_->stream, rw, static_cast<const float*>(z),
                            int(zzero), bP, ctl);
     }
     else {
         bdot(*c_, n_, k, bR, static_cast<const double*>(z), int(zzero), bparts_);
         bfinal(*c_, k, bparts_, np, ctl, kFinRhoCg, nullptr);
-        hipLaunchKernelGGL(k_bcg_p<double>, dim3(fb), dim3(kBlock), 0, c_->stream, nk, k,
+        hipLaunchKernelGGL(k_bcg_p<double>, dim3(wb), dim3(kBlock), 0, c_->stream, rw,
                            static_cast<const double*>(z), int(zzero), bP, ctl);
     }
     bspmm<double, kSpmv, kBDotX>(*c_, n_, k, A_.rp, A_.ci, A_.v, bP, bQ, nullptr, nullptr, 0.0, bparts_, ctl,
