@@ -296,10 +296,15 @@ struct GridBarrier {
             const unsigned old =
                 __hip_atomic_fetch_add(words + kBarStride * (1 + g), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (old + 1 == k * members) {
+                // the group's last block passes on what its members released
+                // (acquire their arrivals, release them with its own)
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
                 const unsigned t = __hip_atomic_fetch_add(words, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (t + 1 == k * groups)
+                if (t + 1 == k * groups) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
                     for (unsigned q = 0; q < groups; ++q)
                         __hip_atomic_store(rel + kBarStride * q, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
             while (__hip_atomic_load(rel + kBarStride * g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k)
                 __builtin_amdgcn_s_sleep(1);
