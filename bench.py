@@ -93,7 +93,8 @@ def main():
     A = DeviceCsr(rp, ci, v)
     db = DeviceArray.from_host(b)
     dx = DeviceArray(n, np.float64)
-    zeros = np.zeros(n)
+    # the initial guess x0 = 0, resident in HBM like the other inputs
+    dzero = DeviceArray.from_host(np.zeros(n))
 
     def make(precision):
         amg = AmgOptions(coarse_rows=256, pre_sweeps=2, post_sweeps=2, jacobi_weight=2.0 / 3.0,
@@ -101,7 +102,7 @@ def main():
         return Cg(KrylovOptions(relative_tolerance=1e-8, use_graph=not args.no_graph), amg)
 
     def step(solver):
-        dx.upload(zeros)
+        dx.copy_from(dzero)
         solver.setup_device(n, nnz, *A.args())
         return solver.apply_device(db, dx)
 

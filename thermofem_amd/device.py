@@ -73,6 +73,12 @@ class DeviceArray:
         if self.n:
             check(lib().tf_memcpy(self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes))
 
+    def copy_from(self, other):
+        """Device-to-device copy of another array of the same size and dtype."""
+        assert other.n == self.n and other.dtype == self.dtype
+        if self.n:
+            check(lib().tf_memcpy(self.ptr, other.ptr, self.nbytes))
+
     def get(self):
         out = np.empty(self.n, self.dtype)
         if self.n:
