@@ -153,28 +153,44 @@ __global__ __launch_bounds__(kBlock) void k_scg_update(const STile* __restrict__
     }
 }
 
-// Per segment: its tiles' partials added in tile order (lane-strided, then
-// the wave reduction), then the recurrence (mode >= 0) or the plain sum
-// written to out (mode < 0).
-__global__ __launch_bounds__(1024) void k_sfinal(int segments, const int* __restrict__ tileOff,
-                                                 const double* __restrict__ partials, KCtl* ctl, int mode,
-                                                 double* __restrict__ out) {
-    const int lane = int(threadIdx.x) & (kWave - 1), w = int(threadIdx.x) / kWave;
-    for (int s = w; s < segments; s += 1024 / kWave) {
-        if (mode >= 0) {
-            const KCtl* c = ctl + s;
-            if (stopped(c)) continue;
-            if (mode == kFinTrue && (!c->check || c->early)) continue;
-        }
-        double sum = 0;
-        for (int i = tileOff[s] + lane; i < tileOff[s + 1]; i += kWave) sum += partials[i];
-        sum = wave_sum(sum);
-        if (lane == 0) {
-            if (mode >= 0)
-                finalize(mode, sum, ctl + s);
-            else
-                out[s] = sum;
-        }
+// A segment's tile partials [lo, hi), lane-strided: lane L adds entries
+// lo + L, lo + L + 64, ... in order (the caller's wave_sum then combines the
+// lanes); the loads go out 8 at a time before their adds.
+__device__ __forceinline__ double lane_strided_sum(const double* __restrict__ p, int lo, int hi) {
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    double sum = 0;
+    for (int i0 = lo + lane; i0 < hi; i0 += 8 * kWave) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[min(i0 + u * kWave, hi - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + u * kWave < hi) sum += v[u];
+    }
+    return sum;
+}
+
+// Per segment, one single-wave block: its tiles' partials added in tile
+// order (lane_strided_sum, then the wave reduction), then the recurrence
+// (mode >= 0) or the plain sum written to out (mode < 0).  (One 1024-thread
+// block walking the segments took ~22 us a launch with 32 segments.)
+__global__ __launch_bounds__(kWave) void k_sfinal(const int* __restrict__ tileOff,
+                                                  const double* __restrict__ partials, KCtl* ctl, int mode,
+                                                  double* __restrict__ out) {
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    const int s = int(blockIdx.x);
+    if (mode >= 0) {
+        const KCtl* c = ctl + s;
+        if (stopped(c)) return;
+        if (mode == kFinTrue && (!c->check || c->early)) return;
+    }
+    double sum = lane_strided_sum(partials, tileOff[s], tileOff[s + 1]);
+    sum = wave_sum(sum);
+    if (lane == 0) {
+        if (mode >= 0)
+            finalize(mode, sum, ctl + s);
+        else
+            out[s] = sum;
     }
 }
 
@@ -348,9 +364,8 @@ __device__ void p_init_segments(const PSolve& P) {
     const int lane = int(threadIdx.x) & (kWave - 1);
     const int waves = int(gridDim.x) * (kBlock / kWave);
     for (int s = int(blockIdx.x) * (kBlock / kWave) + int(threadIdx.x) / kWave; s < P.segments; s += waves) {
-        double bb = 0, rr = 0;
-        for (int i = P.tileOff[s] + lane; i < P.tileOff[s + 1]; i += kWave) bb += P.parts2[i];
-        for (int i = P.tileOff[s] + lane; i < P.tileOff[s + 1]; i += kWave) rr += P.parts[i];
+        double bb = lane_strided_sum(P.parts2, P.tileOff[s], P.tileOff[s + 1]);
+        double rr = lane_strided_sum(P.parts, P.tileOff[s], P.tileOff[s + 1]);
         bb = wave_sum(bb);
         rr = wave_sum(rr);
         if (lane == 0) {
@@ -394,8 +409,7 @@ __device__ void p_finalize(const PSolve& P, int mode, int* checks) {
             if (stopped(c)) continue;
             if (mode == kFinTrue && (!c->check || c->early)) continue;
         }
-        double sum = 0;
-        for (int i = P.tileOff[s] + lane; i < P.tileOff[s + 1]; i += kWave) sum += P.parts[i];
+        double sum = lane_strided_sum(P.parts, P.tileOff[s], P.tileOff[s + 1]);
         sum = wave_sum(sum);
         if (lane == 0) {
             if (mode >= 0) {
@@ -774,8 +788,8 @@ void SolverEngine::launch_segment_iteration() {
     double *sB = S.v[0], *sX = S.v[1], *sR = S.v[2], *sZ = S.v[3], *sP = S.v[4], *sQ = S.v[5], *sW = S.v[6];
     const KCtl* cctl = S.ctl;
     auto fin = [&](int mode) {
-        hipLaunchKernelGGL(k_sfinal, dim3(1), dim3(1024), 0, c_->stream, segments, S.dTileOff, S.parts, S.ctl, mode,
-                           static_cast<double*>(nullptr));
+        hipLaunchKernelGGL(k_sfinal, dim3(unsigned(segments)), dim3(kWave), 0, c_->stream, S.dTileOff, S.parts,
+                           S.ctl, mode, static_cast<double*>(nullptr));
     };
     hipLaunchKernelGGL(k_smask, grid, dim3(kBlock), 0, c_->stream, S.dTiles, T, sR, sW, cctl);
     precondition(sW, sZ);
@@ -899,8 +913,8 @@ void SolverEngine::apply_segmented(int segments, const int* off, const double* b
     convert<double, double>(*c_, size_t(n_), x, sX);
 
     auto sums = [&](double* dst) {
-        hipLaunchKernelGGL(k_sfinal, dim3(1), dim3(1024), 0, c_->stream, segments, S.dTileOff, S.parts, S.ctl, -1,
-                           S.sums);
+        hipLaunchKernelGGL(k_sfinal, dim3(unsigned(segments)), dim3(kWave), 0, c_->stream, S.dTileOff, S.parts,
+                           S.ctl, -1, S.sums);
         check(hipMemcpyAsync(S.hsums, S.sums, sizeof(double) * size_t(segments), hipMemcpyDeviceToHost, c_->stream),
               "segment sums download");
         c_->sync();
